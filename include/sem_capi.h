@@ -1,0 +1,155 @@
+/*
+ * sem_capi.h -- C ABI of the MI355X SEM forward/adjoint library (libmun_sem.so).
+ *
+ * 2-D acoustic spectral-element solver on a structured mesh of order-p GLL
+ * elements (p = 4), the one Bspline_Coupling/SEM/sem_waveform/core.py
+ * implements on the CPU.  The reference is pure Python and has no C ABI; each
+ * entry point below states which reference call it replaces.
+ *
+ * Conventions
+ *   - Plain C types only.  Every function returns an int status: SEM_OK (0) or
+ *     a negative SEM_E* code; sem_last_error() gives a thread-local message.
+ *   - Bulk arrays are DEVICE pointers owned by the caller (e.g. torch tensor
+ *     data_ptr()).  The plan owns only its constant tables.  Scratch memory is
+ *     a caller-provided workspace sized by sem_workspace_bytes().
+ *   - State precision is fixed per plan (SEM_FP32 or SEM_FP64); arrays typed
+ *     "real" below are float for SEM_FP32 plans and double for SEM_FP64 plans.
+ *     Gradients and misfits are always double.
+ *   - Work is enqueued asynchronously on the given stream (NULL = default).
+ *     A plan may be used by one stream at a time; distinct plans are
+ *     independent.  Receiver sets must be added before the first solve.
+ *   - Node numbering is lexicographic on the (NZ x NX) GLL grid, x fastest:
+ *     node = jz * NX + ix, NX = nelem_x*p + 1, NZ = nelem_z*p + 1.
+ */
+#ifndef MUN_SEM_CAPI_H
+#define MUN_SEM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEM_OK 0
+#define SEM_E_INVALID (-1)     /* bad argument / descriptor                     */
+#define SEM_E_UNSUPPORTED (-2) /* e.g. polynomial order without a kernel        */
+#define SEM_E_NOMEM (-3)       /* workspace too small / allocation failed       */
+#define SEM_E_DEVICE (-4)      /* HIP runtime error                             */
+#define SEM_E_NONFINITE (-5)   /* NaN/Inf in an input model                     */
+
+#define SEM_FP32 0
+#define SEM_FP64 1
+
+/* adjoint recurrence: the reference's (sponge ignored in the adjoint,
+ * core.py:1252-1263) or the exact discrete adjoint of the damped forward. */
+#define SEM_ADJ_REFERENCE 0
+#define SEM_ADJ_EXACT 1
+
+/* seismogram sampling: R u^{n+1} after the update (run_forward_and_adjoint,
+ * core.py:1211,1218) or R u^n before it (run(), core.py:633-639). */
+#define SEM_RECORD_POST 0
+#define SEM_RECORD_PRE 1
+
+/* absorbing (sponge) sides, bitmask; the reference damps all four
+ * (boundary.py:38-48). */
+#define SEM_SIDE_XMIN 1u
+#define SEM_SIDE_XMAX 2u
+#define SEM_SIDE_ZMIN 4u
+#define SEM_SIDE_ZMAX 8u
+
+typedef struct sem_plan sem_plan;
+
+typedef struct {
+    int nelem_x, nelem_z;      /* elements per axis                                */
+    int order;                 /* polynomial order p (kernels: 4)                  */
+    double xmin, xmax, zmin, zmax;
+    double dt;                 /* time step [s], used as given (no CFL shrink)    */
+    int nt;                    /* number of time steps                             */
+    double sponge_width;       /* [m]   (config method.pml_thickness)              */
+    double sponge_cmax;        /* [m/s] c_max in d0 (boundary.py:30)               */
+    double sponge_R;           /* reflection target R (boundary.py:5, 1e-8)        */
+    unsigned sponge_sides;     /* SEM_SIDE_* mask                                  */
+    int precision;             /* SEM_FP32 | SEM_FP64                              */
+    int adjoint_mode;          /* SEM_ADJ_REFERENCE | SEM_ADJ_EXACT                */
+    int record_mode;           /* SEM_RECORD_POST | SEM_RECORD_PRE                 */
+    int checkpoint_every;      /* steps per wavefield checkpoint; <=0 or >=nt:
+                                  keep the full forward history instead           */
+    double lagrange_eps;       /* +eps in Lagrange denominators (utils.py:36), 1e-12 */
+} sem_desc;
+
+/* One batch of independent solves on a plan's mesh.  Solve b uses model
+ * model_of[b] (or b), source node src_node[b], wavelet row b*wavelet_stride and
+ * receiver set rec_set[b].  Seismograms / observations are [batch][nt][rec_pitch]
+ * with rec_pitch >= the largest receiver count of the sets used. */
+typedef struct {
+    int batch;
+    const void* c2;            /* device real [nmodel][N]  squared velocity        */
+    const int* model_of;       /* device int [batch] or NULL (identity)            */
+    const void* wavelet;       /* device real [.][nt]  source time function        */
+    long wavelet_stride;       /* elements between solves' wavelets (0 = shared)   */
+    const int* src_node;       /* device int [batch]  injection node ids           */
+    const int* rec_set;        /* HOST int [batch]  receiver set id per solve      */
+    int rec_pitch;             /* row length of seis/obs                           */
+    void* seis;                /* device real [batch][nt][rec_pitch]  output       */
+} sem_batch;
+
+/* Replaces SEMSimulation.__init__'s mesh/operator/sponge set-up
+ * (core.py:151-325, mesh.py:4-43, operators.py:6-42, boundary.py:5-58). */
+int sem_plan_create(const sem_desc* desc, sem_plan** out);
+int sem_plan_destroy(sem_plan* plan);
+/* NX, NZ, N of the plan's grid. */
+int sem_plan_dims(const sem_plan* plan, int* NX, int* NZ, long* N);
+/* 1-D node coordinates (host double [NX], [NZ]). */
+int sem_plan_axes(const sem_plan* plan, double* x1d, double* z1d);
+
+/* Register receivers at arbitrary positions (host double [nrec][2], x then z);
+ * replaces setup_receivers / _setup_receivers_from_positions
+ * (receivers.py:6-80, CO2_plume/2D_model/inversion/sem_waveform/core.py:90-172). */
+int sem_receiver_set_add(sem_plan* plan, int nrec, const double* xz, int* set_id);
+int sem_receiver_set_size(const sem_plan* plan, int set_id, int* nrec);
+
+/* Nearest GLL node to (x, z); replaces find_closest_node (mesh.py:45-54). */
+int sem_source_node(const sem_plan* plan, double x, double z, int* node);
+
+/* Workspace bytes for sem_forward (with_adjoint = 0) or sem_forward_adjoint
+ * (with_adjoint = 1) at the given batch size and receiver pitch. */
+int sem_workspace_bytes(const sem_plan* plan, int batch, int rec_pitch, int with_adjoint, size_t* bytes);
+
+/* Forward solve of a batch; replaces run() / _time_stepping_loop_torch
+ * (core.py:334-435, 598-659).  Writes b->seis. */
+int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_bytes, void* stream);
+
+/* Forward + misfit + adjoint + gradient; replaces the node-level part of
+ * run_forward_and_adjoint (core.py:1067-1278) including
+ * accumulate_sensitivity_dJdc_torch (core.py:49-85).
+ *   obs      device real [batch][nt][rec_pitch]   observed data y_obs
+ *   misfit   device double [batch]               0.5 * ||d_syn - y_obs||^2
+ *   grad_c   device double [batch][N] or NULL     d loglik / d c at the nodes,
+ *            loglik = -misfit * inv_sigma2 (core.py:1230; inv_sigma2 =
+ *            1/(noise_std^2 + 1e-24) as core.py:1086)
+ *   grad_c2  device double [batch][N] or NULL     d loglik / d (c^2)
+ * The gradient-time integral is accumulated in double. */
+int sem_forward_adjoint(sem_plan* plan, const sem_batch* b, const void* obs, double inv_sigma2,
+                        double* misfit, double* grad_c, double* grad_c2,
+                        void* workspace, size_t ws_bytes, void* stream);
+
+/* Tensor-product cubic B-spline velocity nodes (clamped uniform knots on the
+ * domain; no reference counterpart -- nearest analogue is the gridded
+ * background of CO2_plume/one_stage/single_cubic/sem_waveform/velocity.py:212-274):
+ *   c[m] = base + Bz coef[m] Bx^T,   c2 = c^2 in the plan's precision.
+ * coef: device double [nmodel][nbz][nbx]; base: device double [N] or NULL;
+ * c: device double [nmodel][N] (may be NULL); c2: device real [nmodel][N]. */
+int sem_bspline_map(sem_plan* plan, int nmodel, int nbz, int nbx, const double* coef,
+                    const double* base, double* c, void* c2, void* stream);
+/* Chain rule onto the nodes: g_coef[m] = Bz^T g_c[m] Bx. */
+int sem_bspline_project(sem_plan* plan, int nmodel, int nbz, int nbx, const double* g_c,
+                        double* g_coef, void* stream);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* sem_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUN_SEM_CAPI_H */

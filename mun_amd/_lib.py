@@ -1,0 +1,83 @@
+"""ctypes binding of libmun_sem.so (the C ABI in include/sem_capi.h).
+
+The library is the only compute path: importing this module on a machine
+without the built library, or calling into it without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmun_sem.so")
+
+SEM_OK = 0
+SEM_FP32, SEM_FP64 = 0, 1
+SEM_ADJ_REFERENCE, SEM_ADJ_EXACT = 0, 1
+SEM_RECORD_POST, SEM_RECORD_PRE = 0, 1
+SIDE_BITS = {"xmin": 1, "xmax": 2, "zmin": 4, "zmax": 8}
+ERROR_NAMES = {-1: "SEM_E_INVALID", -2: "SEM_E_UNSUPPORTED", -3: "SEM_E_NOMEM", -4: "SEM_E_DEVICE",
+               -5: "SEM_E_NONFINITE"}
+
+EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axes", "sem_receiver_set_add",
+           "sem_receiver_set_size", "sem_source_node", "sem_workspace_bytes", "sem_forward",
+           "sem_forward_adjoint", "sem_bspline_map", "sem_bspline_project", "sem_last_error"]
+
+
+class SemDesc(C.Structure):
+    _fields_ = [("nelem_x", C.c_int), ("nelem_z", C.c_int), ("order", C.c_int),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+                ("dt", C.c_double), ("nt", C.c_int),
+                ("sponge_width", C.c_double), ("sponge_cmax", C.c_double), ("sponge_R", C.c_double),
+                ("sponge_sides", C.c_uint), ("precision", C.c_int), ("adjoint_mode", C.c_int),
+                ("record_mode", C.c_int), ("checkpoint_every", C.c_int), ("lagrange_eps", C.c_double)]
+
+
+class SemBatch(C.Structure):
+    _fields_ = [("batch", C.c_int), ("c2", C.c_void_p), ("model_of", C.c_void_p), ("wavelet", C.c_void_p),
+                ("wavelet_stride", C.c_long), ("src_node", C.c_void_p), ("rec_set", C.POINTER(C.c_int)),
+                ("rec_pitch", C.c_int), ("seis", C.c_void_p)]
+
+
+class SemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libmun_sem.so (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mun_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i, ip, d, dp = C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_double)
+    sz = C.c_size_t
+    L.sem_last_error.restype = C.c_char_p
+    L.sem_plan_create.argtypes = [C.POINTER(SemDesc), C.POINTER(vp)]
+    L.sem_plan_destroy.argtypes = [vp]
+    L.sem_plan_dims.argtypes = [vp, ip, ip, C.POINTER(C.c_long)]
+    L.sem_plan_axes.argtypes = [vp, dp, dp]
+    L.sem_receiver_set_add.argtypes = [vp, i, dp, ip]
+    L.sem_receiver_set_size.argtypes = [vp, i, ip]
+    L.sem_source_node.argtypes = [vp, d, d, ip]
+    L.sem_workspace_bytes.argtypes = [vp, i, i, i, C.POINTER(sz)]
+    L.sem_forward.argtypes = [vp, C.POINTER(SemBatch), vp, sz, vp]
+    L.sem_forward_adjoint.argtypes = [vp, C.POINTER(SemBatch), vp, d, vp, vp, vp, vp, sz, vp]
+    L.sem_bspline_map.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]
+    L.sem_bspline_project.argtypes = [vp, i, i, i, vp, vp, vp]
+    for name in EXPORTS:
+        if name != "sem_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != SEM_OK:
+        raise SemError(rc, lib().sem_last_error().decode())

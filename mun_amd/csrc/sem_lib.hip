@@ -1,0 +1,825 @@
+// Host side of libmun_sem.so: plan set-up (FP64 tables), receiver sets,
+// workspace layout and the forward / forward+adjoint orchestration.
+// The C ABI is declared in include/sem_capi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sem_capi.h"
+#include "sem_kernels.h"
+
+using namespace sem;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(SEM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SEM_E_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------- 1-D GLL algebra
+// P_n(x), P_n'(x), P_n''(x) by the three-term recurrence (interior points).
+void legendre(int n, double x, double& p, double& dp, double& d2p) {
+    double p0 = 1.0, p1 = x;
+    if (n == 0) {
+        p = 1.0;
+        dp = d2p = 0.0;
+        return;
+    }
+    for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    p = p1;
+    dp = n * (p0 - x * p1) / (1.0 - x * x);
+    d2p = (2.0 * x * dp - n * (n + 1.0) * p) / (1.0 - x * x);
+}
+
+// GLL nodes: +-1 and the roots of P_p', Newton from Chebyshev-Lobatto guesses
+// (the scheme of Bspline_Coupling/SEM/sem_waveform/utils.py:5-21).
+std::vector<double> gll_nodes(int p) {
+    std::vector<double> x(p + 1);
+    for (int k = 0; k <= p; ++k) x[k] = -std::cos(M_PI * k / p);
+    for (int k = 1; k < p; ++k) {
+        double xi = x[k];
+        for (int it = 0; it < 100; ++it) {
+            double P, dP, d2P;
+            legendre(p, xi, P, dP, d2P);
+            double step = dP / d2P;
+            xi -= step;
+            if (std::fabs(step) < 1e-16) break;
+        }
+        x[k] = xi;
+    }
+    x[0] = -1.0;
+    x[p] = 1.0;
+    return x;
+}
+
+std::vector<double> gll_weights(const std::vector<double>& x) {
+    int p = (int)x.size() - 1;
+    std::vector<double> w(p + 1);
+    for (int i = 0; i <= p; ++i) {
+        double P0 = 1.0, P1 = x[i];
+        for (int k = 2; k <= p; ++k) {
+            double P2 = ((2.0 * k - 1.0) * x[i] * P1 - (k - 1.0) * P0) / k;
+            P0 = P1;
+            P1 = P2;
+        }
+        w[i] = 2.0 / (p * (p + 1.0) * P1 * P1);
+    }
+    return w;
+}
+
+// D[i][j] = l_j'(x_i) from barycentric weights, negative-sum diagonal
+// (utils.py:39-61).
+std::vector<double> diff_matrix(const std::vector<double>& x) {
+    int n = (int)x.size();
+    std::vector<double> bw(n, 1.0), D(n * n, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k)
+            if (k != j) bw[j] /= (x[j] - x[k]);
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (i != j) {
+                D[i * n + j] = bw[j] / (bw[i] * (x[i] - x[j]));
+                s += D[i * n + j];
+            }
+        D[i * n + i] = -s;
+    }
+    return D;
+}
+
+// Lagrange polynomial with +eps in each denominator (utils.py:31-37).
+double lagrange(double x, int i, const std::vector<double>& nodes, double eps) {
+    double r = 1.0;
+    for (int j = 0; j < (int)nodes.size(); ++j)
+        if (j != i) r *= (x - nodes[j]) / (nodes[i] - nodes[j] + eps);
+    return r;
+}
+
+// Clamped uniform cubic B-spline on [a, b] with n control values: for each
+// coordinate, the first non-zero basis index and up to 4 values.
+void bspline_tables(const std::vector<double>& coord, double a, double b, int n, std::vector<int>& first,
+                    std::vector<double>& vals) {
+    const int k = 3;
+    std::vector<double> t;
+    for (int i = 0; i < k; ++i) t.push_back(a);
+    for (int i = 0; i <= n - k; ++i) t.push_back(a + (b - a) * i / (n - k));
+    for (int i = 0; i < k; ++i) t.push_back(b);
+    const int m = (int)coord.size();
+    first.assign(m, 0);
+    vals.assign(4 * m, 0.0);
+    for (int c = 0; c < m; ++c) {
+        double x = std::min(std::max(coord[c], a), b);
+        // knot span s with t[s] <= x < t[s+1], right end in the last span
+        int s = k;
+        while (s < n - 1 && x >= t[s + 1]) ++s;
+        // Cox-de Boor triangle for degree k at span s
+        double N[4] = {1.0, 0.0, 0.0, 0.0}, left[4], right[4];
+        for (int d = 1; d <= k; ++d) {
+            left[d] = x - t[s + 1 - d];
+            right[d] = t[s + d] - x;
+            double saved = 0.0;
+            for (int r = 0; r < d; ++r) {
+                double tmp = N[r] / (right[r + 1] + left[d - r]);
+                N[r] = saved + right[r + 1] * tmp;
+                saved = left[d - r] * tmp;
+            }
+            N[d] = saved;
+        }
+        first[c] = s - k;
+        for (int r = 0; r < 4; ++r) vals[4 * c + r] = N[r];
+    }
+}
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int upload(const void* src, size_t bytes) {
+        if (bytes > n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+            hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+            if (e != hipSuccess) return fail(SEM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            n = bytes;
+        }
+        if (bytes) {
+            hipError_t e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(SEM_E_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        }
+        return SEM_OK;
+    }
+};
+
+struct RecHost {
+    int ex, ez, r, owned;
+    double lx[5], lz[5];
+};
+
+struct BsplineTables {
+    DevBuf bz0, bzw, bx0, bxw, plo, phi, qlo, qhi;
+};
+
+}  // namespace
+
+struct sem_plan {
+    sem_desc d;
+    int P = 4, NX = 0, NZ = 0;
+    long N = 0, Npad = 0;
+    double dx = 0, dz = 0, J = 0;
+    int ntx = 0, ntz = 0, ntiles = 0;
+    std::vector<double> xi, wq, Dm, x1d, z1d;
+    std::vector<double> invWx, dtWz, Dxv, Dzv;
+    DevBuf d_invWx, d_dtWz, d_Dx, d_Dz;
+    // receiver sets
+    std::vector<std::vector<RecHost>> sets;  // per set, entries across tiles (ordered by tile)
+    std::vector<int> set_size;
+    DevBuf d_recs, d_rec_ptr;
+    bool rec_dirty = true;
+    std::map<std::pair<int, int>, BsplineTables*> bsp;
+    ~sem_plan() {
+        for (auto& kv : bsp) delete kv.second;
+    }
+};
+
+namespace {
+
+template <typename T>
+int upload_vec(DevBuf& buf, const std::vector<double>& v) {
+    std::vector<T> tmp(v.begin(), v.end());
+    return buf.upload(tmp.data(), tmp.size() * sizeof(T));
+}
+
+template <typename T>
+int upload_tables(sem_plan* pl) {
+    int rc;
+    if ((rc = upload_vec<T>(pl->d_invWx, pl->invWx))) return rc;
+    if ((rc = upload_vec<T>(pl->d_dtWz, pl->dtWz))) return rc;
+    if ((rc = upload_vec<T>(pl->d_Dx, pl->Dxv))) return rc;
+    if ((rc = upload_vec<T>(pl->d_Dz, pl->Dzv))) return rc;
+    return SEM_OK;
+}
+
+template <typename T>
+int upload_receivers(sem_plan* pl) {
+    std::vector<RecEntry<T, 4>> flat;
+    std::vector<int> ptr;
+    for (auto& s : pl->sets) {
+        // entries of a set are stored grouped by tile: rebuild the tile CSR
+        std::vector<std::vector<const RecHost*>> by_tile(pl->ntiles);
+        for (auto& e : s) {
+            // tiles whose extended element block [tx*TX-1, tx*TX+TX-1] holds e
+            for (int tz = std::max(0, e.ez / TZ - 1); tz <= std::min(pl->ntz - 1, e.ez / TZ + 1); ++tz)
+                for (int tx = std::max(0, e.ex / TX - 1); tx <= std::min(pl->ntx - 1, e.ex / TX + 1); ++tx) {
+                    if (e.ex < tx * TX - 1 || e.ex > tx * TX + TX - 1) continue;
+                    if (e.ez < tz * TZ - 1 || e.ez > tz * TZ + TZ - 1) continue;
+                    by_tile[tz * pl->ntx + tx].push_back(&e);
+                }
+        }
+        for (int t = 0; t < pl->ntiles; ++t) {
+            ptr.push_back((int)flat.size());
+            const int tx = t % pl->ntx, tz = t / pl->ntx;
+            for (const RecHost* e : by_tile[t]) {
+                RecEntry<T, 4> r;
+                r.ex = e->ex;
+                r.ez = e->ez;
+                r.r = e->r;
+                r.owned = (e->ex / TX == tx && e->ez / TZ == tz) ? 1 : 0;
+                for (int q = 0; q < 5; ++q) {
+                    r.lx[q] = (T)e->lx[q];
+                    r.lz[q] = (T)e->lz[q];
+                }
+                flat.push_back(r);
+            }
+        }
+        ptr.push_back((int)flat.size());
+    }
+    int rc;
+    if ((rc = pl->d_recs.upload(flat.data(), flat.size() * sizeof(RecEntry<T, 4>)))) return rc;
+    if ((rc = pl->d_rec_ptr.upload(ptr.data(), ptr.size() * sizeof(int)))) return rc;
+    pl->rec_dirty = false;
+    return SEM_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {
+    bool full_history = false;
+    int S = 0, nseg = 0;
+    size_t U0, U1, W, CKU, CKW, SEG, L0, L1, OM, RES, G, NREC, RSET, total;
+};
+
+Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
+    const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
+    const size_t state = (size_t)B * pl->Npad * Ts;
+    Layout L{};
+    const int nt = pl->d.nt;
+    L.full_history = adj && (pl->d.checkpoint_every <= 0 || pl->d.checkpoint_every >= nt);
+    L.S = L.full_history ? nt : pl->d.checkpoint_every;
+    L.nseg = adj ? (L.full_history ? 1 : (nt + L.S - 1) / L.S) : 0;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += align256(bytes);
+        return at;
+    };
+    L.U0 = take(state);
+    L.U1 = take(state);
+    L.W = take(state);
+    L.NREC = take((size_t)B * 4);
+    L.RSET = take((size_t)B * 4);
+    if (adj) {
+        if (!L.full_history) {
+            L.CKU = take(state * L.nseg);
+            L.CKW = take(state * L.nseg);
+            L.SEG = take(state * (size_t)(L.S + 1));
+        } else {
+            L.SEG = take(state * (size_t)(nt + 1));
+        }
+        L.L0 = take(state);
+        L.L1 = take(state);
+        L.OM = take(state);
+        L.RES = take((size_t)B * nt * pitch * Ts);
+        L.G = take((size_t)B * pl->Npad * 8);
+    }
+    L.total = o;
+    return L;
+}
+
+template <typename T>
+int fill(T* p, T v, long n, hipStream_t st) {
+    if (n <= 0) return SEM_OK;
+    int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    fill_kernel<T><<<blocks, 256, 0, st>>>(p, v, n);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+template <typename T>
+StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L) {
+    StepArgs<T, 4> a;
+    std::memset(&a, 0, sizeof(a));
+    const int P = 4;
+    for (int i = 0; i <= P; ++i) {
+        a.ec.w[i] = (T)pl->wq[i];
+        for (int k = 0; k <= P; ++k) a.ec.D[i][k] = (T)pl->Dm[i * (P + 1) + k];
+    }
+    a.ec.cx = (T)(4.0 / (pl->dx * pl->dx));
+    a.ec.cz = (T)(4.0 / (pl->dz * pl->dz));
+    a.nex = pl->d.nelem_x;
+    a.nez = pl->d.nelem_z;
+    a.NX = pl->NX;
+    a.NZ = pl->NZ;
+    a.ntx = pl->ntx;
+    a.ntiles = pl->ntiles;
+    a.c2 = (const T*)bt->c2;
+    a.c2_bs = pl->N;
+    a.model_of = bt->model_of;
+    a.invWx = (const T*)pl->d_invWx.p;
+    a.dtWz = (const T*)pl->d_dtWz.p;
+    a.Dx = (const T*)pl->d_Dx.p;
+    a.Dz = (const T*)pl->d_Dz.p;
+    a.damp = 1;
+    a.src_node = bt->src_node;
+    a.wavelet = (const T*)bt->wavelet;
+    a.wav_bs = bt->wavelet_stride;
+    a.dt2 = (T)(pl->d.dt * pl->d.dt);
+    a.rec_set_of = (const int*)(ws + L.RSET);
+    a.rec_ptr = (const int*)pl->d_rec_ptr.p;
+    a.recs = (const RecEntry<T, 4>*)pl->d_recs.p;
+    a.seis = (T*)bt->seis;
+    a.seis_bs = (long)pl->d.nt * bt->rec_pitch;
+    a.rec_pitch = bt->rec_pitch;
+    a.rec_t = -1;
+    a.invJ = (T)(1.0 / pl->J);
+    a.res_t = -1;
+    return a;
+}
+
+template <typename T, int MODE>
+int launch_step(const sem_plan* pl, int B, const StepArgs<T, 4>& a, hipStream_t st) {
+    dim3 grid(pl->ntiles, B);
+    step_kernel<T, 4, MODE><<<grid, NTHREADS, 0, st>>>(a);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout& L) {
+    if (!bt || bt->batch <= 0 || bt->batch > 65535) return fail(SEM_E_INVALID, "batch must be in [1, 65535]");
+    if (!bt->c2 || !bt->wavelet || !bt->src_node || !bt->rec_set || !bt->seis)
+        return fail(SEM_E_INVALID, "sem_batch: null pointer in c2/wavelet/src_node/rec_set/seis");
+    if (ws_bytes < L.total)
+        return fail(SEM_E_NOMEM, "workspace too small: need " + std::to_string(L.total) + " bytes");
+    for (int b = 0; b < bt->batch; ++b) {
+        int s = bt->rec_set[b];
+        if (s < 0 || s >= (int)pl->sets.size())
+            return fail(SEM_E_INVALID, "rec_set[" + std::to_string(b) + "] is not a registered receiver set");
+        if (pl->set_size[s] > bt->rec_pitch) return fail(SEM_E_INVALID, "rec_pitch smaller than a receiver set");
+    }
+    return SEM_OK;
+}
+
+template <typename T>
+int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hipStream_t st) {
+    if (pl->rec_dirty) {
+        int rc = upload_receivers<T>(pl);
+        if (rc) return rc;
+    }
+    std::vector<int> nrec(bt->batch);
+    for (int b = 0; b < bt->batch; ++b) nrec[b] = pl->set_size[bt->rec_set[b]];
+    HIP_TRY(hipMemcpyAsync(ws + L.RSET, bt->rec_set, sizeof(int) * bt->batch, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ws + L.NREC, nrec.data(), sizeof(int) * bt->batch, hipMemcpyHostToDevice, st));
+    // the pageable source buffers above must stay valid until the copies land
+    HIP_TRY(hipStreamSynchronize(st));
+    return SEM_OK;
+}
+
+// Forward time loop.  Checkpoint mode: ping-pong U0/U1 with (u, w) mirrored
+// into checkpoint k after step k*S-1.  Full-history mode: u^n in SEG slot n.
+template <typename T>
+int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, bool adj, hipStream_t st) {
+    const int B = bt->batch, nt = pl->d.nt;
+    const long sb = (long)B * pl->Npad;
+    StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);
+    T* U0 = (T*)(ws + L.U0);
+    T* U1 = (T*)(ws + L.U1);
+    T* W = (T*)(ws + L.W);
+    T* SEG = (T*)(ws + L.SEG);
+    int rc;
+    if ((rc = fill<T>(W, T(0), sb, st))) return rc;
+    if (adj && L.full_history) {
+        if ((rc = fill<T>(SEG, T(0), sb, st))) return rc;
+    } else {
+        if ((rc = fill<T>(U0, T(0), sb, st))) return rc;
+    }
+    if (adj && !L.full_history) {
+        if ((rc = fill<T>((T*)(ws + L.CKU), T(0), sb, st))) return rc;
+        if ((rc = fill<T>((T*)(ws + L.CKW), T(0), sb, st))) return rc;
+    }
+    const bool post = pl->d.record_mode == SEM_RECORD_POST;
+    a.u_in_bs = a.u_out_bs = a.w_bs = pl->Npad;
+    a.w = W;
+    a.ck_bs = pl->Npad;
+    const T* last = nullptr;
+    for (int n = 0; n < nt; ++n) {
+        if (adj && L.full_history) {
+            a.u_in = SEG + (long)n * sb;
+            a.u_out = SEG + (long)(n + 1) * sb;
+        } else {
+            a.u_in = (n & 1) ? U1 : U0;
+            a.u_out = (n & 1) ? U0 : U1;
+        }
+        a.n = n;
+        a.rec_t = post ? n - 1 : n;
+        a.ck_u = a.ck_w = nullptr;
+        if (adj && !L.full_history && (n + 1) % L.S == 0 && n + 1 < nt) {
+            const long k = (n + 1) / L.S;
+            a.ck_u = (T*)(ws + L.CKU) + k * sb;
+            a.ck_w = (T*)(ws + L.CKW) + k * sb;
+        }
+        if ((rc = launch_step<T, MODE_FWD>(pl, B, a, st))) return rc;
+        last = a.u_out;
+    }
+    if (post) {
+        a.u_in = last;
+        a.rec_t = nt - 1;
+        if ((rc = launch_step<T, MODE_REC>(pl, B, a, st))) return rc;
+    }
+    return SEM_OK;
+}
+
+template <typename T>
+int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, hipStream_t st) {
+    Layout L = layout(pl, bt->batch, bt->rec_pitch, false);
+    int rc;
+    if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
+    char* ws = (char*)wsv;
+    if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
+    if ((rc = fill<T>((T*)bt->seis, T(0), (long)bt->batch * pl->d.nt * bt->rec_pitch, st))) return rc;
+    return forward_loop<T>(pl, bt, ws, L, false, st);
+}
+
+template <typename T>
+int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, double inv_sigma2, double* misfit,
+                         double* grad_c, double* grad_c2, void* wsv, size_t ws_bytes, hipStream_t st) {
+    Layout L = layout(pl, bt->batch, bt->rec_pitch, true);
+    int rc;
+    if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
+    if (!obs || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
+    char* ws = (char*)wsv;
+    const int B = bt->batch, nt = pl->d.nt;
+    const long sb = (long)B * pl->Npad;
+    if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
+    if ((rc = fill<T>((T*)bt->seis, T(0), (long)B * nt * bt->rec_pitch, st))) return rc;
+    if ((rc = forward_loop<T>(pl, bt, ws, L, true, st))) return rc;
+
+    // residual and misfit
+    T* RES = (T*)(ws + L.RES);
+    residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs, RES, misfit,
+                                          (const int*)(ws + L.NREC), nt, bt->rec_pitch,
+                                          (long)nt * bt->rec_pitch);
+    LAUNCH_CHECK();
+
+    // adjoint sweep
+    T* L0 = (T*)(ws + L.L0);
+    T* L1 = (T*)(ws + L.L1);
+    T* OM = (T*)(ws + L.OM);
+    double* G = (double*)(ws + L.G);
+    if ((rc = fill<T>(L0, T(0), sb, st))) return rc;
+    if ((rc = fill<T>(OM, T(0), sb, st))) return rc;
+    if ((rc = fill<double>(G, 0.0, sb, st))) return rc;
+
+    StepArgs<T, 4> f = base_args<T>(pl, bt, ws, L);  // recompute steps
+    f.u_in_bs = f.u_out_bs = f.w_bs = pl->Npad;
+    StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);  // adjoint steps
+    a.u_in_bs = a.u_out_bs = a.w_bs = a.uf_bs = a.G_bs = pl->Npad;
+    a.w = OM;
+    a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
+    a.src_node = nullptr;
+    a.resid = RES;
+    a.resid_bs = (long)nt * bt->rec_pitch;
+    a.G = G;
+    T* SEG = (T*)(ws + L.SEG);
+    T* lam_in = L0;
+    T* lam_out = L1;
+    for (int k = L.nseg - 1; k >= 0; --k) {
+        const int s0 = k * L.S;
+        const int len = std::min(L.S, nt - s0);
+        const T* slot0 = SEG;  // u^{s0}
+        if (!L.full_history) {
+            // recompute u^{s0+1..s0+len} from checkpoint k into SEG[1..len]
+            slot0 = (T*)(ws + L.CKU) + (long)k * sb;
+            f.w = (T*)(ws + L.CKW) + (long)k * sb;
+            for (int m = 0; m < len; ++m) {
+                f.u_in = (m == 0) ? slot0 : SEG + (long)m * sb;
+                f.u_out = SEG + (long)(m + 1) * sb;
+                f.n = s0 + m;
+                if ((rc = launch_step<T, MODE_FWD>(pl, B, f, st))) return rc;
+            }
+        }
+        for (int it = s0 + len - 1; it >= s0; --it) {
+            a.u_in = lam_in;
+            a.u_out = lam_out;
+            a.uf = SEG + (long)(it - s0 + 1) * sb;  // u^{it+1}
+            a.res_t = it;
+            if ((rc = launch_step<T, MODE_ADJ>(pl, B, a, st))) return rc;
+            std::swap(lam_in, lam_out);
+        }
+    }
+    if (grad_c || grad_c2) {
+        dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), B);
+        grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->Npad, (const T*)bt->c2, bt->model_of, pl->N, pl->N,
+                                                      pl->J * inv_sigma2, grad_c, grad_c2);
+        LAUNCH_CHECK();
+    }
+    return SEM_OK;
+}
+
+BsplineTables* get_bspline(sem_plan* pl, int nbz, int nbx, int& rc) {
+    auto key = std::make_pair(nbz, nbx);
+    auto it = pl->bsp.find(key);
+    if (it != pl->bsp.end()) return it->second;
+    auto* t = new BsplineTables;
+    std::vector<int> bz0, bx0;
+    std::vector<double> bzw, bxw;
+    bspline_tables(pl->z1d, pl->d.zmin, pl->d.zmax, nbz, bz0, bzw);
+    bspline_tables(pl->x1d, pl->d.xmin, pl->d.xmax, nbx, bx0, bxw);
+    // support ranges [lo, hi) per basis function
+    auto ranges = [](const std::vector<int>& first, int n, std::vector<int>& lo, std::vector<int>& hi) {
+        lo.assign(n, (int)first.size());
+        hi.assign(n, 0);
+        for (int c = 0; c < (int)first.size(); ++c)
+            for (int r = 0; r < 4; ++r) {
+                int q = first[c] + r;
+                if (q < n) {
+                    lo[q] = std::min(lo[q], c);
+                    hi[q] = std::max(hi[q], c + 1);
+                }
+            }
+        for (int q = 0; q < n; ++q)
+            if (hi[q] < lo[q]) lo[q] = hi[q] = 0;
+    };
+    std::vector<int> plo, phi, qlo, qhi;
+    ranges(bz0, nbz, plo, phi);
+    ranges(bx0, nbx, qlo, qhi);
+    rc = SEM_OK;
+    if (!rc) rc = t->bz0.upload(bz0.data(), bz0.size() * 4);
+    if (!rc) rc = t->bzw.upload(bzw.data(), bzw.size() * 8);
+    if (!rc) rc = t->bx0.upload(bx0.data(), bx0.size() * 4);
+    if (!rc) rc = t->bxw.upload(bxw.data(), bxw.size() * 8);
+    if (!rc) rc = t->plo.upload(plo.data(), plo.size() * 4);
+    if (!rc) rc = t->phi.upload(phi.data(), phi.size() * 4);
+    if (!rc) rc = t->qlo.upload(qlo.data(), qlo.size() * 4);
+    if (!rc) rc = t->qhi.upload(qhi.data(), qhi.size() * 4);
+    if (rc) {
+        delete t;
+        return nullptr;
+    }
+    pl->bsp[key] = t;
+    return t;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+const char* sem_last_error(void) { return g_err.c_str(); }
+
+int sem_plan_create(const sem_desc* d, sem_plan** out) {
+    if (!d || !out) return fail(SEM_E_INVALID, "null argument");
+    *out = nullptr;
+    if (d->order != 4) return fail(SEM_E_UNSUPPORTED, "only polynomial order 4 has kernels");
+    if (d->nelem_x < 1 || d->nelem_z < 1) return fail(SEM_E_INVALID, "nelem_x/nelem_z must be >= 1");
+    if (!(d->xmax > d->xmin) || !(d->zmax > d->zmin)) return fail(SEM_E_INVALID, "empty domain");
+    if (!(d->dt > 0) || d->nt < 1) return fail(SEM_E_INVALID, "dt must be > 0 and nt >= 1");
+    if (d->precision != SEM_FP32 && d->precision != SEM_FP64) return fail(SEM_E_INVALID, "bad precision");
+    if (d->adjoint_mode != SEM_ADJ_REFERENCE && d->adjoint_mode != SEM_ADJ_EXACT)
+        return fail(SEM_E_INVALID, "bad adjoint_mode");
+    if (d->record_mode != SEM_RECORD_POST && d->record_mode != SEM_RECORD_PRE)
+        return fail(SEM_E_INVALID, "bad record_mode");
+    if (d->sponge_sides && !(d->sponge_width > 0)) return fail(SEM_E_INVALID, "sponge_width must be > 0");
+    auto* pl = new sem_plan;
+    pl->d = *d;
+    const int P = 4;
+    pl->NX = d->nelem_x * P + 1;
+    pl->NZ = d->nelem_z * P + 1;
+    pl->N = (long)pl->NX * pl->NZ;
+    pl->Npad = (pl->N + 63) / 64 * 64;
+    pl->dx = (d->xmax - d->xmin) / d->nelem_x;
+    pl->dz = (d->zmax - d->zmin) / d->nelem_z;
+    pl->J = pl->dx * pl->dz * 0.25;
+    pl->ntx = (d->nelem_x + TX - 1) / TX;
+    pl->ntz = (d->nelem_z + TZ - 1) / TZ;
+    pl->ntiles = pl->ntx * pl->ntz;
+    pl->xi = gll_nodes(P);
+    pl->wq = gll_weights(pl->xi);
+    pl->Dm = diff_matrix(pl->xi);
+    // node coordinates (mesh.py:22-30 arithmetic)
+    auto axis = [&](double a, double b, int ne, std::vector<double>& out1) {
+        out1.assign(ne * P + 1, 0.0);
+        for (int e = 0; e < ne; ++e) {
+            double x0 = a + (b - a) * e / ne, x1 = a + (b - a) * (e + 1) / ne;
+            if (e == ne - 1) x1 = b;
+            for (int i = 0; i <= P; ++i) out1[e * P + i] = 0.5 * ((1 - pl->xi[i]) * x0 + (1 + pl->xi[i]) * x1);
+        }
+    };
+    axis(d->xmin, d->xmax, d->nelem_x, pl->x1d);
+    axis(d->zmin, d->zmax, d->nelem_z, pl->z1d);
+    // assembled 1-D weights -> lumped mass M = J Wx Wz (operators.py:65-84)
+    auto wsum = [&](int ne, std::vector<double>& W) {
+        W.assign(ne * P + 1, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (int i = 0; i <= P; ++i) W[e * P + i] += pl->wq[i];
+    };
+    std::vector<double> Wx, Wz;
+    wsum(d->nelem_x, Wx);
+    wsum(d->nelem_z, Wz);
+    pl->invWx.resize(pl->NX);
+    pl->dtWz.resize(pl->NZ);
+    for (int i = 0; i < pl->NX; ++i) pl->invWx[i] = 1.0 / Wx[i];
+    for (int j = 0; j < pl->NZ; ++j) pl->dtWz[j] = d->dt * d->dt / Wz[j];
+    // sponge: cubic ramp, d0 = -(m+1) c_max ln R / (2 delta)  (boundary.py:26-52)
+    const double R = d->sponge_R > 0 ? d->sponge_R : 1e-8;
+    const double del = d->sponge_width;
+    const double d0 = del > 0 ? -4.0 * d->sponge_cmax * std::log(R) / (2.0 * del) : 0.0;
+    auto ramp = [&](double depth) { return d0 * std::pow(depth / del, 3); };
+    pl->Dxv.assign(pl->NX, 1.0);
+    pl->Dzv.assign(pl->NZ, 1.0);
+    for (int i = 0; i < pl->NX; ++i) {
+        double x = pl->x1d[i], s = 0.0;
+        if ((d->sponge_sides & SEM_SIDE_XMIN) && x < d->xmin + del) s = ramp(d->xmin + del - x);
+        if ((d->sponge_sides & SEM_SIDE_XMAX) && x > d->xmax - del) s = ramp(x - (d->xmax - del));
+        pl->Dxv[i] = std::exp(-s * d->dt);
+    }
+    for (int j = 0; j < pl->NZ; ++j) {
+        double z = pl->z1d[j], s = 0.0;
+        if ((d->sponge_sides & SEM_SIDE_ZMIN) && z < d->zmin + del) s = ramp(d->zmin + del - z);
+        if ((d->sponge_sides & SEM_SIDE_ZMAX) && z > d->zmax - del) s = ramp(z - (d->zmax - del));
+        pl->Dzv[j] = std::exp(-s * d->dt);
+    }
+    int rc = d->precision == SEM_FP64 ? upload_tables<double>(pl) : upload_tables<float>(pl);
+    if (rc) {
+        delete pl;
+        return rc;
+    }
+    *out = pl;
+    return SEM_OK;
+}
+
+int sem_plan_destroy(sem_plan* plan) {
+    delete plan;
+    return SEM_OK;
+}
+
+int sem_plan_dims(const sem_plan* pl, int* NX, int* NZ, long* N) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    if (NX) *NX = pl->NX;
+    if (NZ) *NZ = pl->NZ;
+    if (N) *N = pl->N;
+    return SEM_OK;
+}
+
+int sem_plan_axes(const sem_plan* pl, double* x1d, double* z1d) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    if (x1d) std::memcpy(x1d, pl->x1d.data(), sizeof(double) * pl->NX);
+    if (z1d) std::memcpy(z1d, pl->z1d.data(), sizeof(double) * pl->NZ);
+    return SEM_OK;
+}
+
+int sem_receiver_set_add(sem_plan* pl, int nrec, const double* xz, int* set_id) {
+    if (!pl || nrec < 0 || (nrec > 0 && !xz) || !set_id) return fail(SEM_E_INVALID, "bad receiver set");
+    std::vector<RecHost> s;
+    const auto& d = pl->d;
+    for (int r = 0; r < nrec; ++r) {
+        const double rx = xz[2 * r], rz = xz[2 * r + 1];
+        if (!std::isfinite(rx) || !std::isfinite(rz)) return fail(SEM_E_INVALID, "non-finite receiver position");
+        // element by truncation, clamped (receivers.py:32-35)
+        int ex = (int)((rx - d.xmin) / pl->dx);
+        int ez = (int)((rz - d.zmin) / pl->dz);
+        ex = std::max(0, std::min(ex, d.nelem_x - 1));
+        ez = std::max(0, std::min(ez, d.nelem_z - 1));
+        const double xi = 2.0 * (rx - (d.xmin + ex * pl->dx)) / pl->dx - 1.0;
+        const double eta = 2.0 * (rz - (d.zmin + ez * pl->dz)) / pl->dz - 1.0;
+        RecHost e;
+        e.ex = ex;
+        e.ez = ez;
+        e.r = r;
+        e.owned = 0;
+        for (int q = 0; q <= 4; ++q) {
+            e.lx[q] = lagrange(xi, q, pl->xi, d.lagrange_eps);
+            e.lz[q] = lagrange(eta, q, pl->xi, d.lagrange_eps);
+        }
+        s.push_back(e);
+    }
+    pl->sets.push_back(std::move(s));
+    pl->set_size.push_back(nrec);
+    pl->rec_dirty = true;
+    *set_id = (int)pl->sets.size() - 1;
+    return SEM_OK;
+}
+
+int sem_receiver_set_size(const sem_plan* pl, int set_id, int* nrec) {
+    if (!pl || !nrec || set_id < 0 || set_id >= (int)pl->sets.size()) return fail(SEM_E_INVALID, "bad set id");
+    *nrec = pl->set_size[set_id];
+    return SEM_OK;
+}
+
+int sem_source_node(const sem_plan* pl, double x, double z, int* node) {
+    if (!pl || !node) return fail(SEM_E_INVALID, "null argument");
+    // nearest node; on a tensor grid the squared distance separates per axis
+    auto nearest = [](const std::vector<double>& v, double q) {
+        int best = 0;
+        double bd = std::fabs(v[0] - q);
+        for (int i = 1; i < (int)v.size(); ++i) {
+            double dd = std::fabs(v[i] - q);
+            if (dd < bd) {
+                bd = dd;
+                best = i;
+            }
+        }
+        return best;
+    };
+    *node = nearest(pl->z1d, z) * pl->NX + nearest(pl->x1d, x);
+    return SEM_OK;
+}
+
+int sem_workspace_bytes(const sem_plan* pl, int batch, int rec_pitch, int with_adjoint, size_t* bytes) {
+    if (!pl || !bytes || batch < 1 || rec_pitch < 0) return fail(SEM_E_INVALID, "bad argument");
+    *bytes = layout(pl, batch, rec_pitch, with_adjoint != 0).total;
+    return SEM_OK;
+}
+
+int sem_forward(sem_plan* pl, const sem_batch* b, void* ws, size_t ws_bytes, void* stream) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? forward_impl<double>(pl, b, ws, ws_bytes, st)
+                                       : forward_impl<float>(pl, b, ws, ws_bytes, st);
+}
+
+int sem_forward_adjoint(sem_plan* pl, const sem_batch* b, const void* obs, double inv_sigma2, double* misfit,
+                        double* grad_c, double* grad_c2, void* ws, size_t ws_bytes, void* stream) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64
+               ? forward_adjoint_impl<double>(pl, b, obs, inv_sigma2, misfit, grad_c, grad_c2, ws, ws_bytes, st)
+               : forward_adjoint_impl<float>(pl, b, obs, inv_sigma2, misfit, grad_c, grad_c2, ws, ws_bytes, st);
+}
+
+int sem_bspline_map(sem_plan* pl, int nmodel, int nbz, int nbx, const double* coef, const double* base, double* c,
+                    void* c2, void* stream) {
+    if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !coef || !c2)
+        return fail(SEM_E_INVALID, "sem_bspline_map: bad argument (need >= 4 nodes per axis)");
+    int rc = SEM_OK;
+    BsplineTables* t = get_bspline(pl, nbz, nbx, rc);
+    if (!t) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 4096), nmodel);
+    if (pl->d.precision == SEM_FP64)
+        bspline_map_kernel<double><<<grid, 256, 0, st>>>(coef, nbz, nbx, base, (int*)t->bz0.p, (double*)t->bzw.p,
+                                                         (int*)t->bx0.p, (double*)t->bxw.p, pl->NX, pl->NZ, c,
+                                                         (double*)c2);
+    else
+        bspline_map_kernel<float><<<grid, 256, 0, st>>>(coef, nbz, nbx, base, (int*)t->bz0.p, (double*)t->bzw.p,
+                                                        (int*)t->bx0.p, (double*)t->bxw.p, pl->NX, pl->NZ, c,
+                                                        (float*)c2);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double* g_c, double* g_coef,
+                        void* stream) {
+    if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !g_c || !g_coef)
+        return fail(SEM_E_INVALID, "sem_bspline_project: bad argument");
+    int rc = SEM_OK;
+    BsplineTables* t = get_bspline(pl, nbz, nbx, rc);
+    if (!t) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    double* tmp = nullptr;
+    const size_t bytes = (size_t)nmodel * pl->NZ * nbx * sizeof(double);
+    HIP_TRY(hipMallocAsync((void**)&tmp, bytes, st));
+    const long total = (long)pl->NZ * nbx;
+    dim3 g1((unsigned)std::min<long>((total + 255) / 256, 4096), nmodel);
+    bspline_proj_x_kernel<<<g1, 256, 0, st>>>(g_c, pl->NX, pl->NZ, nbx, (int*)t->qlo.p, (int*)t->qhi.p,
+                                              (int*)t->bx0.p, (double*)t->bxw.p, tmp);
+    LAUNCH_CHECK();
+    dim3 g2((unsigned)((nbz * nbx + 255) / 256), nmodel);
+    bspline_proj_z_kernel<<<g2, 256, 0, st>>>(tmp, pl->NZ, nbz, nbx, (int*)t->plo.p, (int*)t->phi.p,
+                                              (int*)t->bz0.p, (double*)t->bzw.p, g_coef);
+    LAUNCH_CHECK();
+    HIP_TRY(hipFreeAsync(tmp, st));
+    return SEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+"""Python host layer over the C ABI: plans, batches and device buffers.
+
+All bulk data lives in torch tensors on the GPU; this module only hands their
+pointers to libmun_sem.so.  There is no CPU compute path: every method raises
+if the library is missing or the tensors are not on a GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+PRECISIONS = {"fp32": (L.SEM_FP32, torch.float32), "fp64": (L.SEM_FP64, torch.float64)}
+ADJ_MODES = {"reference": L.SEM_ADJ_REFERENCE, "exact": L.SEM_ADJ_EXACT}
+RECORD = {"post": L.SEM_RECORD_POST, "pre": L.SEM_RECORD_PRE}
+
+
+def ricker(t, f0, t0=0.1):
+    """(1 - 2 a) exp(-a), a = (pi f0 (t - t0))^2
+    (Bspline_Coupling/SEM/sem_waveform/sources.py:3-5)."""
+    a = (np.pi * f0 * (t - t0)) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+def source_wavelet(nt: int, dt: float, f0: float, t0: float = 0.1, gate_time: float = 0.3,
+                   amp: float = 1.0) -> np.ndarray:
+    """Ricker * amp with the DC term trapz(w)/(t_end - t_0) removed, then zero
+    for t >= gate_time (sources.py:7-21)."""
+    t = np.arange(nt) * dt
+    w = ricker(t, f0, t0) * amp
+    if nt > 1:
+        trap = float(np.sum(0.5 * (w[1:] + w[:-1]) * np.diff(t)))
+        w = w - trap / (t[-1] - t[0])
+    w[t >= gate_time] = 0.0
+    return w
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(device) -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Plan:
+    """One mesh + time axis + sponge + precision; reusable across batches.
+
+    Mirrors the set-up half of SEMSimulation.__init__
+    (Bspline_Coupling/SEM/sem_waveform/core.py:151-325)."""
+
+    def __init__(self, nelem_x: int, nelem_z: int, xmin: float, xmax: float, zmin: float, zmax: float,
+                 dt: float, nt: int, order: int = 4, sponge_width: float = 200.0, sponge_cmax: float = 3000.0,
+                 sponge_R: float = 1e-8, sponge_sides: Sequence[str] = ("xmin", "xmax", "zmin", "zmax"),
+                 precision: str = "fp32", adjoint_mode: str = "reference", record: str = "post",
+                 checkpoint_every: Optional[int] = 100, lagrange_eps: float = 1e-12, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("mun_amd.Plan needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device if device is not None else "cuda")
+        self.lib = L.lib()
+        code, self.dtype = PRECISIONS[precision]
+        sides = 0
+        for s in sponge_sides:
+            sides |= L.SIDE_BITS[s]
+        d = L.SemDesc(nelem_x, nelem_z, order, xmin, xmax, zmin, zmax, dt, nt, sponge_width, sponge_cmax,
+                      sponge_R, sides, code, ADJ_MODES[adjoint_mode], RECORD[record],
+                      0 if checkpoint_every is None else int(checkpoint_every), lagrange_eps)
+        self.desc = d
+        self.precision, self.adjoint_mode, self.record = precision, adjoint_mode, record
+        self.nt, self.dt = int(nt), float(dt)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            L.check(self.lib.sem_plan_create(C.byref(d), C.byref(h)))
+        self.h = h
+        NX, NZ, N = C.c_int(), C.c_int(), C.c_long()
+        L.check(self.lib.sem_plan_dims(h, C.byref(NX), C.byref(NZ), C.byref(N)))
+        self.NX, self.NZ, self.N = NX.value, NZ.value, N.value
+        self.x1d = np.zeros(self.NX)
+        self.z1d = np.zeros(self.NZ)
+        L.check(self.lib.sem_plan_axes(h, self.x1d.ctypes.data_as(C.POINTER(C.c_double)),
+                                       self.z1d.ctypes.data_as(C.POINTER(C.c_double))))
+        self.rec_sizes = []
+        self._ws = None
+
+    @classmethod
+    def from_mesh(cls, m, **kw):
+        """Build from a ``workloads.MeshSpec``."""
+        kw.setdefault("checkpoint_every", m.checkpoint_every)
+        return cls(m.nelem_x, m.nelem_z, m.xmin, m.xmax, m.zmin, m.zmax, m.dt, m.nt, m.order,
+                   m.sponge_width, m.sponge_cmax, 1e-8, m.sponge_sides, **kw)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.sem_plan_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # ------------------------------------------------------------ set-up
+    @property
+    def coords(self) -> np.ndarray:
+        X, Z = np.meshgrid(self.x1d, self.z1d, indexing="xy")
+        return np.stack([X.reshape(-1), Z.reshape(-1)], axis=1)
+
+    def add_receivers(self, xz) -> int:
+        xz = np.ascontiguousarray(np.asarray(xz, dtype=np.float64).reshape(-1, 2))
+        sid = C.c_int()
+        L.check(self.lib.sem_receiver_set_add(self.h, xz.shape[0], xz.ctypes.data_as(C.POINTER(C.c_double)),
+                                              C.byref(sid)))
+        self.rec_sizes.append(xz.shape[0])
+        return sid.value
+
+    def source_node(self, x: float, z: float) -> int:
+        n = C.c_int()
+        L.check(self.lib.sem_source_node(self.h, float(x), float(z), C.byref(n)))
+        return n.value
+
+    def wavelet(self, f0: float, amp: float = 1.0, t0: float = 0.1, gate_time: float = 0.3) -> torch.Tensor:
+        w = source_wavelet(self.nt, self.dt, f0, t0, gate_time, amp)
+        return torch.as_tensor(w, dtype=self.dtype, device=self.device)
+
+    # ------------------------------------------------------------ velocity
+    def bspline_map(self, coef: torch.Tensor, base: Optional[torch.Tensor] = None, want_c: bool = True):
+        """c = base + Bz coef Bx^T on the GPU; returns (c float64 [M,N] or None, c2 [M,N])."""
+        coef = coef.to(device=self.device, dtype=torch.float64).contiguous()
+        if coef.dim() == 2:
+            coef = coef[None]
+        M, nbz, nbx = coef.shape
+        c = torch.empty((M, self.N), dtype=torch.float64, device=self.device) if want_c else None
+        c2 = torch.empty((M, self.N), dtype=self.dtype, device=self.device)
+        if base is not None:
+            base = base.to(device=self.device, dtype=torch.float64).contiguous()
+        L.check(self.lib.sem_bspline_map(self.h, M, nbz, nbx, _ptr(coef), _ptr(base), _ptr(c), _ptr(c2),
+                                         _stream_ptr(self.device)))
+        return c, c2
+
+    def bspline_project(self, g_c: torch.Tensor, nbz: int, nbx: int) -> torch.Tensor:
+        """g_coef = Bz^T g_c Bx (chain rule of ``bspline_map``)."""
+        g_c = g_c.to(device=self.device, dtype=torch.float64).contiguous()
+        if g_c.dim() == 1:
+            g_c = g_c[None]
+        M = g_c.shape[0]
+        out = torch.empty((M, nbz, nbx), dtype=torch.float64, device=self.device)
+        L.check(self.lib.sem_bspline_project(self.h, M, nbz, nbx, _ptr(g_c), _ptr(out),
+                                             _stream_ptr(self.device)))
+        return out
+
+    # ------------------------------------------------------------ solves
+    def workspace(self, batch: int, pitch: int, adjoint: bool) -> torch.Tensor:
+        n = C.c_size_t()
+        L.check(self.lib.sem_workspace_bytes(self.h, batch, pitch, int(adjoint), C.byref(n)))
+        if self._ws is None or self._ws.numel() < n.value:
+            self._ws = None
+            self._ws = torch.empty(max(n.value, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def workspace_bytes(self, batch: int, pitch: int, adjoint: bool) -> int:
+        n = C.c_size_t()
+        L.check(self.lib.sem_workspace_bytes(self.h, batch, pitch, int(adjoint), C.byref(n)))
+        return n.value
+
+    def _batch(self, c2, wavelet, src_nodes, rec_sets, model_of, seis, pitch):
+        B = len(rec_sets)
+        rs = (C.c_int * B)(*[int(s) for s in rec_sets])
+        wstride = 0 if wavelet.dim() == 1 else wavelet.shape[-1]
+        bt = L.SemBatch(B, _ptr(c2), _ptr(model_of), _ptr(wavelet), wstride, _ptr(src_nodes), rs, pitch,
+                        _ptr(seis))
+        bt._keep = rs
+        return bt
+
+    def _prep(self, c2, wavelet, src_nodes, rec_sets, model_of):
+        dev = self.device
+        c2 = c2.to(device=dev, dtype=self.dtype).contiguous()
+        if c2.dim() == 1:
+            c2 = c2[None]
+        wavelet = wavelet.to(device=dev, dtype=self.dtype).contiguous()
+        src = torch.as_tensor(np.asarray(src_nodes, dtype=np.int32).reshape(-1), device=dev)
+        B = src.numel()
+        rec_sets = list(rec_sets) if rec_sets is not None else [0] * B
+        if len(rec_sets) == 1 and B > 1:
+            rec_sets = rec_sets * B
+        if len(rec_sets) != B:
+            raise ValueError("rec_sets and src_nodes must have the same length")
+        if model_of is not None:
+            model_of = torch.as_tensor(np.asarray(model_of, dtype=np.int32).reshape(-1), device=dev)
+            if model_of.numel() != B or int(model_of.max()) >= c2.shape[0] or int(model_of.min()) < 0:
+                raise ValueError("model_of must hold one model index per solve")
+        elif c2.shape[0] != B:
+            raise ValueError(f"c2 holds {c2.shape[0]} models for {B} solves; pass model_of")
+        if wavelet.dim() == 2 and wavelet.shape[0] != B:
+            raise ValueError("per-solve wavelets need one row per solve")
+        if wavelet.shape[-1] != self.nt:
+            raise ValueError(f"wavelet must have nt={self.nt} samples")
+        if not bool(torch.isfinite(c2).all()):
+            raise ValueError("velocity model contains non-finite values (NaN/Inf)")
+        pitch = max(self.rec_sizes[s] for s in rec_sets)
+        return c2, wavelet, src, rec_sets, model_of, pitch
+
+    def forward(self, c2, wavelet, src_nodes, rec_sets=None, model_of=None) -> torch.Tensor:
+        """Batched forward solve; returns seismograms [B, nt, nrec_max]."""
+        c2, wavelet, src, rec_sets, model_of, pitch = self._prep(c2, wavelet, src_nodes, rec_sets, model_of)
+        B = len(rec_sets)
+        seis = torch.empty((B, self.nt, pitch), dtype=self.dtype, device=self.device)
+        ws = self.workspace(B, pitch, False)
+        bt = self._batch(c2, wavelet, src, rec_sets, model_of, seis, pitch)
+        L.check(self.lib.sem_forward(self.h, C.byref(bt), _ptr(ws), ws.numel(), _stream_ptr(self.device)))
+        return seis
+
+    def forward_adjoint(self, c2, wavelet, src_nodes, obs, noise_std, rec_sets=None, model_of=None,
+                        want_grad_c: bool = True, want_grad_c2: bool = False):
+        """Forward + misfit + adjoint gradient for a batch.
+
+        Returns dict(loglik [B] float64, misfit [B], grad_c [B,N] (d loglik/dc),
+        grad_c2 (optional), seis [B,nt,pitch])."""
+        c2, wavelet, src, rec_sets, model_of, pitch = self._prep(c2, wavelet, src_nodes, rec_sets, model_of)
+        B = len(rec_sets)
+        obs = obs.to(device=self.device, dtype=self.dtype)
+        if obs.dim() == 2:
+            obs = obs[None]
+        if obs.shape[0] != B or obs.shape[1] != self.nt or obs.shape[2] > pitch:
+            raise ValueError(f"obs must be [B={B}, nt={self.nt}, <= {pitch}]")
+        if obs.shape[2] < pitch:
+            obs = torch.nn.functional.pad(obs, (0, pitch - obs.shape[2]))
+        obs = obs.contiguous()
+        seis = torch.empty((B, self.nt, pitch), dtype=self.dtype, device=self.device)
+        misfit = torch.empty(B, dtype=torch.float64, device=self.device)
+        gc = torch.empty((B, self.N), dtype=torch.float64, device=self.device) if want_grad_c else None
+        gc2 = torch.empty((B, self.N), dtype=torch.float64, device=self.device) if want_grad_c2 else None
+        sig = np.asarray(noise_std, dtype=np.float64).reshape(-1)
+        if sig.size > 1 and not np.all(sig == sig[0]):
+            raise ValueError("one noise_std per batch")
+        inv_s2 = 1.0 / (float(sig[0]) ** 2 + 1e-24)
+        ws = self.workspace(B, pitch, True)
+        bt = self._batch(c2, wavelet, src, rec_sets, model_of, seis, pitch)
+        L.check(self.lib.sem_forward_adjoint(self.h, C.byref(bt), _ptr(obs), inv_s2, _ptr(misfit), _ptr(gc),
+                                             _ptr(gc2), _ptr(ws), ws.numel(), _stream_ptr(self.device)))
+        return {"misfit": misfit, "loglik": -misfit * inv_s2, "grad_c": gc, "grad_c2": gc2, "seis": seis,
+                "inv_sigma2": inv_s2}

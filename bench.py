@@ -1,0 +1,180 @@
+"""Benchmark: batched forward+adjoint SEM solves per second on one MI355X.
+
+Workload (one "step" = one batch): 256 velocity models on a 200 x 200
+element, degree-4 mesh (801 x 801 GLL nodes, 4 km x 4 km), 5-layer background
++ 16 x 16 cubic B-spline perturbations N(0, 150 m/s), 15 Hz Ricker source,
+200 surface receivers, dt 0.25 ms, 6000 steps, wavefield checkpoints every
+100 steps; each solve = forward + misfit + adjoint + gradient projected onto
+the 256 B-spline nodes, FP32 state with FP64 gradient accumulation.
+
+Printed beside it: the plain path (one solve per launch sequence, batch 1),
+the live roofline of the time-step kernel (HIP events on the launch stream)
+and the CPU oracle (FP64, structured like the reference: assembled CSR
+stiffness + sparse mat-vec per step) timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--nt NT]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FWD_BYTES_PER_NODE = 20  # u^n, w^n, c^2 read; u^{n+1}, w^{n+1} written (FP32)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, nsteps=24):
+    """FP64 oracle timed on a bounded sample of the same mesh: stiffness
+    assembly + `nsteps` forward steps + `nsteps` adjoint steps (with the
+    element-wise gradient correlation), extrapolated to one full solve."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    import oracle as O
+    m = wl.mesh
+    prob = O.Problem(m.xmin, m.xmax, m.zmin, m.zmax, m.nelem_x, m.nelem_z, 4, m.dt, nsteps,
+                     tuple(wl.sources[0]), wl.f0, wl.receivers, sponge_width=m.sponge_width,
+                     sponge_cmax=m.sponge_cmax, sponge_sides=m.sponge_sides)
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    try:
+        disc = O.discretize(prob)
+        c, _, _ = O.tensor_bspline_velocity(disc.mesh, wl.coef[0], wl.base)
+        t0 = time.perf_counter()
+        K = O.stiffness_csr(c, disc.mesh, disc.ops)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        seis, hist = O.forward(disc, c, record="post", history="full", K=K)
+        t_fwd = (time.perf_counter() - t0) / nsteps
+        t0 = time.perf_counter()
+        O.adjoint(disc, c, hist, seis * 0.01, 1.0, mode="reference", K=K)
+        t_adj = (time.perf_counter() - t0) / nsteps
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+    t_solve = t_setup + m.nt * (t_fwd + t_adj)
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle on the 200x200 deg-4 mesh: CSR assembly ({t_setup:.2f} s) + {nsteps} forward "
+                      f"steps ({t_fwd * 1e3:.1f} ms/step) + {nsteps} adjoint steps ({t_adj * 1e3:.1f} ms/step), "
+                      f"extrapolated to {m.nt} steps = {t_solve:.0f} s per forward+adjoint solve, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from mun_amd import workloads as W
+    from mun_amd.batch import BatchSolver
+
+    dev = torch.device("cuda:0")
+    wl = W.batch200(nmodel=args.batch, nt=args.nt)
+    m = wl.mesh
+    log(f"[bench] workload {wl.name}: batch {args.batch}, mesh {m.nelem_x}x{m.nelem_z} (N={m.N}), nt={m.nt}")
+    solver = BatchSolver.from_workload(wl, precision="fp32", adjoint_mode="reference", device=dev)
+    plan = solver.plan
+    coef = torch.as_tensor(wl.coef, device=dev)
+    # observations from the seeded true model (one forward solve)
+    obs = solver.forward(torch.as_tensor(wl.true_coef[None], device=dev))[0]
+    torch.cuda.synchronize()
+    noise = 1e-9
+
+    def step():
+        return solver.loglik_grad(coef, obs, noise)
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i}: {time.perf_counter() - t0:.2f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ll, grad, _ = step()
+    torch.cuda.synchronize()
+    dt_total = time.perf_counter() - t0
+    ms_per_step = dt_total / args.steps * 1e3
+    value = args.batch * args.steps / dt_total
+    log(f"[bench] timed {args.steps} batch(es): {ms_per_step:.1f} ms/batch, {value:.3f} solves/s")
+    assert bool(torch.isfinite(ll).all()) and bool(torch.isfinite(grad).all())
+
+    # live roofline of the time-step kernel: a forward-only pass is nt launches
+    # of the step kernel (plus one receiver pass), timed with events on the
+    # stream the library launches on
+    _, c2 = solver.velocity(coef)
+    src, model_of = solver._expand(args.batch)
+    st = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+    e0.record(st)
+    plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_launch = e0.elapsed_time(e1) / 1e3 / m.nt
+    bytes_launch = FWD_BYTES_PER_NODE * m.N * args.batch
+    achieved = bytes_launch / t_launch / 1e9
+    log(f"[bench] step kernel: {t_launch * 1e6:.1f} us/launch, {achieved:.0f} GB/s")
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "step_fwd_batch200_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tj = json.load(f)
+        if tj.get("batch") == args.batch:
+            traffic = tj.get("bytes_per_launch")
+
+    out = {"metric": "fwd+adjoint SEM solves/s (1 GPU, batched vs plain); p50/p99 solve latency",
+           "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"batch200: {args.batch} models x (200x200 deg-4 elements, 801x801 nodes, "
+                                  f"{m.nt} steps) forward+adjoint+gradient, checkpoints every 100 steps",
+                      "batch": args.batch, "nt": m.nt, "nodes": m.N, "parallelism": "none (1 GPU)"},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "kernel": "step_kernel<float,4,MODE_FWD>",
+                        "algorithmic_bytes_per_launch": bytes_launch},
+           "latency_s": {"p50": round(ms_per_step / 1e3, 3), "p99": round(ms_per_step / 1e3, 3),
+                         "note": "batched path: every solve completes with its batch"}}
+
+    if not args.no_plain:
+        plain = BatchSolver(plan, wl.sources, wl.receivers, wl.f0, solver.nbz, solver.nbx, wl.base)
+        plain.loglik_grad(coef[:1], obs, noise)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain.loglik_grad(coef[:1], obs, noise)
+        torch.cuda.synchronize()
+        tp1 = time.perf_counter() - t0
+        out["plain_solves_per_s"] = round(1.0 / tp1, 4)
+        out["batched_vs_plain"] = round(value * tp1, 3)
+        log(f"[bench] plain path: {tp1:.2f} s per solve")
+    if not args.no_cpu:
+        log("[bench] cpu baseline (bounded sample) ...")
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -146,6 +146,25 @@ int sem_bspline_map(sem_plan* plan, int nmodel, int nbz, int nbx, const double* 
 int sem_bspline_project(sem_plan* plan, int nmodel, int nbz, int nbx, const double* g_c,
                         double* g_coef, void* stream);
 
+/* Closed-curve anomaly velocity, the reference's parameterisation: signed
+ * distance to a closed cubic B-spline through 6 control points (nearest of
+ * `samples` curve samples + `newton_steps` Newton steps, sign from a
+ * point-in-polygon test), v = v_out + (v_in - v_out) sigmoid(-sd/tau).
+ * Replaces build_velocity_on_sem_nodes / signed_distance_to_spline_newton
+ * (Bspline_Coupling/SEM/sem_waveform/velocity.py:37-202).
+ *   ctrl  device double [nmodel][6][2]  absolute control points (x, z)
+ *   c     device double [nmodel][N] or NULL;  c2 device real [nmodel][N]
+ *   aux   device double [nmodel][4][N]  (sd, u*, n_out.x, n_out.z) for the projection
+ * samples <= 2048. */
+int sem_curve_velocity(sem_plan* plan, int nmodel, const double* ctrl, double v_inside, double v_outside,
+                       double tau, int samples, int newton_steps, double* c, void* c2, double* aux, void* stream);
+/* Analytic projection of a nodal gradient dL/dv onto the 12 control
+ * coordinates, interleaved [x0, z0, x1, z1, ...]; replaces
+ * _analytical_gradient_projection_improved (core.py:902-970, 1357-1363).
+ *   dL_dv device double [nmodel][N];  grad_ctrl device double [nmodel][12] */
+int sem_curve_project(sem_plan* plan, int nmodel, const double* dL_dv, const double* aux, double v_inside,
+                      double v_outside, double tau, double* grad_ctrl, void* stream);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* sem_last_error(void);
 

@@ -21,7 +21,8 @@ ERROR_NAMES = {-1: "SEM_E_INVALID", -2: "SEM_E_UNSUPPORTED", -3: "SEM_E_NOMEM", 
 
 EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axes", "sem_receiver_set_add",
            "sem_receiver_set_size", "sem_source_node", "sem_workspace_bytes", "sem_forward",
-           "sem_forward_adjoint", "sem_bspline_map", "sem_bspline_project", "sem_last_error"]
+           "sem_forward_adjoint", "sem_bspline_map", "sem_bspline_project", "sem_curve_velocity",
+           "sem_curve_project", "sem_last_error"]
 
 
 class SemDesc(C.Structure):
@@ -71,6 +72,8 @@ def lib() -> C.CDLL:
     L.sem_forward_adjoint.argtypes = [vp, C.POINTER(SemBatch), vp, d, vp, vp, vp, vp, sz, vp]
     L.sem_bspline_map.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]
     L.sem_bspline_project.argtypes = [vp, i, i, i, vp, vp, vp]
+    L.sem_curve_velocity.argtypes = [vp, i, vp, d, d, d, i, i, vp, vp, vp, vp]
+    L.sem_curve_project.argtypes = [vp, i, vp, vp, d, d, d, vp, vp]
     for name in EXPORTS:
         if name != "sem_last_error":
             getattr(L, name).restype = C.c_int

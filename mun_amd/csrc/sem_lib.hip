@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/sem_capi.h"
+#include "sem_curve.h"
 #include "sem_kernels.h"
 
 using namespace sem;
@@ -201,7 +202,7 @@ struct sem_plan {
     int ntx = 0, ntz = 0, ntiles = 0;
     std::vector<double> xi, wq, Dm, x1d, z1d;
     std::vector<double> invWx, dtWz, Dxv, Dzv;
-    DevBuf d_invWx, d_dtWz, d_Dx, d_Dz;
+    DevBuf d_invWx, d_dtWz, d_Dx, d_Dz, d_x1d, d_z1d;
     // receiver sets
     std::vector<std::vector<RecHost>> sets;  // per set, entries across tiles (ordered by tile)
     std::vector<int> set_size;
@@ -670,6 +671,8 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
         pl->Dzv[j] = std::exp(-s * d->dt);
     }
     int rc = d->precision == SEM_FP64 ? upload_tables<double>(pl) : upload_tables<float>(pl);
+    if (!rc) rc = pl->d_x1d.upload(pl->x1d.data(), sizeof(double) * pl->NX);
+    if (!rc) rc = pl->d_z1d.upload(pl->z1d.data(), sizeof(double) * pl->NZ);
     if (rc) {
         delete pl;
         return rc;
@@ -819,6 +822,42 @@ int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double
                                               (int*)t->bz0.p, (double*)t->bzw.p, g_coef);
     LAUNCH_CHECK();
     HIP_TRY(hipFreeAsync(tmp, st));
+    return SEM_OK;
+}
+
+int sem_curve_velocity(sem_plan* pl, int nmodel, const double* ctrl, double v_in, double v_out, double tau,
+                       int samples, int newton_steps, double* c, void* c2, double* aux, void* stream) {
+    if (!pl || nmodel < 1 || nmodel > 65535 || !ctrl || !c2 || !aux)
+        return fail(SEM_E_INVALID, "sem_curve_velocity: bad argument");
+    if (samples < 2 || samples > CURVE_MAX_SAMPLES || newton_steps < 0)
+        return fail(SEM_E_INVALID, "sem_curve_velocity: samples must be in [2, 2048], newton_steps >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 1024), nmodel);
+    if (pl->d.precision == SEM_FP64)
+        curve_velocity_kernel<double><<<grid, 256, 0, st>>>(ctrl, samples, newton_steps, v_in, v_out, tau,
+                                                            (double*)pl->d_x1d.p, (double*)pl->d_z1d.p, pl->NX,
+                                                            pl->NZ, c, (double*)c2, aux);
+    else
+        curve_velocity_kernel<float><<<grid, 256, 0, st>>>(ctrl, samples, newton_steps, v_in, v_out, tau,
+                                                           (double*)pl->d_x1d.p, (double*)pl->d_z1d.p, pl->NX,
+                                                           pl->NZ, c, (float*)c2, aux);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int sem_curve_project(sem_plan* pl, int nmodel, const double* dL_dv, const double* aux, double v_in, double v_out,
+                      double tau, double* grad_ctrl, void* stream) {
+    if (!pl || nmodel < 1 || nmodel > 65535 || !dL_dv || !aux || !grad_ctrl)
+        return fail(SEM_E_INVALID, "sem_curve_project: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nblk = (int)std::min<long>((pl->N + 255) / 256, 512);
+    double* partial = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * 18 * nblk * nmodel, st));
+    curve_project_kernel<<<dim3(nblk, nmodel), 256, 0, st>>>(dL_dv, aux, v_in, v_out, tau, pl->N, partial);
+    LAUNCH_CHECK();
+    curve_project_finish_kernel<<<nmodel, 64, 0, st>>>(partial, nblk, grad_ctrl);
+    LAUNCH_CHECK();
+    HIP_TRY(hipFreeAsync(partial, st));
     return SEM_OK;
 }
 

@@ -84,6 +84,7 @@ typedef struct {
  * with rec_pitch >= the largest receiver count of the sets used. */
 typedef struct {
     int batch;
+    int nmodel;                /* rows of c2                                       */
     const void* c2;            /* device real [nmodel][N]  squared velocity        */
     const int* model_of;       /* device int [batch] or NULL (identity)            */
     const void* wavelet;       /* device real [.][nt]  source time function        */

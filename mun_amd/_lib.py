@@ -35,7 +35,7 @@ class SemDesc(C.Structure):
 
 
 class SemBatch(C.Structure):
-    _fields_ = [("batch", C.c_int), ("c2", C.c_void_p), ("model_of", C.c_void_p), ("wavelet", C.c_void_p),
+    _fields_ = [("batch", C.c_int), ("nmodel", C.c_int), ("c2", C.c_void_p), ("model_of", C.c_void_p), ("wavelet", C.c_void_p),
                 ("wavelet_stride", C.c_long), ("src_node", C.c_void_p), ("rec_set", C.POINTER(C.c_int)),
                 ("rec_pitch", C.c_int), ("seis", C.c_void_p)]
 

@@ -15,51 +15,67 @@
 // increment at full FP32 precision.  The adjoint uses the same step with the
 // receiver residual as source (and D = 1 in the reference mode).
 //
-// Work decomposition: one thread per element.  A 256-thread workgroup covers a
-// 16 x 16 block of elements: 15 x 15 owned elements plus one halo column on
-// the left and one halo row below.  Each thread keeps its element's 5 x 5
-// values in registers and computes the 25 local contributions; element-edge
-// contributions reach the owning element through a 9-value LDS exchange.
-// Every node is written by exactly one thread (owner computes, no atomics), so
-// results are bitwise deterministic and independent of batch composition.
+// Memory layout: every nodal array is [solve][NZ][pitch] with pitch a
+// multiple of 32 elements (128-byte rows) and 256-byte aligned solves.
+//
+// Work decomposition: one thread per element.  A workgroup covers a block of
+// EX x EZ = 17 x 15 elements: 16 x 14 owned elements plus one halo column on
+// the left and one halo row below.  Owned x-ranges are 64 nodes (256 bytes)
+// and start on 256-byte boundaries, so owned rows are written as whole cache
+// lines.  Each thread keeps its element's 5 x 5 values in registers and
+// computes the 25 local contributions; element-edge contributions reach the
+// owning element through a 9-value LDS exchange.  Every node is written by
+// exactly one thread (owner computes, no atomics), so results are bitwise
+// deterministic and independent of batch composition.
+//
+// Workgroups are numbered so that each XCD (workgroup id mod 8 under the
+// hardware's round-robin dispatch) walks a contiguous range of (tile, solve)
+// pairs: neighbouring tiles share halo lines in the same L2.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace sem {
 
-constexpr int EX = 16;            // element threads along x per workgroup
-constexpr int EZ = 16;            // element threads along z per workgroup
-constexpr int TX = EX - 1;        // owned elements along x per tile
+constexpr int EX = 17;            // element threads along x per workgroup
+constexpr int EZ = 15;            // element threads along z per workgroup
+constexpr int TX = EX - 1;        // owned elements along x per tile (64 nodes)
 constexpr int TZ = EZ - 1;        // owned elements along z per tile
-constexpr int NTHREADS = EX * EZ;
+constexpr int NTHREADS = 256;     // EX * EZ = 255 element threads + 1 idle
+constexpr int NXCD = 8;           // XCDs of an MI355X
 
 enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2 };
 
 template <typename T, int P>
 struct ElemConst {
-    T D[P + 1][P + 1];  // D[a][k] = l_k'(xi_a) on the reference element
-    T w[P + 1];         // GLL weights
-    T cx, cz;           // (2/dx)^2, (2/dz)^2
+    T D[P + 1][P + 1];   // D[a][k] = l_k'(xi_a): derivative at node a
+    T Dw[P + 1][P + 1];  // Dw[a][i] = D[a][i] * w_a: weighted back-projection
+    T ax[P + 1];         // (2/dx)^2 w_j : row factor of the x part
+    T az[P + 1];         // (2/dz)^2 w_i : column factor of the z part
+    T gx[P + 1][P + 1];  // (2/dx)^2 w_j w_q : gradient weight, x part
+    T gz[P + 1][P + 1];  // (2/dz)^2 w_q w_i : gradient weight, z part
 };
 
 template <typename T, int P>
 struct RecEntry {
-    int ex, ez;         // element holding the receiver
-    int r;              // receiver column within its set
-    int owned;          // 1 in the tile that owns the element (writes the sample)
-    T lx[P + 1];        // Lagrange weights along x
-    T lz[P + 1];        // Lagrange weights along z
+    int ex, ez;          // element holding the receiver
+    int r;               // receiver column within its set
+    int owned;           // 1 in the tile that owns the element (writes the sample)
+    T lx[P + 1];         // Lagrange weights along x
+    T lz[P + 1];         // Lagrange weights along z
 };
 
 template <typename T, int P>
 struct StepArgs {
     ElemConst<T, P> ec;
-    int nex, nez, NX, NZ;
-    int ntx, ntiles;
-    const T* u_in;   long u_in_bs;     // u^n (or lambda_k), read with halo
-    T* u_out;        long u_out_bs;    // u^{n+1} (or lambda_{k-1}), owned nodes
-    T* w;            long w_bs;        // increment, updated in place
-    const T* c2;     long c2_bs;
+    int nex, nez, NX, NZ, pitch;
+    int ntx, ntiles, total, nper;      // tiles, tiles*batch, workgroups per XCD
+    long sb;                           // elements between solves of every nodal array
+    const T* u_in;                     // u^n (or lambda_k), read with halo
+    T* u_out;                          // u^{n+1} (or lambda_{k-1}), owned nodes
+    T* w;                              // increment, updated in place
+    const T* c2;                       // padded squared velocity, per model
     const int* model_of;
     const T* invWx;                    // [NX]  1 / Wx
     const T* dtWz;                     // [NZ]  dt^2 / Wz
@@ -67,7 +83,7 @@ struct StepArgs {
     const T* Dz;                       // [NZ]  exp(-sz dt)
     int damp;
     // point source (forward): a[src] += dt^2 s_n
-    const int* src_node;
+    const int* src_node;               // lexicographic node id (jz * NX + ix)
     const T* wavelet; long wav_bs; int n; T dt2;
     // receivers
     const int* rec_set_of;             // [batch] receiver set id
@@ -75,234 +91,370 @@ struct StepArgs {
     const RecEntry<T, P>* recs;
     T* seis;         long seis_bs; int rec_pitch; int rec_t;
     // checkpoint mirrors of (u^{n+1}, w^{n+1}), may be null
-    T* ck_u;         T* ck_w;      long ck_bs;
+    T* ck_u;         T* ck_w;
     // adjoint: residual source, forward wavefield, gradient accumulator
     const T* resid;  long resid_bs; int res_t; T invJ;
-    const T* uf;     long uf_bs;
-    double* G;       long G_bs;
+    const T* uf;
+    double* G;
 };
+
+// 16-byte vector of T
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+    using type = float4;
+};
+template <>
+struct V16<double> {
+    using type = double2;
+};
+
+// five consecutive values starting at a 16-byte aligned address
+template <typename T>
+__device__ __forceinline__ void load5(const T* __restrict__ p, T (&v)[5]) {
+    if constexpr (std::is_same_v<T, float>) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+    } else {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 c = *reinterpret_cast<const double2*>(p + 2);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = c.x;
+        v[3] = c.y;
+    }
+    v[4] = p[4];
+}
+
+template <typename T>
+__device__ __forceinline__ void load4(const T* __restrict__ p, T (&v)[4]) {
+    if constexpr (std::is_same_v<T, float>) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+    } else {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 c = *reinterpret_cast<const double2*>(p + 2);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = c.x;
+        v[3] = c.y;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* __restrict__ p, const T (&v)[4]) {
+    if constexpr (std::is_same_v<T, float>) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+    }
+}
+
+// Element contributions r = Ke u / J (x and z parts with folded constants);
+// in the adjoint mode also the gradient correlation g of (uf, u).
+template <typename T, int P, bool CORR>
+__device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const T (&u)[P + 1][P + 1],
+                                               const T (&c2)[P + 1][P + 1], const T (&v)[P + 1][P + 1],
+                                               T (&r)[P + 1][P + 1], T (&g)[P + 1][P + 1]) {
+    constexpr int NP = P + 1;
+    // x direction, row by row
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        T f[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            T d = T(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[j][k];
+            f[q] = c2[j][q] * d;
+            if constexpr (CORR) {
+                T du = T(0);
+#pragma unroll
+                for (int k = 0; k < NP; ++k) du += ec.D[q][k] * v[j][k];
+                g[j][q] = ec.gx[j][q] * (du * d);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) s += ec.Dw[q][i] * f[q];
+            r[j][i] = ec.ax[j] * s;
+        }
+    }
+    // z direction, column by column
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        T f[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            T d = T(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[k][i];
+            f[q] = c2[q][i] * d;
+            if constexpr (CORR) {
+                T du = T(0);
+#pragma unroll
+                for (int k = 0; k < NP; ++k) du += ec.D[q][k] * v[k][i];
+                g[q][i] += ec.gz[q][i] * (du * d);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) s += ec.Dw[q][j] * f[q];
+            r[j][i] += ec.az[i] * s;
+        }
+    }
+}
 
 template <typename T, int P, int MODE>
 __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) {
+    static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
+    constexpr bool ADJ = (MODE == MODE_ADJ);
     __shared__ T xr_col[EZ][EX][NP];   // element's i = P column of contributions
     __shared__ T xr_row[EZ][EX][P];    // element's j = P row (i < P)
-    __shared__ T xg_col[MODE == MODE_ADJ ? EZ : 1][EX][NP];
-    __shared__ T xg_row[MODE == MODE_ADJ ? EZ : 1][EX][P];
+    __shared__ T xg_col[ADJ ? EZ : 1][EX][NP];
+    __shared__ T xg_row[ADJ ? EZ : 1][EX][P];
+
+    // XCD-contiguous numbering of (tile, solve) pairs
+    const int wgid = blockIdx.x;
+    const int lin = (wgid % NXCD) * a.nper + wgid / NXCD;
+    if (lin >= a.total) return;  // whole workgroup leaves together
+    const int tile = lin % a.ntiles;
+    const long b = lin / a.ntiles;
 
     const int t = threadIdx.x;
     const int exl = t % EX, ezl = t / EX;
-    const int tile = blockIdx.x;
     const int tx = tile % a.ntx, tz = tile / a.ntx;
-    const long b = blockIdx.y;
     const int ex = tx * TX - 1 + exl;
     const int ez = tz * TZ - 1 + ezl;
-    const bool valid = ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
+    const bool valid = t < EX * EZ && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
     const bool owned = valid && exl >= 1 && ezl >= 1;
-    const long NX = a.NX;
-    const long base = (long)(P * ez) * NX + (long)(P * ex);
+    const int pitch = a.pitch;
+    const int base = (P * ez) * pitch + P * ex;  // offset of node (P ez, P ex) within a solve
+    const long soff = b * a.sb;
 
-    T u[NP][NP];
-    T r[NP][NP];
-    T g[MODE == MODE_ADJ ? NP : 1][MODE == MODE_ADJ ? NP : 1];
-
-#pragma unroll
-    for (int j = 0; j < NP; ++j)
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            r[j][i] = T(0);
-            u[j][i] = T(0);
-        }
-    if constexpr (MODE == MODE_ADJ) {
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-#pragma unroll
-            for (int i = 0; i < NP; ++i) g[j][i] = T(0);
-    }
-
-    if (valid) {
-        const T* __restrict__ uin = a.u_in + b * a.u_in_bs + base;
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-#pragma unroll
-            for (int i = 0; i < NP; ++i) u[j][i] = uin[j * NX + i];
-    }
-
-    if constexpr (MODE == MODE_FWD || MODE == MODE_REC) {
-        // seismogram sample of the field being read (R u at this time level)
-        if (a.rec_t >= 0 && owned) {
-            const int set = a.rec_set_of[b];
-            const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
-            const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
-            for (int k = beg; k < end; ++k) {
-                const RecEntry<T, P>& e = a.recs[k];
-                if (e.owned && e.ex == ex && e.ez == ez) {
-                    T acc = T(0);
-#pragma unroll
-                    for (int j = 0; j < NP; ++j) {
-                        T row = T(0);
-#pragma unroll
-                        for (int i = 0; i < NP; ++i) row += e.lx[i] * u[j][i];
-                        acc += e.lz[j] * row;
-                    }
-                    a.seis[b * a.seis_bs + (long)a.rec_t * a.rec_pitch + e.r] = acc;
-                }
-            }
-        }
-        if constexpr (MODE == MODE_REC) return;
-    }
-
-    if (valid) {
-        const long m = a.model_of ? a.model_of[b] : b;
-        const T* __restrict__ c2p = a.c2 + m * a.c2_bs + base;
-        T cw[NP][NP];
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-#pragma unroll
-            for (int i = 0; i < NP; ++i) cw[j][i] = c2p[j * NX + i] * (a.ec.w[j] * a.ec.w[i]);
-
-        // forward wavefield derivatives for the gradient correlation
-        T dxu[MODE == MODE_ADJ ? NP : 1][MODE == MODE_ADJ ? NP : 1];
-        T dzu[MODE == MODE_ADJ ? NP : 1][MODE == MODE_ADJ ? NP : 1];
-        if constexpr (MODE == MODE_ADJ) {
-            const T* __restrict__ ufp = a.uf + b * a.uf_bs + base;
-            T v[NP][NP];
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int i = 0; i < NP; ++i) v[j][i] = ufp[j * NX + i];
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int q = 0; q < NP; ++q) {
-                    T sx = T(0), sz = T(0);
-#pragma unroll
-                    for (int k = 0; k < NP; ++k) {
-                        sx += a.ec.D[q][k] * v[j][k];
-                        sz += a.ec.D[j][k] * v[k][q];
-                    }
-                    dxu[j][q] = sx;   // d/dxi at (j, q)
-                    dzu[j][q] = sz;   // d/deta at (j, q)
-                }
-        }
-
-        // x direction: rows
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            T f[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                T d = T(0);
-#pragma unroll
-                for (int k = 0; k < NP; ++k) d += a.ec.D[q][k] * u[j][k];
-                f[q] = cw[j][q] * a.ec.cx * d;
-                if constexpr (MODE == MODE_ADJ)
-                    g[j][q] = (a.ec.w[j] * a.ec.w[q]) * a.ec.cx * (dxu[j][q] * d);
-            }
-#pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                T s = T(0);
-#pragma unroll
-                for (int q = 0; q < NP; ++q) s += a.ec.D[q][i] * f[q];
-                r[j][i] = s;
-            }
-        }
-        // z direction: columns
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            T f[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                T d = T(0);
-#pragma unroll
-                for (int k = 0; k < NP; ++k) d += a.ec.D[q][k] * u[k][i];
-                f[q] = cw[q][i] * a.ec.cz * d;
-                if constexpr (MODE == MODE_ADJ)
-                    g[q][i] += (a.ec.w[q] * a.ec.w[i]) * a.ec.cz * (dzu[q][i] * d);
-            }
+    if constexpr (MODE == MODE_REC) {
+        // seismogram samples only: touch the receiver elements alone
+        if (a.rec_t < 0 || !owned) return;
+        const int set = a.rec_set_of[b];
+        const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
+        const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
+        for (int k = beg; k < end; ++k) {
+            const RecEntry<T, P>& e = a.recs[k];
+            if (!(e.owned && e.ex == ex && e.ez == ez)) continue;
+            const T* __restrict__ uin = a.u_in + soff + base;
+            T acc = T(0);
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
-                T s = T(0);
+                T row = T(0);
 #pragma unroll
-                for (int q = 0; q < NP; ++q) s += a.ec.D[q][j] * f[q];
-                r[j][i] += s;
+                for (int i = 0; i < NP; ++i) row += e.lx[i] * uin[j * pitch + i];
+                acc += e.lz[j] * row;
             }
+            a.seis[b * a.seis_bs + (long)a.rec_t * a.rec_pitch + e.r] = acc;
+        }
+        return;
+    } else {
+        T u[NP][NP], r[NP][NP];
+        T g[ADJ ? NP : 1][ADJ ? NP : 1];
+        if (valid) {
+            const T* __restrict__ uin = a.u_in + soff + base;
+            const long m = a.model_of ? a.model_of[b] : b;
+            const T* __restrict__ c2p = a.c2 + m * a.sb + base;
+            T c2[NP][NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                load5(uin + j * pitch, u[j]);
+                load5(c2p + j * pitch, c2[j]);
+            }
+            T v[ADJ ? NP : 1][ADJ ? NP : 1];
+            if constexpr (ADJ) {
+                const T* __restrict__ ufp = a.uf + soff + base;
+#pragma unroll
+                for (int j = 0; j < NP; ++j) load5(ufp + j * pitch, v[j]);
+                element_action<T, P, true>(a.ec, u, c2, v, r, g);
+            } else {
+                element_action<T, P, false>(a.ec, u, c2, u, r, r);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    r[j][i] = T(0);
+                    u[j][i] = T(0);
+                    if constexpr (ADJ) g[j][i] = T(0);
+                }
         }
 
-        if constexpr (MODE == MODE_ADJ) {
-            // adjoint source R^T res, injected into the receiver element's slots
-            const int set = a.rec_set_of[b];
-            const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
-            const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
-            for (int k = beg; k < end; ++k) {
-                const RecEntry<T, P>& e = a.recs[k];
-                if (e.ex == ex && e.ez == ez) {
-                    const T res = a.resid[b * a.resid_bs + (long)a.res_t * a.rec_pitch + e.r] * a.invJ;
+        if constexpr (MODE == MODE_FWD) {
+            // seismogram sample of the field being read (R u at this time level)
+            if (a.rec_t >= 0 && owned) {
+                const int set = a.rec_set_of[b];
+                const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
+                const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
+                for (int k = beg; k < end; ++k) {
+                    const RecEntry<T, P>& e = a.recs[k];
+                    if (e.owned && e.ex == ex && e.ez == ez) {
+                        T acc = T(0);
 #pragma unroll
-                    for (int j = 0; j < NP; ++j)
+                        for (int j = 0; j < NP; ++j) {
+                            T row = T(0);
 #pragma unroll
-                        for (int i = 0; i < NP; ++i) r[j][i] -= (e.lz[j] * e.lx[i]) * res;
+                            for (int i = 0; i < NP; ++i) row += e.lx[i] * u[j][i];
+                            acc += e.lz[j] * row;
+                        }
+                        a.seis[b * a.seis_bs + (long)a.rec_t * a.rec_pitch + e.r] = acc;
+                    }
                 }
             }
         }
-    }
+        if constexpr (ADJ) {
+            // adjoint source R^T res, injected into the receiver element's slots
+            if (valid) {
+                const int set = a.rec_set_of[b];
+                const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
+                const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
+                for (int k = beg; k < end; ++k) {
+                    const RecEntry<T, P>& e = a.recs[k];
+                    if (e.ex == ex && e.ez == ez) {
+                        const T res = a.resid[b * a.resid_bs + (long)a.res_t * a.rec_pitch + e.r] * a.invJ;
+#pragma unroll
+                        for (int j = 0; j < NP; ++j)
+#pragma unroll
+                            for (int i = 0; i < NP; ++i) r[j][i] -= (e.lz[j] * e.lx[i]) * res;
+                    }
+                }
+            }
+        }
 
+        if (t < EX * EZ) {
 #pragma unroll
-    for (int j = 0; j < NP; ++j) xr_col[ezl][exl][j] = r[j][P];
+            for (int j = 0; j < NP; ++j) xr_col[ezl][exl][j] = r[j][P];
 #pragma unroll
-    for (int i = 0; i < P; ++i) xr_row[ezl][exl][i] = r[P][i];
-    if constexpr (MODE == MODE_ADJ) {
+            for (int i = 0; i < P; ++i) xr_row[ezl][exl][i] = r[P][i];
+            if constexpr (ADJ) {
 #pragma unroll
-        for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
+                for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
 #pragma unroll
-        for (int i = 0; i < P; ++i) xg_row[ezl][exl][i] = g[P][i];
-    }
-    __syncthreads();
-    if (!owned) return;
+                for (int i = 0; i < P; ++i) xg_row[ezl][exl][i] = g[P][i];
+            }
+        }
+        __syncthreads();
+        if (!owned) return;
 
-    const int jmax = (ez == a.nez - 1) ? P : P - 1;
-    const int imax = (ex == a.nex - 1) ? P : P - 1;
-    const int src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
-    const T sterm = (src >= 0) ? a.dt2 * a.wavelet[b * a.wav_bs + a.n] : T(0);
-    T* __restrict__ wp = a.w + b * a.w_bs;
-    T* __restrict__ uo = a.u_out + b * a.u_out_bs;
+        // assemble the owned nodes: own contribution + left / below / below-left
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                if (i == 0) r[j][i] += xr_col[ezl][exl - 1][j];
+                if (j == 0) r[j][i] += (i < P) ? xr_row[ezl - 1][exl][i] : xr_col[ezl - 1][exl][P];
+                if (i == 0 && j == 0) r[j][i] += xr_col[ezl - 1][exl - 1][P];
+                if constexpr (ADJ) {
+                    if (i == 0) g[j][i] += xg_col[ezl][exl - 1][j];
+                    if (j == 0) g[j][i] += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
+                    if (i == 0 && j == 0) g[j][i] += xg_col[ezl - 1][exl - 1][P];
+                }
+            }
 
+        // point source: only the element owning the source node takes the slow path
+        int s_loc = -1;
+        T sterm = T(0);
+        if constexpr (MODE == MODE_FWD) {
+            const int src = a.src_node[b];
+            const int sz = src / a.NX, sx = src - sz * a.NX;
+            const int lj = sz - P * ez, li = sx - P * ex;
+            const int jm = (ez == a.nez - 1) ? P : P - 1, im = (ex == a.nex - 1) ? P : P - 1;
+            if (lj >= 0 && lj <= jm && li >= 0 && li <= im) {
+                s_loc = lj * NP + li;
+                sterm = a.dt2 * a.wavelet[b * a.wav_bs + a.n];
+            }
+        }
+
+        T* __restrict__ wp = a.w + soff + base;
+        T* __restrict__ uo = a.u_out + soff + base;
+        const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
+        auto update = [&](int j, int i, T wv) -> T {  // returns w^{n+1}; writes u^{n+1} value into r
+            const int gx = P * ex + i, gz = P * ez + j;
+            T av = -(a.invWx[gx] * a.dtWz[gz]) * r[j][i];
+            if constexpr (MODE == MODE_FWD) {
+                if (j * NP + i == s_loc) av += sterm;
+            }
+            const T Dn = a.damp ? a.Dx[gx] * a.Dz[gz] : T(1);
+            const T wn = Dn * (wv + av);
+            r[j][i] = Dn * u[j][i] + wn;
+            return wn;
+        };
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        if (j > jmax) break;
-        const int gz = P * ez + j;
-        const T dz = a.dtWz[gz];
-        const T Dzv = a.Dz[gz];
+        for (int j = 0; j < P; ++j) {
+            T wv[4], wn[4], un[4];
+            load4(wp + j * pitch, wv);
 #pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            if (i > imax) break;
-            T acc = r[j][i];
-            if (i == 0) acc += xr_col[ezl][exl - 1][j];
-            if (j == 0) acc += (i < P) ? xr_row[ezl - 1][exl][i] : xr_col[ezl - 1][exl][P];
-            if (i == 0 && j == 0) acc += xr_col[ezl - 1][exl - 1][P];
-            const int gx = P * ex + i;
-            const long node = base + j * NX + i;
-            T av = -(a.invWx[gx] * dz) * acc;
-            if (node == src) av += sterm;
-            const T Dn = a.damp ? a.Dx[gx] * Dzv : T(1);
-            const T wn = Dn * (wp[node] + av);
-            const T un = Dn * u[j][i] + wn;
-            wp[node] = wn;
-            uo[node] = un;
+            for (int i = 0; i < P; ++i) {
+                wn[i] = update(j, i, wv[i]);
+                un[i] = r[j][i];
+            }
+            store4(wp + j * pitch, wn);
+            store4(uo + j * pitch, un);
             if constexpr (MODE == MODE_FWD) {
                 if (a.ck_u) {
-                    a.ck_u[b * a.ck_bs + node] = un;
-                    a.ck_w[b * a.ck_bs + node] = wn;
+                    store4(a.ck_w + soff + base + j * pitch, wn);
+                    store4(a.ck_u + soff + base + j * pitch, un);
                 }
             }
-            if constexpr (MODE == MODE_ADJ) {
-                T ga = g[j][i];
-                if (i == 0) ga += xg_col[ezl][exl - 1][j];
-                if (j == 0) ga += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
-                if (i == 0 && j == 0) ga += xg_col[ezl - 1][exl - 1][P];
-                a.G[b * a.G_bs + node] += (double)ga;
+            if constexpr (ADJ) {
+                double* Gp = a.G + soff + base + j * pitch;
+                double gv[4];
+                load4(Gp, gv);
+#pragma unroll
+                for (int i = 0; i < P; ++i) gv[i] += (double)g[j][i];
+                store4(Gp, gv);
             }
         }
+        // domain edges: right column (i = P) of the last element column, top row (j = P)
+        if (last_x || last_z) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    if (!((i == P && last_x && (j < P || last_z)) || (j == P && last_z && i < P))) continue;
+                    const int o = j * pitch + i;
+                    const T wn = update(j, i, wp[o]);
+                    wp[o] = wn;
+                    uo[o] = r[j][i];
+                    if constexpr (MODE == MODE_FWD) {
+                        if (a.ck_u) {
+                            a.ck_w[soff + base + o] = wn;
+                            a.ck_u[soff + base + o] = r[j][i];
+                        }
+                    }
+                    if constexpr (ADJ) a.G[soff + base + o] += (double)g[j][i];
+                }
+        }
+    }
+}
+
+// Copy [nmodel][N] (row length NX) into the padded [nmodel][NZ][pitch] layout.
+template <typename T>
+__global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch, long sb) {
+    const long m = blockIdx.y;
+    const long N = (long)NX * NZ;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
+        const long jz = k / NX, ix = k - jz * NX;
+        dst[m * sb + jz * pitch + ix] = src[m * N + k];
     }
 }
 
@@ -335,16 +487,18 @@ __global__ void fill_kernel(T* p, T v, long n) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) p[k] = v;
 }
 
-// grad_c2 = scale * G ; grad_c = 2 sqrt(c2) grad_c2   (G has stride G_bs, outputs stride N)
+// grad_c2 = scale * G ; grad_c = 2 sqrt(c2) grad_c2   (G padded, outputs [b][N])
 template <typename T>
-__global__ void grad_finalize_kernel(const double* G, long G_bs, const T* c2, const int* model_of, long c2_bs,
-                                     long N, double scale, double* grad_c, double* grad_c2) {
+__global__ void grad_finalize_kernel(const double* G, long sb, int NX, int NZ, int pitch, const T* c2,
+                                     const int* model_of, double scale, double* grad_c, double* grad_c2) {
     const long b = blockIdx.y;
     const long m = model_of ? model_of[b] : b;
+    const long N = (long)NX * NZ;
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
-        const double g2 = scale * G[b * G_bs + k];
+        const long jz = k / NX, ix = k - jz * NX;
+        const double g2 = scale * G[b * sb + jz * pitch + ix];
         if (grad_c2) grad_c2[b * N + k] = g2;
-        if (grad_c) grad_c[b * N + k] = 2.0 * sqrt((double)c2[m * c2_bs + k]) * g2;
+        if (grad_c) grad_c[b * N + k] = 2.0 * sqrt((double)c2[m * sb + jz * pitch + ix]) * g2;
     }
 }
 
@@ -378,19 +532,26 @@ __global__ void bspline_map_kernel(const double* coef, int nbz, int nbx, const d
 }
 
 // stage 1 of g_coef = Bz^T g Bx:  t[m, jz, q] = sum_ix g[m, jz, ix] Bx[ix, q]
-__global__ void bspline_proj_x_kernel(const double* g, int NX, int NZ, int nbx, const int* qlo, const int* qhi,
-                                      const int* bx0, const double* bxw, double* tmp) {
+// one workgroup per (model, row): the row is read once, coalesced, into LDS.
+__global__ __launch_bounds__(256) void bspline_proj_x_kernel(const double* g, int NX, int NZ, int nbx,
+                                                             const int* bx0, const double* bxw, double* tmp) {
+    extern __shared__ double acc_sh[];  // [256][nbx]
     const long m = blockIdx.y;
-    const long total = (long)NZ * nbx;
-    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
-        const int jz = (int)(k / nbx), q = (int)(k % nbx);
-        const double* gr = g + m * (long)NX * NZ + (long)jz * NX;
+    const int jz = blockIdx.x;
+    const double* gr = g + m * (long)NX * NZ + (long)jz * NX;
+    double* mine = acc_sh + threadIdx.x * nbx;
+    for (int q = 0; q < nbx; ++q) mine[q] = 0.0;
+    for (int ix = threadIdx.x; ix < NX; ix += blockDim.x) {
+        const double v = gr[ix];
+        const int q0 = bx0[ix];
+        for (int o = 0; o < 4; ++o)
+            if (q0 + o < nbx) mine[q0 + o] += v * bxw[ix * 4 + o];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nbx; q += blockDim.x) {
         double s = 0.0;
-        for (int ix = qlo[q]; ix < qhi[q]; ++ix) {
-            const int o = q - bx0[ix];
-            if (o >= 0 && o < 4) s += gr[ix] * bxw[ix * 4 + o];
-        }
-        tmp[m * total + k] = s;
+        for (int k = 0; k < (int)blockDim.x; ++k) s += acc_sh[k * nbx + q];
+        tmp[(m * NZ + jz) * nbx + q] = s;
     }
 }
 

@@ -197,7 +197,8 @@ struct BsplineTables {
 struct sem_plan {
     sem_desc d;
     int P = 4, NX = 0, NZ = 0;
-    long N = 0, Npad = 0;
+    long N = 0, sb = 0;
+    int pitch = 0;
     double dx = 0, dz = 0, J = 0;
     int ntx = 0, ntz = 0, ntiles = 0;
     std::vector<double> xi, wq, Dm, x1d, z1d;
@@ -278,12 +279,13 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Layout {
     bool full_history = false;
     int S = 0, nseg = 0;
-    size_t U0, U1, W, CKU, CKW, SEG, L0, L1, OM, RES, G, NREC, RSET, total;
+    size_t C2P, U0, U1, W, CKU, CKW, SEG, L0, L1, OM, RES, G, NREC, RSET, total;
 };
 
+// Workspace regions; every nodal array is [batch][sb] in the padded layout.
 Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
-    const size_t state = (size_t)B * pl->Npad * Ts;
+    const size_t state = (size_t)B * pl->sb * Ts;
     Layout L{};
     const int nt = pl->d.nt;
     L.full_history = adj && (pl->d.checkpoint_every <= 0 || pl->d.checkpoint_every >= nt);
@@ -295,6 +297,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
         o += align256(bytes);
         return at;
     };
+    L.C2P = take(state);
     L.U0 = take(state);
     L.U1 = take(state);
     L.W = take(state);
@@ -312,7 +315,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
         L.L1 = take(state);
         L.OM = take(state);
         L.RES = take((size_t)B * nt * pitch * Ts);
-        L.G = take((size_t)B * pl->Npad * 8);
+        L.G = take((size_t)B * pl->sb * 8);
     }
     L.total = o;
     return L;
@@ -332,20 +335,28 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     StepArgs<T, 4> a;
     std::memset(&a, 0, sizeof(a));
     const int P = 4;
+    const double cx = 4.0 / (pl->dx * pl->dx), cz = 4.0 / (pl->dz * pl->dz);
     for (int i = 0; i <= P; ++i) {
-        a.ec.w[i] = (T)pl->wq[i];
-        for (int k = 0; k <= P; ++k) a.ec.D[i][k] = (T)pl->Dm[i * (P + 1) + k];
+        a.ec.ax[i] = (T)(cx * pl->wq[i]);
+        a.ec.az[i] = (T)(cz * pl->wq[i]);
+        for (int k = 0; k <= P; ++k) {
+            a.ec.D[i][k] = (T)pl->Dm[i * (P + 1) + k];
+            a.ec.Dw[i][k] = (T)(pl->Dm[i * (P + 1) + k] * pl->wq[i]);
+            a.ec.gx[i][k] = (T)(cx * pl->wq[i] * pl->wq[k]);
+            a.ec.gz[i][k] = (T)(cz * pl->wq[i] * pl->wq[k]);
+        }
     }
-    a.ec.cx = (T)(4.0 / (pl->dx * pl->dx));
-    a.ec.cz = (T)(4.0 / (pl->dz * pl->dz));
     a.nex = pl->d.nelem_x;
     a.nez = pl->d.nelem_z;
     a.NX = pl->NX;
     a.NZ = pl->NZ;
+    a.pitch = pl->pitch;
     a.ntx = pl->ntx;
     a.ntiles = pl->ntiles;
-    a.c2 = (const T*)bt->c2;
-    a.c2_bs = pl->N;
+    a.total = pl->ntiles * bt->batch;
+    a.nper = (a.total + NXCD - 1) / NXCD;
+    a.sb = pl->sb;
+    a.c2 = (const T*)(ws + L.C2P);
     a.model_of = bt->model_of;
     a.invWx = (const T*)pl->d_invWx.p;
     a.dtWz = (const T*)pl->d_dtWz.p;
@@ -369,17 +380,19 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
 }
 
 template <typename T, int MODE>
-int launch_step(const sem_plan* pl, int B, const StepArgs<T, 4>& a, hipStream_t st) {
-    dim3 grid(pl->ntiles, B);
-    step_kernel<T, 4, MODE><<<grid, NTHREADS, 0, st>>>(a);
+int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
+    step_kernel<T, 4, MODE><<<a.nper * NXCD, NTHREADS, 0, st>>>(a);
     LAUNCH_CHECK();
     return SEM_OK;
 }
 
 int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout& L) {
-    if (!bt || bt->batch <= 0 || bt->batch > 65535) return fail(SEM_E_INVALID, "batch must be in [1, 65535]");
+    if (!bt || bt->batch <= 0) return fail(SEM_E_INVALID, "batch must be >= 1");
+    if ((long)pl->ntiles * bt->batch >= (1L << 31) - 64) return fail(SEM_E_INVALID, "batch too large for this mesh");
     if (!bt->c2 || !bt->wavelet || !bt->src_node || !bt->rec_set || !bt->seis)
         return fail(SEM_E_INVALID, "sem_batch: null pointer in c2/wavelet/src_node/rec_set/seis");
+    if (bt->nmodel < 1 || bt->nmodel > bt->batch) return fail(SEM_E_INVALID, "nmodel must be in [1, batch]");
+    if (!bt->model_of && bt->nmodel != bt->batch) return fail(SEM_E_INVALID, "model_of is NULL but nmodel != batch");
     if (ws_bytes < L.total)
         return fail(SEM_E_NOMEM, "workspace too small: need " + std::to_string(L.total) + " bytes");
     for (int b = 0; b < bt->batch; ++b) {
@@ -401,6 +414,10 @@ int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hi
     for (int b = 0; b < bt->batch; ++b) nrec[b] = pl->set_size[bt->rec_set[b]];
     HIP_TRY(hipMemcpyAsync(ws + L.RSET, bt->rec_set, sizeof(int) * bt->batch, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ws + L.NREC, nrec.data(), sizeof(int) * bt->batch, hipMemcpyHostToDevice, st));
+    // squared velocities into the padded layout
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), bt->nmodel);
+    pad_rows_kernel<T><<<grid, 256, 0, st>>>((const T*)bt->c2, (T*)(ws + L.C2P), pl->NX, pl->NZ, pl->pitch, pl->sb);
+    LAUNCH_CHECK();
     // the pageable source buffers above must stay valid until the copies land
     HIP_TRY(hipStreamSynchronize(st));
     return SEM_OK;
@@ -411,32 +428,30 @@ int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hi
 template <typename T>
 int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, bool adj, hipStream_t st) {
     const int B = bt->batch, nt = pl->d.nt;
-    const long sb = (long)B * pl->Npad;
+    const long state = (long)B * pl->sb;
     StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);
     T* U0 = (T*)(ws + L.U0);
     T* U1 = (T*)(ws + L.U1);
     T* W = (T*)(ws + L.W);
     T* SEG = (T*)(ws + L.SEG);
     int rc;
-    if ((rc = fill<T>(W, T(0), sb, st))) return rc;
+    if ((rc = fill<T>(W, T(0), state, st))) return rc;
     if (adj && L.full_history) {
-        if ((rc = fill<T>(SEG, T(0), sb, st))) return rc;
+        if ((rc = fill<T>(SEG, T(0), state, st))) return rc;
     } else {
-        if ((rc = fill<T>(U0, T(0), sb, st))) return rc;
+        if ((rc = fill<T>(U0, T(0), state, st))) return rc;
     }
     if (adj && !L.full_history) {
-        if ((rc = fill<T>((T*)(ws + L.CKU), T(0), sb, st))) return rc;
-        if ((rc = fill<T>((T*)(ws + L.CKW), T(0), sb, st))) return rc;
+        if ((rc = fill<T>((T*)(ws + L.CKU), T(0), state, st))) return rc;
+        if ((rc = fill<T>((T*)(ws + L.CKW), T(0), state, st))) return rc;
     }
     const bool post = pl->d.record_mode == SEM_RECORD_POST;
-    a.u_in_bs = a.u_out_bs = a.w_bs = pl->Npad;
     a.w = W;
-    a.ck_bs = pl->Npad;
     const T* last = nullptr;
     for (int n = 0; n < nt; ++n) {
         if (adj && L.full_history) {
-            a.u_in = SEG + (long)n * sb;
-            a.u_out = SEG + (long)(n + 1) * sb;
+            a.u_in = SEG + (long)n * state;
+            a.u_out = SEG + (long)(n + 1) * state;
         } else {
             a.u_in = (n & 1) ? U1 : U0;
             a.u_out = (n & 1) ? U0 : U1;
@@ -446,23 +461,23 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
         a.ck_u = a.ck_w = nullptr;
         if (adj && !L.full_history && (n + 1) % L.S == 0 && n + 1 < nt) {
             const long k = (n + 1) / L.S;
-            a.ck_u = (T*)(ws + L.CKU) + k * sb;
-            a.ck_w = (T*)(ws + L.CKW) + k * sb;
+            a.ck_u = (T*)(ws + L.CKU) + k * state;
+            a.ck_w = (T*)(ws + L.CKW) + k * state;
         }
-        if ((rc = launch_step<T, MODE_FWD>(pl, B, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_FWD>(pl, a, st))) return rc;
         last = a.u_out;
     }
     if (post) {
         a.u_in = last;
         a.rec_t = nt - 1;
-        if ((rc = launch_step<T, MODE_REC>(pl, B, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_REC>(pl, a, st))) return rc;
     }
     return SEM_OK;
 }
 
 template <typename T>
 int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, hipStream_t st) {
-    Layout L = layout(pl, bt->batch, bt->rec_pitch, false);
+    Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, false);
     int rc;
     if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
     char* ws = (char*)wsv;
@@ -474,13 +489,13 @@ int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, 
 template <typename T>
 int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, double inv_sigma2, double* misfit,
                          double* grad_c, double* grad_c2, void* wsv, size_t ws_bytes, hipStream_t st) {
-    Layout L = layout(pl, bt->batch, bt->rec_pitch, true);
+    Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, true);
     int rc;
     if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
     if (!obs || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
     char* ws = (char*)wsv;
     const int B = bt->batch, nt = pl->d.nt;
-    const long sb = (long)B * pl->Npad;
+    const long state = (long)B * pl->sb;
     if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
     if ((rc = fill<T>((T*)bt->seis, T(0), (long)B * nt * bt->rec_pitch, st))) return rc;
     if ((rc = forward_loop<T>(pl, bt, ws, L, true, st))) return rc;
@@ -497,14 +512,12 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     T* L1 = (T*)(ws + L.L1);
     T* OM = (T*)(ws + L.OM);
     double* G = (double*)(ws + L.G);
-    if ((rc = fill<T>(L0, T(0), sb, st))) return rc;
-    if ((rc = fill<T>(OM, T(0), sb, st))) return rc;
-    if ((rc = fill<double>(G, 0.0, sb, st))) return rc;
+    if ((rc = fill<T>(L0, T(0), state, st))) return rc;
+    if ((rc = fill<T>(OM, T(0), state, st))) return rc;
+    if ((rc = fill<double>(G, 0.0, state, st))) return rc;
 
     StepArgs<T, 4> f = base_args<T>(pl, bt, ws, L);  // recompute steps
-    f.u_in_bs = f.u_out_bs = f.w_bs = pl->Npad;
     StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);  // adjoint steps
-    a.u_in_bs = a.u_out_bs = a.w_bs = a.uf_bs = a.G_bs = pl->Npad;
     a.w = OM;
     a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
     a.src_node = nullptr;
@@ -517,31 +530,31 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     for (int k = L.nseg - 1; k >= 0; --k) {
         const int s0 = k * L.S;
         const int len = std::min(L.S, nt - s0);
-        const T* slot0 = SEG;  // u^{s0}
         if (!L.full_history) {
             // recompute u^{s0+1..s0+len} from checkpoint k into SEG[1..len]
-            slot0 = (T*)(ws + L.CKU) + (long)k * sb;
-            f.w = (T*)(ws + L.CKW) + (long)k * sb;
+            const T* slot0 = (T*)(ws + L.CKU) + (long)k * state;
+            f.w = (T*)(ws + L.CKW) + (long)k * state;
             for (int m = 0; m < len; ++m) {
-                f.u_in = (m == 0) ? slot0 : SEG + (long)m * sb;
-                f.u_out = SEG + (long)(m + 1) * sb;
+                f.u_in = (m == 0) ? slot0 : SEG + (long)m * state;
+                f.u_out = SEG + (long)(m + 1) * state;
                 f.n = s0 + m;
-                if ((rc = launch_step<T, MODE_FWD>(pl, B, f, st))) return rc;
+                if ((rc = launch_step<T, MODE_FWD>(pl, f, st))) return rc;
             }
         }
         for (int it = s0 + len - 1; it >= s0; --it) {
             a.u_in = lam_in;
             a.u_out = lam_out;
-            a.uf = SEG + (long)(it - s0 + 1) * sb;  // u^{it+1}
+            a.uf = SEG + (long)(it - s0 + 1) * state;  // u^{it+1}
             a.res_t = it;
-            if ((rc = launch_step<T, MODE_ADJ>(pl, B, a, st))) return rc;
+            if ((rc = launch_step<T, MODE_ADJ>(pl, a, st))) return rc;
             std::swap(lam_in, lam_out);
         }
     }
     if (grad_c || grad_c2) {
         dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), B);
-        grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->Npad, (const T*)bt->c2, bt->model_of, pl->N, pl->N,
-                                                      pl->J * inv_sigma2, grad_c, grad_c2);
+        grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch,
+                                                      (const T*)(ws + L.C2P), bt->model_of, pl->J * inv_sigma2,
+                                                      grad_c, grad_c2);
         LAUNCH_CHECK();
     }
     return SEM_OK;
@@ -617,7 +630,8 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     pl->NX = d->nelem_x * P + 1;
     pl->NZ = d->nelem_z * P + 1;
     pl->N = (long)pl->NX * pl->NZ;
-    pl->Npad = (pl->N + 63) / 64 * 64;
+    pl->pitch = (pl->NX + 31) / 32 * 32;
+    pl->sb = ((long)pl->NZ * pl->pitch + 63) / 64 * 64;
     pl->dx = (d->xmax - d->xmin) / d->nelem_x;
     pl->dz = (d->zmax - d->zmin) / d->nelem_z;
     pl->J = pl->dx * pl->dz * 0.25;
@@ -805,6 +819,7 @@ int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double
                         void* stream) {
     if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !g_c || !g_coef)
         return fail(SEM_E_INVALID, "sem_bspline_project: bad argument");
+    if (nbx > 32) return fail(SEM_E_UNSUPPORTED, "sem_bspline_project: at most 32 nodes along x");
     int rc = SEM_OK;
     BsplineTables* t = get_bspline(pl, nbz, nbx, rc);
     if (!t) return rc;
@@ -812,10 +827,8 @@ int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double
     double* tmp = nullptr;
     const size_t bytes = (size_t)nmodel * pl->NZ * nbx * sizeof(double);
     HIP_TRY(hipMallocAsync((void**)&tmp, bytes, st));
-    const long total = (long)pl->NZ * nbx;
-    dim3 g1((unsigned)std::min<long>((total + 255) / 256, 4096), nmodel);
-    bspline_proj_x_kernel<<<g1, 256, 0, st>>>(g_c, pl->NX, pl->NZ, nbx, (int*)t->qlo.p, (int*)t->qhi.p,
-                                              (int*)t->bx0.p, (double*)t->bxw.p, tmp);
+    bspline_proj_x_kernel<<<dim3(pl->NZ, nmodel), 256, 256 * nbx * sizeof(double), st>>>(
+        g_c, pl->NX, pl->NZ, nbx, (int*)t->bx0.p, (double*)t->bxw.p, tmp);
     LAUNCH_CHECK();
     dim3 g2((unsigned)((nbz * nbx + 255) / 256), nmodel);
     bspline_proj_z_kernel<<<g2, 256, 0, st>>>(tmp, pl->NZ, nbz, nbx, (int*)t->plo.p, (int*)t->phi.p,

@@ -169,7 +169,7 @@ class Plan:
         B = len(rec_sets)
         rs = (C.c_int * B)(*[int(s) for s in rec_sets])
         wstride = 0 if wavelet.dim() == 1 else wavelet.shape[-1]
-        bt = L.SemBatch(B, _ptr(c2), _ptr(model_of), _ptr(wavelet), wstride, _ptr(src_nodes), rs, pitch,
+        bt = L.SemBatch(B, c2.shape[0], _ptr(c2), _ptr(model_of), _ptr(wavelet), wstride, _ptr(src_nodes), rs, pitch,
                         _ptr(seis))
         bt._keep = rs
         return bt

@@ -55,6 +55,9 @@ struct ElemConst {
     T az[P + 1];         // (2/dz)^2 w_i : column factor of the z part
     T gx[P + 1][P + 1];  // (2/dx)^2 w_j w_q : gradient weight, x part
     T gz[P + 1][P + 1];  // (2/dz)^2 w_q w_i : gradient weight, z part
+    T iw[P + 1];         // 1 / w_i          : inverse assembled weight inside an element
+    T iw_shared;         // 1 / (w_0 + w_P)  : inverse weight on a shared element edge
+    T dt2;               // dt^2
 };
 
 template <typename T, int P>
@@ -77,14 +80,12 @@ struct StepArgs {
     T* w;                              // increment, updated in place
     const T* c2;                       // padded squared velocity, per model
     const int* model_of;
-    const T* invWx;                    // [NX]  1 / Wx
-    const T* dtWz;                     // [NZ]  dt^2 / Wz
-    const T* Dx;                       // [NX]  exp(-sx dt)
-    const T* Dz;                       // [NZ]  exp(-sz dt)
+    const T* Dx;                       // [NX padded to 4]  exp(-sx dt)
+    const T* Dz;                       // [NZ padded to 4]  exp(-sz dt)
     int damp;
     // point source (forward): a[src] += dt^2 s_n
     const int* src_node;               // lexicographic node id (jz * NX + ix)
-    const T* wavelet; long wav_bs; int n; T dt2;
+    const T* wavelet; long wav_bs; int n;
     // receivers
     const int* rec_set_of;             // [batch] receiver set id
     const int* rec_ptr;                // [nsets][ntiles + 1] offsets into recs
@@ -269,6 +270,10 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
     } else {
         T u[NP][NP], r[NP][NP];
         T g[ADJ ? NP : 1][ADJ ? NP : 1];
+        // update-phase operands of the owned 4 x 4 block, fetched before the
+        // element math so their latency overlaps it
+        T wv[P][P];
+        T dxv[P], dzv[P];
         if (valid) {
             const T* __restrict__ uin = a.u_in + soff + base;
             const long m = a.model_of ? a.model_of[b] : b;
@@ -278,6 +283,15 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
             for (int j = 0; j < NP; ++j) {
                 load5(uin + j * pitch, u[j]);
                 load5(c2p + j * pitch, c2[j]);
+            }
+            if (owned) {
+                const T* __restrict__ wq = a.w + soff + base;
+#pragma unroll
+                for (int j = 0; j < P; ++j) load4(wq + j * pitch, wv[j]);
+                if (a.damp) {
+                    load4(a.Dx + P * ex, dxv);
+                    load4(a.Dz + P * ez, dzv);
+                }
             }
             T v[ADJ ? NP : 1][ADJ ? NP : 1];
             if constexpr (ADJ) {
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 }
             }
 
-        // point source: only the element owning the source node takes the slow path
+        // point source: at most one element per solve owns the source node
         int s_loc = -1;
         T sterm = T(0);
         if constexpr (MODE == MODE_FWD) {
@@ -380,31 +394,39 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
             const int jm = (ez == a.nez - 1) ? P : P - 1, im = (ex == a.nex - 1) ? P : P - 1;
             if (lj >= 0 && lj <= jm && li >= 0 && li <= im) {
                 s_loc = lj * NP + li;
-                sterm = a.dt2 * a.wavelet[b * a.wav_bs + a.n];
+                sterm = a.ec.dt2 * a.wavelet[b * a.wav_bs + a.n];
             }
         }
 
         T* __restrict__ wp = a.w + soff + base;
         T* __restrict__ uo = a.u_out + soff + base;
         const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
-        auto update = [&](int j, int i, T wv) -> T {  // returns w^{n+1}; writes u^{n+1} value into r
-            const int gx = P * ex + i, gz = P * ez + j;
-            T av = -(a.invWx[gx] * a.dtWz[gz]) * r[j][i];
+        // dt^2 / (Wx Wz) from the GLL weights: shared element edges carry w_0 + w_P,
+        // domain edges w_0 alone
+        auto inv_wx = [&](int i) -> T { return (i == 0 && ex > 0) ? a.ec.iw_shared : a.ec.iw[i]; };
+        auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
+        auto damp_at = [&](int j, int i) -> T {
+            if (!a.damp) return T(1);
+            const T dxi = (i < P) ? dxv[i] : a.Dx[P * ex + i];
+            const T dzj = (j < P) ? dzv[j] : a.Dz[P * ez + j];
+            return dxi * dzj;
+        };
+        auto update = [&](int j, int i, T wold) -> T {  // returns w^{n+1}; leaves u^{n+1} in r[j][i]
+            T av = -((a.ec.dt2 * inv_wz(j)) * inv_wx(i)) * r[j][i];
             if constexpr (MODE == MODE_FWD) {
                 if (j * NP + i == s_loc) av += sterm;
             }
-            const T Dn = a.damp ? a.Dx[gx] * a.Dz[gz] : T(1);
-            const T wn = Dn * (wv + av);
+            const T Dn = damp_at(j, i);
+            const T wn = Dn * (wold + av);
             r[j][i] = Dn * u[j][i] + wn;
             return wn;
         };
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            T wv[4], wn[4], un[4];
-            load4(wp + j * pitch, wv);
+            T wn[4], un[4];
 #pragma unroll
             for (int i = 0; i < P; ++i) {
-                wn[i] = update(j, i, wv[i]);
+                wn[i] = update(j, i, wv[j][i]);
                 un[i] = r[j][i];
             }
             store4(wp + j * pitch, wn);

@@ -202,8 +202,8 @@ struct sem_plan {
     double dx = 0, dz = 0, J = 0;
     int ntx = 0, ntz = 0, ntiles = 0;
     std::vector<double> xi, wq, Dm, x1d, z1d;
-    std::vector<double> invWx, dtWz, Dxv, Dzv;
-    DevBuf d_invWx, d_dtWz, d_Dx, d_Dz, d_x1d, d_z1d;
+    std::vector<double> Dxv, Dzv;
+    DevBuf d_Dx, d_Dz, d_x1d, d_z1d;
     // receiver sets
     std::vector<std::vector<RecHost>> sets;  // per set, entries across tiles (ordered by tile)
     std::vector<int> set_size;
@@ -220,14 +220,13 @@ namespace {
 template <typename T>
 int upload_vec(DevBuf& buf, const std::vector<double>& v) {
     std::vector<T> tmp(v.begin(), v.end());
+    tmp.resize((v.size() + 7) / 8 * 8, T(1));  // vector loads read up to 3 past the end
     return buf.upload(tmp.data(), tmp.size() * sizeof(T));
 }
 
 template <typename T>
 int upload_tables(sem_plan* pl) {
     int rc;
-    if ((rc = upload_vec<T>(pl->d_invWx, pl->invWx))) return rc;
-    if ((rc = upload_vec<T>(pl->d_dtWz, pl->dtWz))) return rc;
     if ((rc = upload_vec<T>(pl->d_Dx, pl->Dxv))) return rc;
     if ((rc = upload_vec<T>(pl->d_Dz, pl->Dzv))) return rc;
     return SEM_OK;
@@ -358,15 +357,15 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     a.sb = pl->sb;
     a.c2 = (const T*)(ws + L.C2P);
     a.model_of = bt->model_of;
-    a.invWx = (const T*)pl->d_invWx.p;
-    a.dtWz = (const T*)pl->d_dtWz.p;
+    for (int i = 0; i <= P; ++i) a.ec.iw[i] = (T)(1.0 / pl->wq[i]);
+    a.ec.iw_shared = (T)(1.0 / (pl->wq[0] + pl->wq[P]));
+    a.ec.dt2 = (T)(pl->d.dt * pl->d.dt);
     a.Dx = (const T*)pl->d_Dx.p;
     a.Dz = (const T*)pl->d_Dz.p;
     a.damp = 1;
     a.src_node = bt->src_node;
     a.wavelet = (const T*)bt->wavelet;
     a.wav_bs = bt->wavelet_stride;
-    a.dt2 = (T)(pl->d.dt * pl->d.dt);
     a.rec_set_of = (const int*)(ws + L.RSET);
     a.rec_ptr = (const int*)pl->d_rec_ptr.p;
     a.recs = (const RecEntry<T, 4>*)pl->d_recs.p;
@@ -652,19 +651,7 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     };
     axis(d->xmin, d->xmax, d->nelem_x, pl->x1d);
     axis(d->zmin, d->zmax, d->nelem_z, pl->z1d);
-    // assembled 1-D weights -> lumped mass M = J Wx Wz (operators.py:65-84)
-    auto wsum = [&](int ne, std::vector<double>& W) {
-        W.assign(ne * P + 1, 0.0);
-        for (int e = 0; e < ne; ++e)
-            for (int i = 0; i <= P; ++i) W[e * P + i] += pl->wq[i];
-    };
-    std::vector<double> Wx, Wz;
-    wsum(d->nelem_x, Wx);
-    wsum(d->nelem_z, Wz);
-    pl->invWx.resize(pl->NX);
-    pl->dtWz.resize(pl->NZ);
-    for (int i = 0; i < pl->NX; ++i) pl->invWx[i] = 1.0 / Wx[i];
-    for (int j = 0; j < pl->NZ; ++j) pl->dtWz[j] = d->dt * d->dt / Wz[j];
+    // the lumped mass M = J Wx Wz (operators.py:65-84) is evaluated in the kernels from the GLL weights
     // sponge: cubic ramp, d0 = -(m+1) c_max ln R / (2 delta)  (boundary.py:26-52)
     const double R = d->sponge_R > 0 ? d->sponge_R : 1e-8;
     const double del = d->sponge_width;

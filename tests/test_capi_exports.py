@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every function include/sem_capi.h
+declares; argument validation runs before any device work (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from mun_amd import _lib as L
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "sem_capi.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sem_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) >= 14
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(L.EXPORTS)
+
+
+def _desc(**kw):
+    d = dict(nelem_x=4, nelem_z=4, order=4, xmin=0.0, xmax=100.0, zmin=0.0, zmax=100.0, dt=1e-3, nt=10,
+             sponge_width=20.0, sponge_cmax=3000.0, sponge_R=1e-8, sponge_sides=15, precision=0,
+             adjoint_mode=0, record_mode=0, checkpoint_every=5, lagrange_eps=1e-12)
+    d.update(kw)
+    return L.SemDesc(**d)
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(order=5), -2), (dict(nelem_x=0), -1), (dict(xmax=0.0), -1), (dict(dt=0.0), -1),
+    (dict(nt=0), -1), (dict(precision=7), -1), (dict(adjoint_mode=3), -1), (dict(record_mode=2), -1),
+    (dict(sponge_width=0.0), -1)])
+def test_plan_create_rejects_bad_descriptors(kw, code):
+    lib = L.lib()
+    h = C.c_void_p()
+    rc = lib.sem_plan_create(C.byref(_desc(**kw)), C.byref(h))
+    assert rc == code
+    assert h.value is None
+    assert lib.sem_last_error().decode()
+    with pytest.raises(L.SemError):
+        L.check(rc)
+
+
+def test_null_arguments_fail_cleanly():
+    lib = L.lib()
+    assert lib.sem_plan_create(None, None) == -1
+    assert lib.sem_plan_dims(None, None, None, None) == -1
+    assert lib.sem_forward(None, None, None, 0, None) == -1
+    assert b"null" in lib.sem_last_error()

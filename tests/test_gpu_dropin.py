@@ -9,7 +9,11 @@ from tests.gpu_helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CTRL = np.array([[-100, -100], [100, -100], [150, -50], [200, 0], [0, 200], [-200, 100]], dtype=np.float64)
+# the reference's control points (Bspline_Coupling/SEM/sem_forward.py:42-43) moved off the mesh
+# lattice: a node lying exactly on the curve has distance ~1e-14 and its outward normal
+# r/(|r| + 1e-15) is rounding noise in any implementation, the reference's included
+CTRL = np.array([[-100, -100], [100, -100], [150, -50], [200, 0], [0, 200], [-200, 100]], dtype=np.float64) \
+    + np.array([3.7, -2.3])
 
 
 def reference_config(precision="fp64", mode="reference", nelem=24):
@@ -46,9 +50,11 @@ def test_curve_velocity_matches_oracle():
     v, sd, us, no = O.closed_curve_velocity(disc.mesh.coords, CTRL, 2000.0, 3000.0, 10.0, 800, 7)
     c, c2, aux = sim._curve_velocity(CTRL)
     got = c[0].cpu().numpy()
-    bad = np.abs(got - v) > 1e-6 * 3000
-    assert bad.mean() < 1e-3, f"{bad.sum()} nodes differ"
-    np.testing.assert_allclose(aux[0, 0].cpu().numpy()[~bad], sd[~bad], atol=1e-8)
+    np.testing.assert_allclose(got, v, rtol=0, atol=1e-9)
+    A = aux[0].cpu().numpy()
+    np.testing.assert_allclose(A[0], sd, atol=1e-8)
+    np.testing.assert_allclose(A[1], us, atol=1e-9)
+    np.testing.assert_allclose(A[2:].T, no, atol=1e-9)
     # the stored model uses the reference node numbering
     perm = O.reference_node_order(24, 24, 4)
     np.testing.assert_allclose(sim.global_coords, disc.mesh.coords[perm], atol=1e-9)

@@ -136,18 +136,24 @@ def test_batched_gradient_matches_central_fd_fp64():
     ll.sum().backward()
     g = c.grad[0].cpu().numpy()
     idx = rng.choice(64, 8, replace=False)
-    h = 1.0
-    fd = []
-    for k in idx:
+
+    def fd(k, h):
         e = torch.zeros_like(coef)
         e.view(-1)[k] = h
         lp = s.loglik_grad(coef + e, obs, 1e-8, want_grad=False)[0].item()
         lm = s.loglik_grad(coef - e, obs, 1e-8, want_grad=False)[0].item()
-        fd.append((lp - lm) / (2 * h))
-    fd = np.array(fd)
-    err = np.abs(g.reshape(-1)[idx] - fd) / np.abs(g).max()
-    print("  max err / max|g| =", err.max())
-    assert err.max() < 1e-5
+        return (lp - lm) / (2 * h)
+
+    f1 = np.array([fd(k, 1.0) for k in idx])           # h = 1 m/s as the workload spec states
+    f2 = np.array([fd(k, 0.5) for k in idx])
+    rich = (4.0 * f2 - f1) / 3.0                        # O(h^4) Richardson extrapolation
+    ga = g.reshape(-1)[idx]
+    scale = np.abs(g).max()
+    e1 = np.abs(ga - f1) / scale
+    er = np.abs(ga - rich) / scale
+    print("  h=1 FD err / max|g| =", e1.max(), " Richardson err =", er.max())
+    assert e1.max() < 1e-3          # FD truncation at h = 1 m/s
+    assert er.max() < 1e-6          # the adjoint itself
 
 
 def test_batched_gradient_matches_oracle_and_shots_share_models():

@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-nt-scale", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -170,6 +174,26 @@ def main():
         out["plain_solves_per_s"] = round(1.0 / tp1, 4)
         out["batched_vs_plain"] = round(value * tp1, 3)
         log(f"[bench] plain path: {tp1:.2f} s per solve")
+    if not args.no_serve:
+        from mun_amd.serving import SEMServer, latency_stats, run_plain
+        log(f"[bench] served stream: {args.serve_requests} requests, Poisson {args.serve_rate}/s, "
+            f"meshes 20..160^2, nt scale {args.serve_nt_scale}")
+        del solver, plan
+        torch.cuda.empty_cache()
+        reqs = W.served_solve_requests(args.serve_requests, args.serve_rate, nt_scale=args.serve_nt_scale)
+        torch.cuda.synchronize()
+        srv = SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000)
+        served = latency_stats(srv.run(reqs, progress=log))
+        plain = latency_stats(run_plain(reqs, progress=log))
+        out["served"] = {"workload": f"{args.serve_requests} Poisson({args.serve_rate}/s) forward+adjoint requests, "
+                                     f"meshes {{20,40,80,160}}^2 deg-4, {int(1000 * args.serve_nt_scale)}-"
+                                     f"{int(5000 * args.serve_nt_scale)} steps, 20-200 receivers",
+                         "continuous": served, "plain_one_at_a_time": plain,
+                         "throughput_ratio": round(served["solves_per_s"] / plain["solves_per_s"], 3)}
+        out["latency_s"] = {"p50": round(served["p50_s"], 4), "p99": round(served["p99_s"], 4),
+                            "note": "served stream (continuous batching), arrival to result"}
+        log(f"[bench] served p50 {served['p50_s']:.3f}s p99 {served['p99_s']:.3f}s {served['solves_per_s']:.2f}/s; "
+            f"plain p50 {plain['p50_s']:.3f}s p99 {plain['p99_s']:.3f}s {plain['solves_per_s']:.2f}/s")
     if not args.no_cpu:
         log("[bench] cpu baseline (bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline(wl)

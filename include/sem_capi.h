@@ -166,6 +166,72 @@ int sem_curve_velocity(sem_plan* plan, int nmodel, const double* ctrl, double v_
 int sem_curve_project(sem_plan* plan, int nmodel, const double* dL_dv, const double* aux, double v_inside,
                       double v_outside, double tau, double* grad_ctrl, void* stream);
 
+/* ---- continuous batching (slots) -------------------------------------------
+ * A slot is one solve that advances through its own phases at its own time
+ * step.  A slot table lives in device memory; sem_slots_step() advances every
+ * slot whose phase matches by one step of its current segment, so requests of
+ * one mesh size with different step counts, sources and receivers share each
+ * launch and may join or leave between segments.  All pointers are device
+ * pointers into the slot's own buffers (nodal arrays in the plan's padded
+ * layout, sem_plan_layout()).  No reference counterpart: the reference runs one
+ * solve per call (core.py:1067) and its drivers loop over samples and sources
+ * (CO2_plume/one_stage/single_cubic/RQS_NF.py:1035-1072, 1488-1492). */
+#define SEM_SLOT_IDLE 0
+#define SEM_SLOT_FORWARD 1    /* u0/u1 ping-pong (by step parity), w increment       */
+#define SEM_SLOT_RECOMPUTE 2  /* u0 = checkpoint u of the segment, w = its w, seg    */
+#define SEM_SLOT_ADJOINT 3    /* u0/u1 = lambda ping-pong by step parity, w = omega  */
+
+typedef struct {
+    int phase;             /* SEM_SLOT_*                                              */
+    int n0, len;           /* first time step of the current segment, its length     */
+    int src_node;          /* injection node                                          */
+    int rec_set;           /* receiver set id                                         */
+    int record;            /* forward: sample seismograms during this segment         */
+    void* u0;
+    void* u1;
+    void* w;
+    const void* c2;        /* padded squared velocity of this solve                   */
+    const void* wavelet;   /* [nt]                                                    */
+    void* seis;            /* [nt][rec_pitch]                                         */
+    void* ck_u;            /* forward: (u, w) at the end of the segment go here, or NULL */
+    void* ck_w;
+    void* seg;             /* recompute: slots 1..len of u;  adjoint: u^{it+1} at slot it-n0+1 */
+    long seg_stride;       /* elements between segment slots                          */
+    const void* resid;     /* adjoint source [nt][rec_pitch]                          */
+    double* G;             /* gradient accumulator (padded layout, double)            */
+} sem_slot;
+
+/* Padded nodal layout: row pitch and per-array element count. */
+int sem_plan_layout(const sem_plan* plan, int* pitch, long* array_elems);
+/* One step of every slot whose phase is in phase_mask (bit 1 << SEM_SLOT_*):
+ * forward/recompute slots run the forward step, adjoint slots the adjoint step
+ * (a mask mixing both launches both kernels).  m = step index within the
+ * current segments; slots with m >= len skip.  rec_pitch: seismogram row length. */
+int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
+                   int rec_pitch, void* stream);
+/* Receiver samples of u (final post-update sample) for slots in phase_mask:
+ * forward slots sample u[(n0+len)&1] into seis row n0+len-1. */
+int sem_slots_record_final(sem_plan* plan, const sem_slot* slots_device, int nslots, unsigned phase_mask,
+                           int rec_pitch, void* stream);
+
+/* Receiver tables for captured launch sequences: reserve device capacity for
+ * `max_entries` tile entries (each receiver appears in 1-4 tiles) and
+ * `max_sets` sets so that later sem_receiver_set_add calls never move the
+ * table; sem_receivers_sync uploads sets added since the last upload (only the
+ * appended part is written, so launches in flight are unaffected). */
+int sem_receivers_reserve(sem_plan* plan, long max_entries, int max_sets);
+int sem_receivers_sync(sem_plan* plan);
+
+/* Per-solve helpers for slot owners (device pointers):
+ * copy [nmodel][N] squared velocities into the padded layout;  */
+int sem_pad_nodal(sem_plan* plan, const void* src, void* dst_padded, int nmodel, void* stream);
+/* res = seis - obs on the first nrec columns of nt rows, misfit = 0.5 sum res^2 (double) */
+int sem_residual(sem_plan* plan, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
+                 int rec_pitch, void* stream);
+/* grad_c2 = scale * G, grad_c = 2 sqrt(c2) grad_c2 for one solve (G, c2 padded; outputs [N]) */
+int sem_gradient_finalize(sem_plan* plan, const double* G, const void* c2_padded, double scale, double* grad_c,
+                          double* grad_c2, void* stream);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* sem_last_error(void);
 

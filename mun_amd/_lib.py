@@ -22,7 +22,8 @@ ERROR_NAMES = {-1: "SEM_E_INVALID", -2: "SEM_E_UNSUPPORTED", -3: "SEM_E_NOMEM", 
 EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axes", "sem_receiver_set_add",
            "sem_receiver_set_size", "sem_source_node", "sem_workspace_bytes", "sem_forward",
            "sem_forward_adjoint", "sem_bspline_map", "sem_bspline_project", "sem_curve_velocity",
-           "sem_curve_project", "sem_last_error"]
+           "sem_curve_project", "sem_plan_layout", "sem_slots_step", "sem_slots_record_final", "sem_pad_nodal",
+           "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize", "sem_last_error"]
 
 
 class SemDesc(C.Structure):
@@ -38,6 +39,17 @@ class SemBatch(C.Structure):
     _fields_ = [("batch", C.c_int), ("nmodel", C.c_int), ("c2", C.c_void_p), ("model_of", C.c_void_p), ("wavelet", C.c_void_p),
                 ("wavelet_stride", C.c_long), ("src_node", C.c_void_p), ("rec_set", C.POINTER(C.c_int)),
                 ("rec_pitch", C.c_int), ("seis", C.c_void_p)]
+
+
+SLOT_IDLE, SLOT_FORWARD, SLOT_RECOMPUTE, SLOT_ADJOINT = 0, 1, 2, 3
+
+
+class SemSlot(C.Structure):
+    _fields_ = [("phase", C.c_int), ("n0", C.c_int), ("len", C.c_int), ("src_node", C.c_int),
+                ("rec_set", C.c_int), ("record", C.c_int), ("u0", C.c_void_p), ("u1", C.c_void_p),
+                ("w", C.c_void_p), ("c2", C.c_void_p), ("wavelet", C.c_void_p), ("seis", C.c_void_p),
+                ("ck_u", C.c_void_p), ("ck_w", C.c_void_p), ("seg", C.c_void_p), ("seg_stride", C.c_long),
+                ("resid", C.c_void_p), ("G", C.c_void_p)]
 
 
 class SemError(RuntimeError):
@@ -74,6 +86,14 @@ def lib() -> C.CDLL:
     L.sem_bspline_project.argtypes = [vp, i, i, i, vp, vp, vp]
     L.sem_curve_velocity.argtypes = [vp, i, vp, d, d, d, i, i, vp, vp, vp, vp]
     L.sem_curve_project.argtypes = [vp, i, vp, vp, d, d, d, vp, vp]
+    L.sem_plan_layout.argtypes = [vp, ip, C.POINTER(C.c_long)]
+    L.sem_slots_step.argtypes = [vp, vp, i, i, C.c_uint, i, vp]
+    L.sem_slots_record_final.argtypes = [vp, vp, i, C.c_uint, i, vp]
+    L.sem_pad_nodal.argtypes = [vp, vp, vp, i, vp]
+    L.sem_receivers_reserve.argtypes = [vp, C.c_long, i]
+    L.sem_receivers_sync.argtypes = [vp]
+    L.sem_residual.argtypes = [vp, vp, vp, vp, vp, i, i, i, vp]
+    L.sem_gradient_finalize.argtypes = [vp, vp, vp, d, vp, vp, vp]
     for name in EXPORTS:
         if name != "sem_last_error":
             getattr(L, name).restype = C.c_int

@@ -36,6 +36,8 @@
 
 #include <type_traits>
 
+#include "../../include/sem_capi.h"
+
 namespace sem {
 
 constexpr int EX = 17;            // element threads along x per workgroup
@@ -97,7 +99,103 @@ struct StepArgs {
     const T* resid;  long resid_bs; int res_t; T invJ;
     const T* uf;
     double* G;
+    // slot mode: per-solve pointers and steps come from a device slot table
+    const sem_slot* slots;  // null: uniform batch mode
+    int m;                  // step index within the slots' current segments
+    unsigned phase_mask;    // slots advanced by this launch (bit 1 << SEM_SLOT_*)
+    int post;               // record mode: 1 = sample after the update
 };
+
+// Per-workgroup pointers and indices of one solve at one step.
+template <typename T>
+struct View {
+    const T* u_in;
+    T* u_out;
+    T* w;
+    const T* c2;
+    const T* uf;
+    double* G;
+    T* ck_u;
+    T* ck_w;
+    T* seis;
+    const T* resid;
+    const T* wav;  // wavelet sample of this step (null: no source)
+    int src, set, rec_t, res_t;
+};
+
+// Resolve the view of solve b; returns false if this solve does not advance.
+template <typename T, int P, int MODE>
+__device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<T>& v) {
+    if (!a.slots) {
+        const long soff = b * a.sb;
+        const long mdl = a.model_of ? a.model_of[b] : b;
+        v.u_in = a.u_in + soff;
+        v.u_out = a.u_out + soff;
+        v.w = a.w + soff;
+        v.c2 = a.c2 + mdl * a.sb;
+        v.uf = a.uf ? a.uf + soff : nullptr;
+        v.G = a.G ? a.G + soff : nullptr;
+        v.ck_u = a.ck_u ? a.ck_u + soff : nullptr;
+        v.ck_w = a.ck_w ? a.ck_w + soff : nullptr;
+        v.seis = a.seis ? a.seis + b * a.seis_bs : nullptr;
+        v.resid = a.resid ? a.resid + b * a.resid_bs : nullptr;
+        v.wav = (MODE == MODE_FWD && a.src_node) ? a.wavelet + b * a.wav_bs + a.n : nullptr;
+        v.src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
+        v.set = a.rec_set_of ? a.rec_set_of[b] : 0;
+        v.rec_t = a.rec_t;
+        v.res_t = a.res_t;
+        return true;
+    }
+    const sem_slot& d = a.slots[b];
+    if (!((a.phase_mask >> d.phase) & 1u)) return false;
+    v.c2 = (const T*)d.c2;
+    v.set = d.rec_set;
+    v.G = d.G;
+    v.seis = (T*)d.seis;
+    v.resid = (const T*)d.resid;
+    v.ck_u = v.ck_w = nullptr;
+    v.uf = nullptr;
+    v.wav = nullptr;
+    v.src = -1;
+    v.rec_t = v.res_t = -1;
+    if (MODE == MODE_REC) {  // final post-update sample of a forward slot
+        const int n = d.n0 + d.len;
+        v.u_in = (const T*)((n & 1) ? d.u1 : d.u0);
+        v.u_out = nullptr;
+        v.w = nullptr;
+        v.rec_t = n - 1;
+        return d.phase == SEM_SLOT_FORWARD && d.record;
+    }
+    if (a.m >= d.len) return false;
+    if (d.phase == SEM_SLOT_FORWARD || d.phase == SEM_SLOT_RECOMPUTE) {
+        const int n = d.n0 + a.m;
+        v.wav = (const T*)d.wavelet + n;
+        v.src = d.src_node;
+        if (d.phase == SEM_SLOT_FORWARD) {
+            v.u_in = (const T*)((n & 1) ? d.u1 : d.u0);
+            v.u_out = (T*)((n & 1) ? d.u0 : d.u1);
+            v.w = (T*)d.w;
+            v.rec_t = d.record ? (a.post ? n - 1 : n) : -1;
+            if (a.m == d.len - 1 && d.ck_u) {
+                v.ck_u = (T*)d.ck_u;
+                v.ck_w = (T*)d.ck_w;
+            }
+        } else {
+            v.u_in = (a.m == 0) ? (const T*)d.u0 : (const T*)d.seg + (long)a.m * d.seg_stride;
+            v.u_out = (T*)d.seg + (long)(a.m + 1) * d.seg_stride;
+            v.w = (T*)d.w;
+        }
+        return MODE == MODE_FWD;
+    }
+    // adjoint: lambda_it in u[it & 1], lambda_{it-1} into the other
+    const int it = d.n0 + d.len - 1 - a.m;
+    v.u_in = (const T*)((it & 1) ? d.u1 : d.u0);
+    v.u_out = (T*)((it & 1) ? d.u0 : d.u1);
+    v.w = (T*)d.w;
+    v.uf = (const T*)d.seg + (long)(it - d.n0 + 1) * d.seg_stride;
+    v.res_t = it;
+    return MODE == MODE_ADJ;
+}
 
 // 16-byte vector of T
 template <typename T>
@@ -244,18 +342,19 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
     const bool owned = valid && exl >= 1 && ezl >= 1;
     const int pitch = a.pitch;
     const int base = (P * ez) * pitch + P * ex;  // offset of node (P ez, P ex) within a solve
-    const long soff = b * a.sb;
+    View<T> V;
+    if (!make_view<T, P, MODE>(a, b, V)) return;  // uniform per workgroup
 
     if constexpr (MODE == MODE_REC) {
         // seismogram samples only: touch the receiver elements alone
-        if (a.rec_t < 0 || !owned) return;
-        const int set = a.rec_set_of[b];
+        if (V.rec_t < 0 || !owned) return;
+        const int set = V.set;
         const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
         const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
         for (int k = beg; k < end; ++k) {
             const RecEntry<T, P>& e = a.recs[k];
             if (!(e.owned && e.ex == ex && e.ez == ez)) continue;
-            const T* __restrict__ uin = a.u_in + soff + base;
+            const T* __restrict__ uin = V.u_in + base;
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
@@ -264,7 +363,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 for (int i = 0; i < NP; ++i) row += e.lx[i] * uin[j * pitch + i];
                 acc += e.lz[j] * row;
             }
-            a.seis[b * a.seis_bs + (long)a.rec_t * a.rec_pitch + e.r] = acc;
+            V.seis[(long)V.rec_t * a.rec_pitch + e.r] = acc;
         }
         return;
     } else {
@@ -275,9 +374,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         T wv[P][P];
         T dxv[P], dzv[P];
         if (valid) {
-            const T* __restrict__ uin = a.u_in + soff + base;
-            const long m = a.model_of ? a.model_of[b] : b;
-            const T* __restrict__ c2p = a.c2 + m * a.sb + base;
+            const T* __restrict__ uin = V.u_in + base;
+            const T* __restrict__ c2p = V.c2 + base;
             T c2[NP][NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
@@ -285,7 +383,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 load5(c2p + j * pitch, c2[j]);
             }
             if (owned) {
-                const T* __restrict__ wq = a.w + soff + base;
+                const T* __restrict__ wq = V.w + base;
 #pragma unroll
                 for (int j = 0; j < P; ++j) load4(wq + j * pitch, wv[j]);
                 if (a.damp) {
@@ -295,7 +393,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
             }
             T v[ADJ ? NP : 1][ADJ ? NP : 1];
             if constexpr (ADJ) {
-                const T* __restrict__ ufp = a.uf + soff + base;
+                const T* __restrict__ ufp = V.uf + base;
 #pragma unroll
                 for (int j = 0; j < NP; ++j) load5(ufp + j * pitch, v[j]);
                 element_action<T, P, true>(a.ec, u, c2, v, r, g);
@@ -315,8 +413,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
 
         if constexpr (MODE == MODE_FWD) {
             // seismogram sample of the field being read (R u at this time level)
-            if (a.rec_t >= 0 && owned) {
-                const int set = a.rec_set_of[b];
+            if (V.rec_t >= 0 && owned) {
+                const int set = V.set;
                 const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
                 const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
                 for (int k = beg; k < end; ++k) {
@@ -330,7 +428,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                             for (int i = 0; i < NP; ++i) row += e.lx[i] * u[j][i];
                             acc += e.lz[j] * row;
                         }
-                        a.seis[b * a.seis_bs + (long)a.rec_t * a.rec_pitch + e.r] = acc;
+                        V.seis[(long)V.rec_t * a.rec_pitch + e.r] = acc;
                     }
                 }
             }
@@ -338,13 +436,13 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         if constexpr (ADJ) {
             // adjoint source R^T res, injected into the receiver element's slots
             if (valid) {
-                const int set = a.rec_set_of[b];
+                const int set = V.set;
                 const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
                 const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
                 for (int k = beg; k < end; ++k) {
                     const RecEntry<T, P>& e = a.recs[k];
                     if (e.ex == ex && e.ez == ez) {
-                        const T res = a.resid[b * a.resid_bs + (long)a.res_t * a.rec_pitch + e.r] * a.invJ;
+                        const T res = V.resid[(long)V.res_t * a.rec_pitch + e.r] * a.invJ;
 #pragma unroll
                         for (int j = 0; j < NP; ++j)
 #pragma unroll
@@ -388,18 +486,18 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         int s_loc = -1;
         T sterm = T(0);
         if constexpr (MODE == MODE_FWD) {
-            const int src = a.src_node[b];
+            const int src = V.src;
             const int sz = src / a.NX, sx = src - sz * a.NX;
             const int lj = sz - P * ez, li = sx - P * ex;
             const int jm = (ez == a.nez - 1) ? P : P - 1, im = (ex == a.nex - 1) ? P : P - 1;
-            if (lj >= 0 && lj <= jm && li >= 0 && li <= im) {
+            if (src >= 0 && lj >= 0 && lj <= jm && li >= 0 && li <= im) {
                 s_loc = lj * NP + li;
-                sterm = a.ec.dt2 * a.wavelet[b * a.wav_bs + a.n];
+                sterm = a.ec.dt2 * (*V.wav);
             }
         }
 
-        T* __restrict__ wp = a.w + soff + base;
-        T* __restrict__ uo = a.u_out + soff + base;
+        T* __restrict__ wp = V.w + base;
+        T* __restrict__ uo = V.u_out + base;
         const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
         // dt^2 / (Wx Wz) from the GLL weights: shared element edges carry w_0 + w_P,
         // domain edges w_0 alone
@@ -432,13 +530,13 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
             store4(wp + j * pitch, wn);
             store4(uo + j * pitch, un);
             if constexpr (MODE == MODE_FWD) {
-                if (a.ck_u) {
-                    store4(a.ck_w + soff + base + j * pitch, wn);
-                    store4(a.ck_u + soff + base + j * pitch, un);
+                if (V.ck_u) {
+                    store4(V.ck_w + base + j * pitch, wn);
+                    store4(V.ck_u + base + j * pitch, un);
                 }
             }
             if constexpr (ADJ) {
-                double* Gp = a.G + soff + base + j * pitch;
+                double* Gp = V.G + base + j * pitch;
                 double gv[4];
                 load4(Gp, gv);
 #pragma unroll
@@ -458,12 +556,12 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                     wp[o] = wn;
                     uo[o] = r[j][i];
                     if constexpr (MODE == MODE_FWD) {
-                        if (a.ck_u) {
-                            a.ck_w[soff + base + o] = wn;
-                            a.ck_u[soff + base + o] = r[j][i];
+                        if (V.ck_u) {
+                            V.ck_w[base + o] = wn;
+                            V.ck_u[base + o] = r[j][i];
                         }
                     }
-                    if constexpr (ADJ) a.G[soff + base + o] += (double)g[j][i];
+                    if constexpr (ADJ) V.G[base + o] += (double)g[j][i];
                 }
         }
     }

@@ -181,6 +181,25 @@ struct DevBuf {
         }
         return SEM_OK;
     }
+    // grow to at least `bytes` (contents are not preserved); returns 1 if reallocated
+    int reserve(size_t bytes, bool& moved) {
+        moved = false;
+        if (bytes <= n) return SEM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail(SEM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        n = bytes;
+        moved = true;
+        return SEM_OK;
+    }
+    int write_at(size_t off, const void* src, size_t bytes) {
+        if (!bytes) return SEM_OK;
+        hipError_t e = hipMemcpy((char*)p + off, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(SEM_E_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return SEM_OK;
+    }
 };
 
 struct RecHost {
@@ -209,6 +228,7 @@ struct sem_plan {
     std::vector<int> set_size;
     DevBuf d_recs, d_rec_ptr;
     bool rec_dirty = true;
+    size_t rec_up_entries = 0, rec_up_ptrs = 0;  // prefix already on the device
     std::map<std::pair<int, int>, BsplineTables*> bsp;
     ~sem_plan() {
         for (auto& kv : bsp) delete kv.second;
@@ -266,9 +286,24 @@ int upload_receivers(sem_plan* pl) {
         }
         ptr.push_back((int)flat.size());
     }
+    // Sets are only ever appended, so the device prefix stays valid: upload the
+    // new suffix in place (kernels in flight keep reading unchanged bytes).  Only
+    // growth past the reserved capacity reallocates.
+    const size_t eb = sizeof(RecEntry<T, 4>);
     int rc;
-    if ((rc = pl->d_recs.upload(flat.data(), flat.size() * sizeof(RecEntry<T, 4>)))) return rc;
-    if ((rc = pl->d_rec_ptr.upload(ptr.data(), ptr.size() * sizeof(int)))) return rc;
+    bool m1 = false, m2 = false;
+    if ((rc = pl->d_recs.reserve(std::max<size_t>(flat.size() * eb, 16), m1))) return rc;
+    if ((rc = pl->d_rec_ptr.reserve(ptr.size() * sizeof(int), m2))) return rc;
+    if (m1 || m2) pl->rec_up_entries = pl->rec_up_ptrs = 0;
+    if (m1 && !m2) pl->rec_up_ptrs = 0;
+    if ((rc = pl->d_recs.write_at(pl->rec_up_entries * eb, flat.data() + pl->rec_up_entries,
+                                  (flat.size() - pl->rec_up_entries) * eb)))
+        return rc;
+    if ((rc = pl->d_rec_ptr.write_at(pl->rec_up_ptrs * sizeof(int), ptr.data() + pl->rec_up_ptrs,
+                                     (ptr.size() - pl->rec_up_ptrs) * sizeof(int))))
+        return rc;
+    pl->rec_up_entries = flat.size();
+    pl->rec_up_ptrs = ptr.size();
     pl->rec_dirty = false;
     return SEM_OK;
 }
@@ -858,6 +893,135 @@ int sem_curve_project(sem_plan* pl, int nmodel, const double* dL_dv, const doubl
     curve_project_finish_kernel<<<nmodel, 64, 0, st>>>(partial, nblk, grad_ctrl);
     LAUNCH_CHECK();
     HIP_TRY(hipFreeAsync(partial, st));
+    return SEM_OK;
+}
+
+int sem_plan_layout(const sem_plan* pl, int* pitch, long* elems) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    if (pitch) *pitch = pl->pitch;
+    if (elems) *elems = pl->sb;
+    return SEM_OK;
+}
+
+}  // extern "C"
+
+static int prepare_receivers(sem_plan* pl) {
+    if (!pl->rec_dirty) return SEM_OK;
+    return pl->d.precision == SEM_FP64 ? upload_receivers<double>(pl) : upload_receivers<float>(pl);
+}
+
+template <typename T>
+static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
+                        bool final_record, hipStream_t st) {
+    int rc = prepare_receivers(pl);
+    if (rc) return rc;
+    sem_batch dummy;
+    std::memset(&dummy, 0, sizeof(dummy));
+    dummy.batch = nslots;
+    dummy.rec_pitch = rec_pitch;
+    Layout L{};
+    StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, L);
+    a.slots = slots;
+    a.m = m;
+    a.phase_mask = mask;
+    a.post = pl->d.record_mode == SEM_RECORD_POST ? 1 : 0;
+    a.rec_set_of = nullptr;
+    a.c2 = nullptr;
+    if (final_record) return launch_step<T, MODE_REC>(pl, a, st);
+    const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_RECOMPUTE);
+    const unsigned adj = 1u << SEM_SLOT_ADJOINT;
+    if (mask & fwd) {
+        a.damp = 1;
+        if ((rc = launch_step<T, MODE_FWD>(pl, a, st))) return rc;
+    }
+    if (mask & adj) {
+        a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
+        if ((rc = launch_step<T, MODE_ADJ>(pl, a, st))) return rc;
+    }
+    return SEM_OK;
+}
+
+extern "C" {
+
+int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
+                   void* stream) {
+    if (!pl || !slots || nslots < 1 || m < 0 || rec_pitch < 0) return fail(SEM_E_INVALID, "sem_slots_step: bad argument");
+    if ((long)pl->ntiles * nslots >= (1L << 31) - 64) return fail(SEM_E_INVALID, "too many slots for this mesh");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, m, mask, rec_pitch, false, st)
+                                       : slots_launch<float>(pl, slots, nslots, m, mask, rec_pitch, false, st);
+}
+
+int sem_slots_record_final(sem_plan* pl, const sem_slot* slots, int nslots, unsigned mask, int rec_pitch,
+                           void* stream) {
+    if (!pl || !slots || nslots < 1) return fail(SEM_E_INVALID, "sem_slots_record_final: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, 0, mask, rec_pitch, true, st)
+                                       : slots_launch<float>(pl, slots, nslots, 0, mask, rec_pitch, true, st);
+}
+
+int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) {
+    if (!pl || max_entries < 0 || max_sets < 0) return fail(SEM_E_INVALID, "sem_receivers_reserve: bad argument");
+    const size_t eb = pl->d.precision == SEM_FP64 ? sizeof(RecEntry<double, 4>) : sizeof(RecEntry<float, 4>);
+    bool m1 = false, m2 = false;
+    int rc = pl->d_recs.reserve(std::max<size_t>(max_entries * eb, 16), m1);
+    if (!rc) rc = pl->d_rec_ptr.reserve((size_t)max_sets * (pl->ntiles + 1) * sizeof(int), m2);
+    if (rc) return rc;
+    if (m1 || m2) {
+        pl->rec_up_entries = pl->rec_up_ptrs = 0;
+        pl->rec_dirty = true;
+    }
+    return prepare_receivers(pl);
+}
+
+int sem_receivers_sync(sem_plan* pl) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    return prepare_receivers(pl);
+}
+
+int sem_pad_nodal(sem_plan* pl, const void* src, void* dst, int nmodel, void* stream) {
+    if (!pl || !src || !dst || nmodel < 1 || nmodel > 65535) return fail(SEM_E_INVALID, "sem_pad_nodal: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), nmodel);
+    if (pl->d.precision == SEM_FP64)
+        pad_rows_kernel<double><<<grid, 256, 0, st>>>((const double*)src, (double*)dst, pl->NX, pl->NZ, pl->pitch, pl->sb);
+    else
+        pad_rows_kernel<float><<<grid, 256, 0, st>>>((const float*)src, (float*)dst, pl->NX, pl->NZ, pl->pitch, pl->sb);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
+                 int rec_pitch, void* stream) {
+    if (!pl || !seis || !obs || !res || !misfit || nrec < 0 || nrec > rec_pitch || nt < 1)
+        return fail(SEM_E_INVALID, "sem_residual: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    // the kernel reads the receiver count from device memory: stage it in the misfit slot
+    static_assert(sizeof(double) >= sizeof(int), "");
+    HIP_TRY(hipMemcpyAsync(misfit, &nrec, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (pl->d.precision == SEM_FP64)
+        residual_kernel<double><<<1, 256, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
+                                                   (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch);
+    else
+        residual_kernel<float><<<1, 256, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
+                                                  (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int sem_gradient_finalize(sem_plan* pl, const double* G, const void* c2p, double scale, double* grad_c,
+                          double* grad_c2, void* stream) {
+    if (!pl || !G || !c2p) return fail(SEM_E_INVALID, "sem_gradient_finalize: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), 1);
+    if (pl->d.precision == SEM_FP64)
+        grad_finalize_kernel<double><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch, (const double*)c2p,
+                                                           nullptr, pl->J * scale, grad_c, grad_c2);
+    else
+        grad_finalize_kernel<float><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch, (const float*)c2p,
+                                                          nullptr, pl->J * scale, grad_c, grad_c2);
+    LAUNCH_CHECK();
     return SEM_OK;
 }
 

@@ -1,0 +1,413 @@
+"""Continuous-batching server for forward+adjoint SEM solves of uneven sizes.
+
+Requests (mesh size class, step count, velocity B-spline nodes, source,
+receivers, observations) arrive over time.  Each size class is a *lane*: one
+plan, one HIP stream and a device-resident slot table (``sem_slot``, see
+include/sem_capi.h).  A solve occupies one slot from admission to completion
+and walks through rounds of S steps:
+
+    FORWARD seg 0 .. K-1  ->  (final sample, residual, misfit)
+    -> RECOMPUTE seg K-1 -> ADJOINT seg K-1 -> ... -> RECOMPUTE seg 0 -> ADJOINT seg 0
+    -> (gradient, projection onto the request's B-spline nodes) -> done
+
+Every round advances *all* occupied slots of a lane together: per step one
+forward-kernel launch covers the slots in FORWARD/RECOMPUTE and one
+adjoint-kernel launch the slots in ADJOINT, so solves at different phases and
+time steps share launches.  Requests that arrive while others run are admitted
+at the next round boundary of their lane as soon as a slot is free and their
+HBM footprint (state + checkpoints + segment buffer + seismograms) fits the
+budget; lanes run on separate streams, so different size classes overlap on
+the device.  A round can be replayed from a captured HIP graph (its launch
+arguments only point at the lane's device slot table, which is rewritten
+between rounds).
+
+Each slot's arithmetic is the batch path's, so a result does not depend on
+which other requests share its launches.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .solver import Plan, source_wavelet
+
+
+@dataclass
+class SolveRequest:
+    """One forward+adjoint solve on a square-element mesh."""
+    rid: int
+    nelem_x: int
+    nelem_z: int
+    xmax: float
+    zmax: float
+    dt: float
+    nt: int
+    coef: np.ndarray              # (nbz, nbx) B-spline velocity nodes
+    src_xz: tuple
+    f0: float
+    rec_xz: np.ndarray            # (nrec, 2)
+    obs: Optional[np.ndarray]     # (nt, nrec) observations, or None for forward-only
+    noise_std: float = 1e-9
+    sponge_width: float = 200.0
+    sponge_cmax: float = 3500.0
+    arrival: float = 0.0
+    base: Optional[np.ndarray] = None
+
+    @property
+    def lane_key(self):
+        return (self.nelem_x, self.nelem_z, self.xmax, self.zmax, self.dt, self.sponge_width, self.sponge_cmax)
+
+
+@dataclass
+class SolveResult:
+    rid: int
+    loglik: float
+    misfit: float
+    grad: np.ndarray              # d loglik / d coef, (nbz, nbx)
+    seis: np.ndarray              # (nt, nrec) synthetic seismograms
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4):
+    """HBM bytes one solve holds while resident."""
+    E = pitch_elems
+    K = max(1, math.ceil(nt / S))
+    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + 3) + E * 8
+    traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
+    return nodal + traces
+
+
+@dataclass
+class _Active:
+    req: SolveRequest
+    slot: int
+    K: int
+    phase: int
+    k: int
+    buf: Dict[str, torch.Tensor] = field(default_factory=dict)
+    footprint: int = 0
+    admitted: float = 0.0
+    rec_set: int = 0
+    nrec: int = 0
+    src_node: int = 0
+
+
+class Lane:
+    def __init__(self, server: "SEMServer", key, capacity: int, nt_max: int, rec_pitch: int):
+        nex, nez, xmax, zmax, dt, sw, scm = key
+        self.server = server
+        self.key = key
+        self.C = capacity
+        self.S = server.S
+        self.rec_pitch = rec_pitch
+        self.plan = Plan(nex, nez, 0.0, xmax, 0.0, zmax, dt, nt_max, 4, sw, scm, 1e-8, server.sides,
+                         precision="fp32", adjoint_mode=server.adjoint_mode, record="post",
+                         checkpoint_every=self.S, device=server.device)
+        pitch, elems = C.c_int(), C.c_long()
+        L.check(self.plan.lib.sem_plan_layout(self.plan.h, C.byref(pitch), C.byref(elems)))
+        self.E = elems.value
+        # receiver tables: one set per admitted request; each receiver is listed in <= 4 tiles
+        self.max_sets = server.max_sets
+        self._reserve()
+        self.stream = torch.cuda.Stream(device=server.device)
+        self.table_host = (L.SemSlot * capacity)()
+        self.table_pinned = torch.empty(C.sizeof(self.table_host), dtype=torch.uint8).pin_memory()
+        self.table_dev = torch.zeros(C.sizeof(self.table_host), dtype=torch.uint8, device=server.device)
+        self.slots: List[Optional[_Active]] = [None] * capacity
+        self.queue = deque()
+        self.event = None
+        self.graph = None
+        self.rounds = 0
+
+    def _reserve(self):
+        L.check(self.plan.lib.sem_receivers_reserve(self.plan.h, 4 * self.rec_pitch * self.max_sets, self.max_sets))
+
+    def ensure_set_capacity(self):
+        """Grow the receiver table before it would move under a captured graph."""
+        if len(self.plan.rec_sizes) + 1 > self.max_sets:
+            self.max_sets *= 2
+            self._reserve()
+            self.graph = None  # captured arguments point at the old table
+
+    # -------------------------------------------------------------- slots
+    def free_slot(self):
+        for i, a in enumerate(self.slots):
+            if a is None:
+                return i
+        return None
+
+    def occupied(self):
+        return [a for a in self.slots if a is not None]
+
+    def _desc(self, a: _Active) -> L.SemSlot:
+        r, b, S = a.req, a.buf, self.S
+        s0 = a.k * S
+        ln = min(S, r.nt - s0)
+        d = L.SemSlot()
+        d.phase, d.n0, d.len = a.phase, s0, ln
+        d.src_node, d.rec_set, d.record = a.src_node, a.rec_set, 1
+        d.c2, d.wavelet = b["c2p"].data_ptr(), b["wav"].data_ptr()
+        d.seis, d.resid, d.G = b["seis"].data_ptr(), b["res"].data_ptr(), b["G"].data_ptr()
+        d.seg, d.seg_stride = b["seg"].data_ptr(), self.E
+        if a.phase == L.SLOT_FORWARD:
+            d.u0, d.u1, d.w = b["u0"].data_ptr(), b["u1"].data_ptr(), b["w"].data_ptr()
+            if a.k + 1 < a.K:
+                d.ck_u, d.ck_w = b["cku"][a.k + 1].data_ptr(), b["ckw"][a.k + 1].data_ptr()
+        elif a.phase == L.SLOT_RECOMPUTE:
+            d.u0, d.w = b["cku"][a.k].data_ptr(), b["ckw"][a.k].data_ptr()
+        elif a.phase == L.SLOT_ADJOINT:
+            d.u0, d.u1, d.w = b["l0"].data_ptr(), b["l1"].data_ptr(), b["om"].data_ptr()
+        return d
+
+    def write_table(self):
+        for i in range(self.C):
+            a = self.slots[i]
+            self.table_host[i] = self._desc(a) if a is not None else L.SemSlot()
+        self.table_pinned.numpy()[:] = np.frombuffer(self.table_host, dtype=np.uint8)
+        self.table_dev.copy_(self.table_pinned, non_blocking=True)
+
+    def _launch_round(self):
+        lib, h = self.plan.lib, self.plan.h
+        sp = C.c_void_p(self.stream.cuda_stream)
+        mask = (1 << L.SLOT_FORWARD) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT)
+        tab = C.c_void_p(self.table_dev.data_ptr())
+        for m in range(self.S):
+            L.check(lib.sem_slots_step(h, tab, self.C, m, mask, self.rec_pitch, sp))
+
+    def launch_round(self, use_graph: bool):
+        with torch.cuda.stream(self.stream):
+            self.write_table()
+            if use_graph:
+                if self.graph is None:
+                    L.check(self.plan.lib.sem_receivers_sync(self.plan.h))
+                    torch.cuda.synchronize(self.server.device)
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        self._launch_round()
+                    self.graph = g
+                self.graph.replay()
+            else:
+                self._launch_round()
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+        self.rounds += 1
+
+
+class SEMServer:
+    """Continuous-batching forward+adjoint SEM service on one GPU."""
+
+    def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
+                 adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
+                 device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512):
+        self.device = torch.device(device if device is not None else "cuda")
+        self.budget = float(hbm_budget_bytes)
+        self.S = int(segment)
+        self.capacity = capacity or {}
+        self.adjoint_mode = adjoint_mode
+        self.sides = tuple(sides)
+        self.use_graph = use_graph
+        self.rec_pitch = int(rec_pitch)
+        self.nt_max = int(nt_max)
+        self.max_sets = int(max_sets)
+        self.lanes: Dict[tuple, Lane] = {}
+        self.in_use = 0
+        self.results: Dict[int, SolveResult] = {}
+        self.t0 = None
+        self.log = []
+
+    def clock(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def lane(self, req: SolveRequest) -> Lane:
+        key = req.lane_key
+        if key not in self.lanes:
+            cap = self.capacity.get(req.nelem_x, max(4, min(64, 2560 // max(req.nelem_x, 1))))
+            self.lanes[key] = Lane(self, key, cap, self.nt_max, self.rec_pitch)
+        return self.lanes[key]
+
+    # -------------------------------------------------------------- admission
+    def _admit(self, lane: Lane, req: SolveRequest, now: float) -> bool:
+        slot = lane.free_slot()
+        if slot is None:
+            return False
+        plan = lane.plan
+        fp = request_footprint(plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S)
+        if self.in_use + fp > self.budget:
+            return False
+        dev, E, S = self.device, lane.E, self.S
+        K = max(1, math.ceil(req.nt / S))
+        nrec = int(np.asarray(req.rec_xz).reshape(-1, 2).shape[0])
+        if nrec > lane.rec_pitch:
+            raise ValueError(f"request {req.rid}: {nrec} receivers exceed the lane pitch {lane.rec_pitch}")
+        with torch.cuda.stream(lane.stream):
+            z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
+            b = {"u0": z(E), "u1": z(E), "w": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E),
+                 "l0": z(E), "l1": z(E), "om": z(E), "G": z(E, dt=torch.float64),
+                 "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
+                 "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
+            if req.obs is not None:
+                b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float32, device=dev)
+            w = source_wavelet(req.nt, req.dt, req.f0)
+            b["wav"] = torch.as_tensor(w, dtype=torch.float32, device=dev)
+            base = None if req.base is None else torch.as_tensor(req.base, device=dev)
+            _, c2 = plan.bspline_map(torch.as_tensor(req.coef, dtype=torch.float64, device=dev)[None], base,
+                                     want_c=False)
+            L.check(plan.lib.sem_pad_nodal(plan.h, C.c_void_p(c2.data_ptr()), C.c_void_p(b["c2p"].data_ptr()), 1,
+                                           C.c_void_p(lane.stream.cuda_stream)))
+            b["c2_flat"] = c2
+        a = _Active(req=req, slot=slot, K=K, phase=L.SLOT_FORWARD, k=0, buf=b, footprint=fp, admitted=now)
+        lane.ensure_set_capacity()
+        a.rec_set = plan.add_receivers(req.rec_xz)
+        a.nrec = nrec
+        a.src_node = plan.source_node(*req.src_xz)
+        L.check(plan.lib.sem_receivers_sync(plan.h))
+        lane.slots[slot] = a
+        self.in_use += fp
+        return True
+
+    # -------------------------------------------------------------- transitions
+    def _after_round(self, lane: Lane):
+        plan = lane.plan
+        lib, h = plan.lib, plan.h
+        sp = C.c_void_p(lane.stream.cuda_stream)
+        finished_fwd = [a for a in lane.occupied() if a.phase == L.SLOT_FORWARD and a.k == a.K - 1]
+        with torch.cuda.stream(lane.stream):
+            if finished_fwd:
+                # final post-update samples for the slots that just completed their forward pass
+                L.check(lib.sem_slots_record_final(h, C.c_void_p(lane.table_dev.data_ptr()), lane.C,
+                                                   1 << L.SLOT_FORWARD, lane.rec_pitch, sp))
+                for a in finished_fwd:
+                    if a.k == a.K - 1:
+                        b = a.buf
+                        L.check(lib.sem_residual(h, C.c_void_p(b["seis"].data_ptr()), C.c_void_p(b["obs"].data_ptr()),
+                                                 C.c_void_p(b["res"].data_ptr()), C.c_void_p(b["misfit"].data_ptr()),
+                                                 a.nrec, a.req.nt, lane.rec_pitch, sp))
+            done = []
+            for a in lane.occupied():
+                if a.phase == L.SLOT_FORWARD:
+                    if a.k + 1 < a.K:
+                        a.k += 1
+                    else:
+                        a.phase, a.k = L.SLOT_RECOMPUTE, a.K - 1
+                elif a.phase == L.SLOT_RECOMPUTE:
+                    a.phase = L.SLOT_ADJOINT
+                elif a.phase == L.SLOT_ADJOINT:
+                    if a.k > 0:
+                        a.phase, a.k = L.SLOT_RECOMPUTE, a.k - 1
+                    else:
+                        done.append(a)
+            for a in done:
+                self._finish(lane, a)
+
+    def _finish(self, lane: Lane, a: _Active):
+        plan, b, req = lane.plan, a.buf, a.req
+        inv_s2 = 1.0 / (req.noise_std ** 2 + 1e-24)
+        gc = torch.empty(plan.N, dtype=torch.float64, device=self.device)
+        L.check(plan.lib.sem_gradient_finalize(plan.h, C.c_void_p(b["G"].data_ptr()), C.c_void_p(b["c2p"].data_ptr()),
+                                               inv_s2, C.c_void_p(gc.data_ptr()), None,
+                                               C.c_void_p(lane.stream.cuda_stream)))
+        nbz, nbx = np.asarray(req.coef).shape
+        g = plan.bspline_project(gc[None], nbz, nbx)[0]
+        lane.stream.synchronize()
+        mis = float(b["misfit"].item())
+        now = self.clock()
+        self.results[req.rid] = SolveResult(req.rid, -mis * inv_s2, mis, g.cpu().numpy(),
+                                            b["seis"][:, :a.nrec].cpu().numpy(), req.arrival, a.admitted, now)
+        lane.slots[a.slot] = None
+        self.in_use -= a.footprint
+
+    # -------------------------------------------------------------- driver
+    def run(self, requests: List[SolveRequest], progress=None, realtime: bool = True) -> Dict[int, SolveResult]:
+        """Serve `requests` (arrival times in seconds from start) to completion."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        self.t0 = time.perf_counter()
+        total = len(pending)
+        last_report = 0.0
+        while pending or any(l.occupied() or l.queue for l in self.lanes.values()):
+            now = self.clock()
+            while pending and (pending[0].arrival <= now or not realtime):
+                r = pending.popleft()
+                if not realtime:
+                    r.arrival = now
+                self.lane(r).queue.append(r)
+            progressed = False
+            for lane in list(self.lanes.values()):
+                if lane.event is not None:
+                    if not lane.event.query():
+                        continue
+                    lane.event = None
+                    self._after_round(lane)
+                    progressed = True
+                while lane.queue and self._admit(lane, lane.queue[0], self.clock()):
+                    lane.queue.popleft()
+                if lane.occupied():
+                    lane.launch_round(self.use_graph)
+                    progressed = True
+            if progress and self.clock() - last_report > 5.0:
+                last_report = self.clock()
+                progress(f"[serve] t={last_report:.1f}s done {len(self.results)}/{total} "
+                         f"resident {sum(len(l.occupied()) for l in self.lanes.values())} "
+                         f"hbm {self.in_use / 1e9:.1f} GB")
+            if not progressed:
+                time.sleep(0.0005)
+        torch.cuda.synchronize(self.device)
+        return self.results
+
+
+def run_plain(requests: List[SolveRequest], device=None, adjoint_mode="reference",
+              sides=("xmin", "xmax", "zmax"), segment=100, progress=None) -> Dict[int, SolveResult]:
+    """The plain path: one request at a time through the batch path at batch 1
+    (one launch sequence per solve, no overlap), in arrival order."""
+    dev = torch.device(device if device is not None else "cuda")
+    plans = {}
+    out = {}
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda q: q.arrival):
+        now = time.perf_counter() - t0
+        if now < r.arrival:
+            time.sleep(r.arrival - now)
+        key = r.lane_key + (r.nt,)
+        if key not in plans:
+            plans[key] = Plan(r.nelem_x, r.nelem_z, 0.0, r.xmax, 0.0, r.zmax, r.dt, r.nt, 4, r.sponge_width,
+                              r.sponge_cmax, 1e-8, sides, precision="fp32", adjoint_mode=adjoint_mode,
+                              checkpoint_every=segment, device=dev)
+        p = plans[key]
+        start = time.perf_counter() - t0
+        rs = p.add_receivers(r.rec_xz)
+        _, c2 = p.bspline_map(torch.as_tensor(r.coef, dtype=torch.float64)[None],
+                              None if r.base is None else torch.as_tensor(r.base))
+        w = torch.as_tensor(source_wavelet(r.nt, r.dt, r.f0), dtype=torch.float32, device=dev)
+        o = p.forward_adjoint(c2, w, [p.source_node(*r.src_xz)], torch.as_tensor(r.obs, device=dev)[None],
+                              r.noise_std, [rs])
+        nbz, nbx = np.asarray(r.coef).shape
+        g = p.bspline_project(o["grad_c"], nbz, nbx)[0]
+        torch.cuda.synchronize(dev)
+        fin = time.perf_counter() - t0
+        nrec = p.rec_sizes[rs]
+        out[r.rid] = SolveResult(r.rid, float(o["loglik"][0]), float(o["misfit"][0]), g.cpu().numpy(),
+                                 o["seis"][0, :, :nrec].cpu().numpy(), r.arrival, start, fin)
+        if progress and len(out) % 8 == 0:
+            progress(f"[plain] {len(out)}/{len(requests)} t={fin:.1f}s")
+    return out
+
+
+def latency_stats(results: Dict[int, SolveResult]):
+    lat = np.array([r.latency for r in results.values()])
+    span = max(r.finished for r in results.values()) - min(r.arrival for r in results.values())
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "solves_per_s": len(results) / span, "makespan_s": span,
+            "n": len(results)}

@@ -198,3 +198,28 @@ def served_stream(nreq: int, rate_hz: float, seed: int = 5, nt_scale: float = 1.
 
 
 WORKLOADS = {"cpu40": cpu40, "single200": single200, "batch200": batch200, "elbo120": elbo120}
+
+
+def served_solve_requests(nreq: int, rate_hz: float, seed: int = 5, nt_scale: float = 1.0, device="cuda",
+                          noise_std: float = 1e-9):
+    """The served stream as solver requests, with observations generated by a
+    forward solve of each request's perturbed ('true') model."""
+    import torch
+    from .serving import SolveRequest
+    from .solver import Plan, source_wavelet
+    reqs = []
+    plans = {}
+    for r in served_stream(nreq, rate_hz, seed, nt_scale):
+        wl, m = r.workload, r.workload.mesh
+        key = (m.nelem_x, m.nt)
+        if key not in plans:
+            plans[key] = Plan.from_mesh(m, precision="fp32", device=device)
+        p = plans[key]
+        rs = p.add_receivers(wl.receivers)
+        _, c2 = p.bspline_map(torch.as_tensor(wl.true_coef)[None])
+        w = torch.as_tensor(source_wavelet(m.nt, m.dt, wl.f0), dtype=torch.float32, device=device)
+        obs = p.forward(c2, w, [p.source_node(*wl.sources[0])], [rs])[0].cpu().numpy()
+        reqs.append(SolveRequest(r.rid, m.nelem_x, m.nelem_z, m.xmax, m.zmax, m.dt, m.nt, wl.coef[0],
+                                 tuple(wl.sources[0]), wl.f0, wl.receivers, obs, noise_std, m.sponge_width,
+                                 m.sponge_cmax, arrival=r.arrival))
+    return reqs

@@ -1,0 +1,88 @@
+"""Continuous batching: each result is the same alone and in the mix.
+
+A stream of uneven requests (three mesh sizes, different step counts,
+sources, receiver counts and B-spline grids) is served with overlapping
+admission; every result is compared with the same request served alone,
+with the plain one-at-a-time batch path, and one request with the FP64
+oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tests.gpu_helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def make_requests(n=10, seed=0):
+    from mun_amd.serving import SolveRequest
+    from mun_amd.solver import Plan, source_wavelet
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        nel = int(rng.choice([8, 12, 20]))
+        Lx = 25.0 * nel
+        nt = int(rng.integers(150, 420))
+        nb = int(rng.integers(5, 9))
+        coef = rng.uniform(1800.0, 3000.0, (nb, nb))
+        true = coef + rng.normal(0.0, 60.0, (nb, nb))
+        nrec = int(rng.integers(5, 30))
+        rec = np.stack([np.linspace(20.0, Lx - 20.0, nrec), np.full(nrec, 2.0)], axis=1)
+        src = (float(rng.uniform(0.3, 0.7) * Lx), 30.0)
+        # observations from the true model, forward path
+        p = Plan(nel, nel, 0.0, Lx, 0.0, Lx, 4e-4, nt, 4, 60.0, 3500.0, 1e-8, ("xmin", "xmax", "zmax"),
+                 precision="fp32", checkpoint_every=100)
+        rs = p.add_receivers(rec)
+        _, c2 = p.bspline_map(torch.as_tensor(true)[None])
+        w = torch.as_tensor(source_wavelet(nt, 4e-4, 12.0), dtype=torch.float32, device="cuda")
+        obs = p.forward(c2, w, [p.source_node(*src)], [rs])[0].cpu().numpy()
+        reqs.append(SolveRequest(i, nel, nel, Lx, Lx, 4e-4, nt, coef, src, 12.0, rec, obs, 1e-8, 60.0, 3500.0,
+                                 arrival=0.002 * i))
+    return reqs
+
+
+def test_served_results_equal_alone_in_mix_and_plain():
+    from mun_amd.serving import SEMServer, latency_stats, run_plain
+    print("[serve] building requests", flush=True)
+    reqs = make_requests()
+    print("[serve] mix", flush=True)
+    srv = SEMServer(hbm_budget_bytes=8e9, segment=50, capacity={8: 3, 12: 2, 20: 2}, use_graph=True,
+                    rec_pitch=32, nt_max=420)
+    mix = srv.run(reqs)
+    assert len(mix) == len(reqs)
+    assert max(l.rounds for l in srv.lanes.values()) > 1
+    # the stream really overlapped: some request was admitted while another was resident
+    adm = sorted((r.admitted, r.finished) for r in mix.values())
+    assert any(adm[i + 1][0] < adm[i][1] for i in range(len(adm) - 1))
+    print("[serve] alone", flush=True)
+    for r in reqs[:4]:
+        solo = SEMServer(hbm_budget_bytes=8e9, segment=50, use_graph=False, rec_pitch=32, nt_max=420).run([r])
+        a, b = solo[r.rid], mix[r.rid]
+        assert a.loglik == b.loglik
+        np.testing.assert_array_equal(a.grad, b.grad)
+        np.testing.assert_array_equal(a.seis, b.seis)
+    print("[serve] plain", flush=True)
+    plain = run_plain(reqs, segment=50)
+    for rid, p in plain.items():
+        m = mix[rid]
+        assert abs(p.loglik - m.loglik) <= 1e-6 * abs(p.loglik)
+        assert rel_l2(m.grad, p.grad) < 1e-6
+    st = latency_stats(mix)
+    print("  served", st, "\n  plain", latency_stats(plain))
+
+
+def test_served_request_matches_oracle():
+    from mun_amd.serving import SEMServer
+    r = make_requests(1, seed=3)[0]
+    print(f"[serve-oracle] nel={r.nelem_x} nt={r.nt}", flush=True)
+    res = SEMServer(hbm_budget_bytes=4e9, segment=64, use_graph=True, rec_pitch=32, nt_max=500).run([r])[r.rid]
+    prob = O.Problem(0.0, r.xmax, 0.0, r.zmax, r.nelem_x, r.nelem_z, 4, r.dt, r.nt, r.src_xz, r.f0, r.rec_xz,
+                     sponge_width=r.sponge_width, sponge_cmax=r.sponge_cmax, sponge_sides=("xmin", "xmax", "zmax"))
+    d = O.discretize(prob)
+    c, Bz, Bx = O.tensor_bspline_velocity(d.mesh, r.coef)
+    ref = O.forward_adjoint(d, c, r.obs, r.noise_std, mode="reference")
+    gref = Bz.T @ ref["dloglik_dc"].reshape(d.mesh.NZ, d.mesh.NX) @ Bx
+    assert rel_l2(res.seis, ref["seis"]) < 1e-5
+    assert rel_l2(res.grad, gref) < 1e-4
+    assert abs(res.loglik - ref["loglik"]) < 1e-4 * abs(ref["loglik"])

@@ -49,6 +49,18 @@ constexpr int NXCD = 8;           // XCDs of an MI355X
 
 enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2 };
 
+// Global-memory qualified pointers: lets the compiler emit global_* instead of
+// flat_* instructions for pointers that come from the slot table.
+#define GMEM __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GMEM T* gm(T* p) {
+    return (GMEM T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const GMEM T* gm(const T* p) {
+    return (const GMEM T*)p;
+}
+
 template <typename T, int P>
 struct ElemConst {
     T D[P + 1][P + 1];   // D[a][k] = l_k'(xi_a): derivative at node a
@@ -109,17 +121,17 @@ struct StepArgs {
 // Per-workgroup pointers and indices of one solve at one step.
 template <typename T>
 struct View {
-    const T* u_in;
-    T* u_out;
-    T* w;
-    const T* c2;
-    const T* uf;
-    double* G;
-    T* ck_u;
-    T* ck_w;
-    T* seis;
-    const T* resid;
-    const T* wav;  // wavelet sample of this step (null: no source)
+    const GMEM T* u_in;
+    GMEM T* u_out;
+    GMEM T* w;
+    const GMEM T* c2;
+    const GMEM T* uf;
+    GMEM double* G;
+    GMEM T* ck_u;
+    GMEM T* ck_w;
+    GMEM T* seis;
+    const GMEM T* resid;
+    const GMEM T* wav;  // wavelet sample of this step (null: no source)
     int src, set, rec_t, res_t;
 };
 
@@ -129,17 +141,17 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     if (!a.slots) {
         const long soff = b * a.sb;
         const long mdl = a.model_of ? a.model_of[b] : b;
-        v.u_in = a.u_in + soff;
-        v.u_out = a.u_out + soff;
-        v.w = a.w + soff;
-        v.c2 = a.c2 + mdl * a.sb;
-        v.uf = a.uf ? a.uf + soff : nullptr;
-        v.G = a.G ? a.G + soff : nullptr;
-        v.ck_u = a.ck_u ? a.ck_u + soff : nullptr;
-        v.ck_w = a.ck_w ? a.ck_w + soff : nullptr;
-        v.seis = a.seis ? a.seis + b * a.seis_bs : nullptr;
-        v.resid = a.resid ? a.resid + b * a.resid_bs : nullptr;
-        v.wav = (MODE == MODE_FWD && a.src_node) ? a.wavelet + b * a.wav_bs + a.n : nullptr;
+        v.u_in = gm(a.u_in + soff);
+        v.u_out = gm(a.u_out + soff);
+        v.w = gm(a.w + soff);
+        v.c2 = gm(a.c2 + mdl * a.sb);
+        v.uf = a.uf ? gm(a.uf + soff) : nullptr;
+        v.G = a.G ? gm(a.G + soff) : nullptr;
+        v.ck_u = a.ck_u ? gm(a.ck_u + soff) : nullptr;
+        v.ck_w = a.ck_w ? gm(a.ck_w + soff) : nullptr;
+        v.seis = a.seis ? gm(a.seis + b * a.seis_bs) : nullptr;
+        v.resid = a.resid ? gm(a.resid + b * a.resid_bs) : nullptr;
+        v.wav = (MODE == MODE_FWD && a.src_node) ? gm(a.wavelet + b * a.wav_bs + a.n) : nullptr;
         v.src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
         v.set = a.rec_set_of ? a.rec_set_of[b] : 0;
         v.rec_t = a.rec_t;
@@ -148,11 +160,11 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     }
     const sem_slot& d = a.slots[b];
     if (!((a.phase_mask >> d.phase) & 1u)) return false;
-    v.c2 = (const T*)d.c2;
+    v.c2 = gm((const T*)d.c2);
     v.set = d.rec_set;
-    v.G = d.G;
-    v.seis = (T*)d.seis;
-    v.resid = (const T*)d.resid;
+    v.G = gm(d.G);
+    v.seis = gm((T*)d.seis);
+    v.resid = gm((const T*)d.resid);
     v.ck_u = v.ck_w = nullptr;
     v.uf = nullptr;
     v.wav = nullptr;
@@ -160,7 +172,7 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     v.rec_t = v.res_t = -1;
     if (MODE == MODE_REC) {  // final post-update sample of a forward slot
         const int n = d.n0 + d.len;
-        v.u_in = (const T*)((n & 1) ? d.u1 : d.u0);
+        v.u_in = gm((const T*)((n & 1) ? d.u1 : d.u0));
         v.u_out = nullptr;
         v.w = nullptr;
         v.rec_t = n - 1;
@@ -169,58 +181,49 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     if (a.m >= d.len) return false;
     if (d.phase == SEM_SLOT_FORWARD || d.phase == SEM_SLOT_RECOMPUTE) {
         const int n = d.n0 + a.m;
-        v.wav = (const T*)d.wavelet + n;
+        v.wav = gm((const T*)d.wavelet + n);
         v.src = d.src_node;
         if (d.phase == SEM_SLOT_FORWARD) {
-            v.u_in = (const T*)((n & 1) ? d.u1 : d.u0);
-            v.u_out = (T*)((n & 1) ? d.u0 : d.u1);
-            v.w = (T*)d.w;
+            v.u_in = gm((const T*)((n & 1) ? d.u1 : d.u0));
+            v.u_out = gm((T*)((n & 1) ? d.u0 : d.u1));
+            v.w = gm((T*)d.w);
             v.rec_t = d.record ? (a.post ? n - 1 : n) : -1;
             if (a.m == d.len - 1 && d.ck_u) {
-                v.ck_u = (T*)d.ck_u;
-                v.ck_w = (T*)d.ck_w;
+                v.ck_u = gm((T*)d.ck_u);
+                v.ck_w = gm((T*)d.ck_w);
             }
         } else {
-            v.u_in = (a.m == 0) ? (const T*)d.u0 : (const T*)d.seg + (long)a.m * d.seg_stride;
-            v.u_out = (T*)d.seg + (long)(a.m + 1) * d.seg_stride;
-            v.w = (T*)d.w;
+            v.u_in = gm((a.m == 0) ? (const T*)d.u0 : (const T*)d.seg + (long)a.m * d.seg_stride);
+            v.u_out = gm((T*)d.seg + (long)(a.m + 1) * d.seg_stride);
+            v.w = gm((T*)d.w);
         }
         return MODE == MODE_FWD;
     }
     // adjoint: lambda_it in u[it & 1], lambda_{it-1} into the other
     const int it = d.n0 + d.len - 1 - a.m;
-    v.u_in = (const T*)((it & 1) ? d.u1 : d.u0);
-    v.u_out = (T*)((it & 1) ? d.u0 : d.u1);
-    v.w = (T*)d.w;
-    v.uf = (const T*)d.seg + (long)(it - d.n0 + 1) * d.seg_stride;
+    v.u_in = gm((const T*)((it & 1) ? d.u1 : d.u0));
+    v.u_out = gm((T*)((it & 1) ? d.u0 : d.u1));
+    v.w = gm((T*)d.w);
+    v.uf = gm((const T*)d.seg + (long)(it - d.n0 + 1) * d.seg_stride);
     v.res_t = it;
     return MODE == MODE_ADJ;
 }
 
-// 16-byte vector of T
-template <typename T>
-struct V16;
-template <>
-struct V16<float> {
-    using type = float4;
-};
-template <>
-struct V16<double> {
-    using type = double2;
-};
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
 
 // five consecutive values starting at a 16-byte aligned address
 template <typename T>
-__device__ __forceinline__ void load5(const T* __restrict__ p, T (&v)[5]) {
+__device__ __forceinline__ void load5(const GMEM T* __restrict__ p, T (&v)[5]) {
     if constexpr (std::is_same_v<T, float>) {
-        const float4 a = *reinterpret_cast<const float4*>(p);
+        const vf4 a = *reinterpret_cast<const GMEM vf4*>(p);
         v[0] = a.x;
         v[1] = a.y;
         v[2] = a.z;
         v[3] = a.w;
     } else {
-        const double2 a = *reinterpret_cast<const double2*>(p);
-        const double2 c = *reinterpret_cast<const double2*>(p + 2);
+        const vd2 a = *reinterpret_cast<const GMEM vd2*>(p);
+        const vd2 c = *reinterpret_cast<const GMEM vd2*>(p + 2);
         v[0] = a.x;
         v[1] = a.y;
         v[2] = c.x;
@@ -230,16 +233,16 @@ __device__ __forceinline__ void load5(const T* __restrict__ p, T (&v)[5]) {
 }
 
 template <typename T>
-__device__ __forceinline__ void load4(const T* __restrict__ p, T (&v)[4]) {
+__device__ __forceinline__ void load4(const GMEM T* __restrict__ p, T (&v)[4]) {
     if constexpr (std::is_same_v<T, float>) {
-        const float4 a = *reinterpret_cast<const float4*>(p);
+        const vf4 a = *reinterpret_cast<const GMEM vf4*>(p);
         v[0] = a.x;
         v[1] = a.y;
         v[2] = a.z;
         v[3] = a.w;
     } else {
-        const double2 a = *reinterpret_cast<const double2*>(p);
-        const double2 c = *reinterpret_cast<const double2*>(p + 2);
+        const vd2 a = *reinterpret_cast<const GMEM vd2*>(p);
+        const vd2 c = *reinterpret_cast<const GMEM vd2*>(p + 2);
         v[0] = a.x;
         v[1] = a.y;
         v[2] = c.x;
@@ -248,12 +251,47 @@ __device__ __forceinline__ void load4(const T* __restrict__ p, T (&v)[4]) {
 }
 
 template <typename T>
-__device__ __forceinline__ void store4(T* __restrict__ p, const T (&v)[4]) {
+__device__ __forceinline__ void store4(GMEM T* __restrict__ p, const T (&v)[4]) {
     if constexpr (std::is_same_v<T, float>) {
-        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+        const vf4 x = {v[0], v[1], v[2], v[3]};
+        *reinterpret_cast<GMEM vf4*>(p) = x;
     } else {
-        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+        const vd2 x = {v[0], v[1]}, y = {v[2], v[3]};
+        *reinterpret_cast<GMEM vd2*>(p) = x;
+        *reinterpret_cast<GMEM vd2*>(p + 2) = y;
+    }
+}
+
+// Non-temporal variants for streams touched once per step (the increment and
+// the new state): they skip cache allocation, which keeps the halo-shared
+// inputs (u, c^2) resident and lets reads and writes stream.
+template <typename T>
+__device__ __forceinline__ void store4nt(GMEM T* __restrict__ p, const T (&v)[4]) {
+    if constexpr (std::is_same_v<T, float>) {
+        const vf4 x = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(x, reinterpret_cast<GMEM vf4*>(p));
+    } else {
+        const vd2 x = {v[0], v[1]}, y = {v[2], v[3]};
+        __builtin_nontemporal_store(x, reinterpret_cast<GMEM vd2*>(p));
+        __builtin_nontemporal_store(y, reinterpret_cast<GMEM vd2*>(p + 2));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load4nt(const GMEM T* __restrict__ p, T (&v)[4]) {
+    if constexpr (std::is_same_v<T, float>) {
+        const vf4 a = __builtin_nontemporal_load(reinterpret_cast<const GMEM vf4*>(p));
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+    } else {
+        const vd2 a = __builtin_nontemporal_load(reinterpret_cast<const GMEM vd2*>(p));
+        const vd2 c = __builtin_nontemporal_load(reinterpret_cast<const GMEM vd2*>(p + 2));
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = c.x;
+        v[3] = c.y;
     }
 }
 
@@ -354,7 +392,7 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         for (int k = beg; k < end; ++k) {
             const RecEntry<T, P>& e = a.recs[k];
             if (!(e.owned && e.ex == ex && e.ez == ez)) continue;
-            const T* __restrict__ uin = V.u_in + base;
+            const GMEM T* __restrict__ uin = V.u_in + base;
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
@@ -374,8 +412,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         T wv[P][P];
         T dxv[P], dzv[P];
         if (valid) {
-            const T* __restrict__ uin = V.u_in + base;
-            const T* __restrict__ c2p = V.c2 + base;
+            const GMEM T* __restrict__ uin = V.u_in + base;
+            const GMEM T* __restrict__ c2p = V.c2 + base;
             T c2[NP][NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
@@ -383,17 +421,17 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 load5(c2p + j * pitch, c2[j]);
             }
             if (owned) {
-                const T* __restrict__ wq = V.w + base;
+                const GMEM T* __restrict__ wq = V.w + base;
 #pragma unroll
-                for (int j = 0; j < P; ++j) load4(wq + j * pitch, wv[j]);
+                for (int j = 0; j < P; ++j) load4nt(wq + j * pitch, wv[j]);
                 if (a.damp) {
-                    load4(a.Dx + P * ex, dxv);
-                    load4(a.Dz + P * ez, dzv);
+                    load4(gm(a.Dx) + P * ex, dxv);
+                    load4(gm(a.Dz) + P * ez, dzv);
                 }
             }
             T v[ADJ ? NP : 1][ADJ ? NP : 1];
             if constexpr (ADJ) {
-                const T* __restrict__ ufp = V.uf + base;
+                const GMEM T* __restrict__ ufp = V.uf + base;
 #pragma unroll
                 for (int j = 0; j < NP; ++j) load5(ufp + j * pitch, v[j]);
                 element_action<T, P, true>(a.ec, u, c2, v, r, g);
@@ -496,8 +534,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
             }
         }
 
-        T* __restrict__ wp = V.w + base;
-        T* __restrict__ uo = V.u_out + base;
+        GMEM T* __restrict__ wp = V.w + base;
+        GMEM T* __restrict__ uo = V.u_out + base;
         const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
         // dt^2 / (Wx Wz) from the GLL weights: shared element edges carry w_0 + w_P,
         // domain edges w_0 alone
@@ -505,8 +543,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
         auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
         auto damp_at = [&](int j, int i) -> T {
             if (!a.damp) return T(1);
-            const T dxi = (i < P) ? dxv[i] : a.Dx[P * ex + i];
-            const T dzj = (j < P) ? dzv[j] : a.Dz[P * ez + j];
+            const T dxi = (i < P) ? dxv[i] : gm(a.Dx)[P * ex + i];
+            const T dzj = (j < P) ? dzv[j] : gm(a.Dz)[P * ez + j];
             return dxi * dzj;
         };
         auto update = [&](int j, int i, T wold) -> T {  // returns w^{n+1}; leaves u^{n+1} in r[j][i]
@@ -527,8 +565,8 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 wn[i] = update(j, i, wv[j][i]);
                 un[i] = r[j][i];
             }
-            store4(wp + j * pitch, wn);
-            store4(uo + j * pitch, un);
+            store4nt(wp + j * pitch, wn);
+            store4nt(uo + j * pitch, un);
             if constexpr (MODE == MODE_FWD) {
                 if (V.ck_u) {
                     store4(V.ck_w + base + j * pitch, wn);
@@ -536,12 +574,12 @@ __global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) 
                 }
             }
             if constexpr (ADJ) {
-                double* Gp = V.G + base + j * pitch;
+                GMEM double* Gp = V.G + base + j * pitch;
                 double gv[4];
-                load4(Gp, gv);
+                load4nt(Gp, gv);
 #pragma unroll
                 for (int i = 0; i < P; ++i) gv[i] += (double)g[j][i];
-                store4(Gp, gv);
+                store4nt(Gp, gv);
             }
         }
         // domain edges: right column (i = P) of the last element column, top row (j = P)

@@ -76,6 +76,8 @@ typedef struct {
     int checkpoint_every;      /* steps per wavefield checkpoint; <=0 or >=nt:
                                   keep the full forward history instead           */
     double lagrange_eps;       /* +eps in Lagrange denominators (utils.py:36), 1e-12 */
+    int tile_rows;             /* step-kernel tile height in element rows incl. the
+                                  halo row: 15, 7 or 3 (0 = 15); see sem_autotune  */
 } sem_desc;
 
 /* One batch of independent solves on a plan's mesh.  Solve b uses model

@@ -32,7 +32,8 @@ class SemDesc(C.Structure):
                 ("dt", C.c_double), ("nt", C.c_int),
                 ("sponge_width", C.c_double), ("sponge_cmax", C.c_double), ("sponge_R", C.c_double),
                 ("sponge_sides", C.c_uint), ("precision", C.c_int), ("adjoint_mode", C.c_int),
-                ("record_mode", C.c_int), ("checkpoint_every", C.c_int), ("lagrange_eps", C.c_double)]
+                ("record_mode", C.c_int), ("checkpoint_every", C.c_int), ("lagrange_eps", C.c_double),
+                ("tile_rows", C.c_int)]
 
 
 class SemBatch(C.Structure):

@@ -33,8 +33,14 @@ class BatchSolver:
         self.base = None if base is None else torch.as_tensor(base, dtype=torch.float64, device=self.dev)
 
     @classmethod
-    def from_workload(cls, wl, **plan_kw):
-        plan = Plan.from_mesh(wl.mesh, **plan_kw)
+    def from_workload(cls, wl, tile_rows="auto", **plan_kw):
+        """Solver for a workload; tile_rows='auto' asks the autotuner for the
+        fastest tile height at this mesh and batch (models x shots)."""
+        if tile_rows == "auto":
+            from .autotune import best_tile_rows
+            tile_rows = best_tile_rows(wl.mesh.nelem_x, wl.mesh.nelem_z, wl.nmodel * wl.nshot,
+                                       plan_kw.get("precision", "fp32"), plan_kw.get("device"))
+        plan = Plan.from_mesh(wl.mesh, tile_rows=int(tile_rows), **plan_kw)
         nbz, nbx = wl.coef.shape[1:]
         return cls(plan, wl.sources, wl.receivers, wl.f0, nbz, nbx, wl.base)
 

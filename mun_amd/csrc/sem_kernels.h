@@ -41,10 +41,11 @@
 namespace sem {
 
 constexpr int EX = 17;            // element threads along x per workgroup
-constexpr int EZ = 15;            // element threads along z per workgroup
 constexpr int TX = EX - 1;        // owned elements along x per tile (64 nodes)
-constexpr int TZ = EZ - 1;        // owned elements along z per tile
-constexpr int NTHREADS = 256;     // EX * EZ = 255 element threads + 1 idle
+// Tile heights (element rows incl. the halo row) with a kernel instance:
+// 15 rows -> 256-thread workgroups, 7 -> 128, 3 -> 64.
+constexpr int EZ_DEFAULT = 15;
+__host__ __device__ constexpr int tile_threads(int ez) { return (EX * ez + 63) / 64 * 64; }
 constexpr int NXCD = 8;           // XCDs of an MI355X
 
 enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2 };
@@ -354,8 +355,9 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
     }
 }
 
-template <typename T, int P, int MODE>
-__global__ __launch_bounds__(NTHREADS) void step_kernel(const StepArgs<T, P> a) {
+template <typename T, int P, int MODE, int EZ>
+__global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T, P> a) {
+    constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
     constexpr bool ADJ = (MODE == MODE_ADJ);

@@ -220,6 +220,8 @@ struct sem_plan {
     int pitch = 0;
     double dx = 0, dz = 0, J = 0;
     int ntx = 0, ntz = 0, ntiles = 0;
+    int ez = EZ_DEFAULT;  // tile height in element rows incl. halo (15, 7 or 3)
+    int tz = EZ_DEFAULT - 1;
     std::vector<double> xi, wq, Dm, x1d, z1d;
     std::vector<double> Dxv, Dzv;
     DevBuf d_Dx, d_Dz, d_x1d, d_z1d;
@@ -261,6 +263,7 @@ int upload_receivers(sem_plan* pl) {
         std::vector<std::vector<const RecHost*>> by_tile(pl->ntiles);
         for (auto& e : s) {
             // tiles whose extended element block [tx*TX-1, tx*TX+TX-1] holds e
+            const int TZ = pl->tz;
             for (int tz = std::max(0, e.ez / TZ - 1); tz <= std::min(pl->ntz - 1, e.ez / TZ + 1); ++tz)
                 for (int tx = std::max(0, e.ex / TX - 1); tx <= std::min(pl->ntx - 1, e.ex / TX + 1); ++tx) {
                     if (e.ex < tx * TX - 1 || e.ex > tx * TX + TX - 1) continue;
@@ -276,7 +279,7 @@ int upload_receivers(sem_plan* pl) {
                 r.ex = e->ex;
                 r.ez = e->ez;
                 r.r = e->r;
-                r.owned = (e->ex / TX == tx && e->ez / TZ == tz) ? 1 : 0;
+                r.owned = (e->ex / TX == tx && e->ez / pl->tz == tz) ? 1 : 0;
                 for (int q = 0; q < 5; ++q) {
                     r.lx[q] = (T)e->lx[q];
                     r.lz[q] = (T)e->lz[q];
@@ -415,7 +418,13 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
 
 template <typename T, int MODE>
 int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
-    step_kernel<T, 4, MODE><<<a.nper * NXCD, NTHREADS, 0, st>>>(a);
+    const unsigned grid = (unsigned)a.nper * NXCD;
+    switch (pl->ez) {
+        case 15: step_kernel<T, 4, MODE, 15><<<grid, tile_threads(15), 0, st>>>(a); break;
+        case 7: step_kernel<T, 4, MODE, 7><<<grid, tile_threads(7), 0, st>>>(a); break;
+        case 3: step_kernel<T, 4, MODE, 3><<<grid, tile_threads(3), 0, st>>>(a); break;
+        default: return fail(SEM_E_UNSUPPORTED, "tile height without a kernel instance");
+    }
     LAUNCH_CHECK();
     return SEM_OK;
 }
@@ -658,6 +667,8 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     if (d->record_mode != SEM_RECORD_POST && d->record_mode != SEM_RECORD_PRE)
         return fail(SEM_E_INVALID, "bad record_mode");
     if (d->sponge_sides && !(d->sponge_width > 0)) return fail(SEM_E_INVALID, "sponge_width must be > 0");
+    if (d->tile_rows != 0 && d->tile_rows != 15 && d->tile_rows != 7 && d->tile_rows != 3)
+        return fail(SEM_E_UNSUPPORTED, "tile_rows must be 0 (default), 15, 7 or 3");
     auto* pl = new sem_plan;
     pl->d = *d;
     const int P = 4;
@@ -669,8 +680,10 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     pl->dx = (d->xmax - d->xmin) / d->nelem_x;
     pl->dz = (d->zmax - d->zmin) / d->nelem_z;
     pl->J = pl->dx * pl->dz * 0.25;
+    pl->ez = d->tile_rows > 0 ? d->tile_rows : EZ_DEFAULT;
+    pl->tz = pl->ez - 1;
     pl->ntx = (d->nelem_x + TX - 1) / TX;
-    pl->ntz = (d->nelem_z + TZ - 1) / TZ;
+    pl->ntz = (d->nelem_z + pl->tz - 1) / pl->tz;
     pl->ntiles = pl->ntx * pl->ntz;
     pl->xi = gll_nodes(P);
     pl->wq = gll_weights(pl->xi);

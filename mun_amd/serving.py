@@ -114,9 +114,14 @@ class Lane:
         self.C = capacity
         self.S = server.S
         self.rec_pitch = rec_pitch
+        tile_rows = 15
+        if server.autotune:
+            from .autotune import best_tile_rows
+            tile_rows = best_tile_rows(nex, nez, capacity, "fp32", server.device)
+        self.tile_rows = tile_rows
         self.plan = Plan(nex, nez, 0.0, xmax, 0.0, zmax, dt, nt_max, 4, sw, scm, 1e-8, server.sides,
                          precision="fp32", adjoint_mode=server.adjoint_mode, record="post",
-                         checkpoint_every=self.S, device=server.device)
+                         checkpoint_every=self.S, device=server.device, tile_rows=tile_rows)
         pitch, elems = C.c_int(), C.c_long()
         L.check(self.plan.lib.sem_plan_layout(self.plan.h, C.byref(pitch), C.byref(elems)))
         self.E = elems.value
@@ -212,7 +217,8 @@ class SEMServer:
 
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
-                 device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512):
+                 device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
+                 autotune: bool = True):
         self.device = torch.device(device if device is not None else "cuda")
         self.budget = float(hbm_budget_bytes)
         self.S = int(segment)
@@ -223,6 +229,7 @@ class SEMServer:
         self.rec_pitch = int(rec_pitch)
         self.nt_max = int(nt_max)
         self.max_sets = int(max_sets)
+        self.autotune = bool(autotune)
         self.lanes: Dict[tuple, Lane] = {}
         self.in_use = 0
         self.results: Dict[int, SolveResult] = {}

@@ -57,7 +57,8 @@ class Plan:
                  dt: float, nt: int, order: int = 4, sponge_width: float = 200.0, sponge_cmax: float = 3000.0,
                  sponge_R: float = 1e-8, sponge_sides: Sequence[str] = ("xmin", "xmax", "zmin", "zmax"),
                  precision: str = "fp32", adjoint_mode: str = "reference", record: str = "post",
-                 checkpoint_every: Optional[int] = 100, lagrange_eps: float = 1e-12, device=None):
+                 checkpoint_every: Optional[int] = 100, lagrange_eps: float = 1e-12, device=None,
+                 tile_rows: int = 0):
         if not torch.cuda.is_available():
             raise RuntimeError("mun_amd.Plan needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device if device is not None else "cuda")
@@ -68,9 +69,10 @@ class Plan:
             sides |= L.SIDE_BITS[s]
         d = L.SemDesc(nelem_x, nelem_z, order, xmin, xmax, zmin, zmax, dt, nt, sponge_width, sponge_cmax,
                       sponge_R, sides, code, ADJ_MODES[adjoint_mode], RECORD[record],
-                      0 if checkpoint_every is None else int(checkpoint_every), lagrange_eps)
+                      0 if checkpoint_every is None else int(checkpoint_every), lagrange_eps, int(tile_rows))
         self.desc = d
         self.precision, self.adjoint_mode, self.record = precision, adjoint_mode, record
+        self.tile_rows = int(tile_rows) or 15
         self.nt, self.dt = int(nt), float(dt)
         h = C.c_void_p()
         with torch.cuda.device(self.device):

@@ -26,11 +26,12 @@ def small_problem(nex=20, nez=17, nt=400, dt=1e-3, sides=("xmin", "xmax", "zmax"
     return prob, disc, coef, c
 
 
-def make_plan(prob, precision="fp64", adjoint_mode="reference", record="post", checkpoint_every=None):
+def make_plan(prob, precision="fp64", adjoint_mode="reference", record="post", checkpoint_every=None,
+              tile_rows=0):
     from mun_amd.solver import Plan
     return Plan(prob.nelem_x, prob.nelem_z, prob.xmin, prob.xmax, prob.zmin, prob.zmax, prob.dt, prob.nt, 4,
                 prob.sponge_width, prob.sponge_cmax, prob.sponge_R, prob.sponge_sides, precision=precision,
-                adjoint_mode=adjoint_mode, record=record, checkpoint_every=checkpoint_every)
+                adjoint_mode=adjoint_mode, record=record, checkpoint_every=checkpoint_every, tile_rows=tile_rows)
 
 
 def gpu_setup(prob, disc, c, plan):

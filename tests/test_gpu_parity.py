@@ -116,3 +116,32 @@ def test_bspline_map_and_project(prob_small):
     ref = Bz.T @ g.reshape(disc.mesh.NZ, disc.mesh.NX) @ Bx
     got = plan.bspline_project(torch.as_tensor(g)[None], *coef.shape)[0].cpu().numpy()
     assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("tile_rows", [15, 7, 3])
+def test_every_tile_height_matches_oracle(prob_small, tile_rows):
+    """Each step-kernel instance (tile heights 15, 7, 3) reproduces the
+    oracle's seismograms and exact-mode gradient in FP64."""
+    prob, disc, coef, c, sp, _ = prob_small
+    print(f"[tile_rows={tile_rows}]", flush=True)
+    ref = O.forward_adjoint(disc, c, sp * 0.8, 1e-9, mode="exact")
+    plan = make_plan(prob, "fp64", adjoint_mode="exact", checkpoint_every=45, tile_rows=tile_rows)
+    rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
+    obs = torch.as_tensor(sp * 0.8, dtype=plan.dtype, device=plan.device)[None]
+    out = plan.forward_adjoint(c2, w, [src], obs, 1e-9, [rs])
+    torch.cuda.synchronize()
+    assert rel_l2(out["seis"][0].cpu().numpy(), ref["seis"]) < 1e-9
+    assert rel_l2(out["grad_c"][0].cpu().numpy(), ref["dloglik_dc"]) < 1e-9
+
+
+def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
+    from mun_amd import autotune
+    monkeypatch.setenv("MUN_AMD_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    autotune._MEM.clear()
+    rep = {}
+    best = autotune.best_tile_rows(20, 20, 8, "fp32", steps=10, report=rep)
+    assert best in autotune.TILE_ROWS
+    assert set(rep["step_us"]) == {"15", "7", "3"}
+    print("  20x20 batch 8:", rep)
+    autotune._MEM.clear()
+    assert autotune.best_tile_rows(20, 20, 8, "fp32", steps=10) == best  # from the file cache

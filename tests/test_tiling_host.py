@@ -1,7 +1,8 @@
 """Host emulation of the step kernel's work decomposition.
 
 Restates, in Python, the index arithmetic of ``step_kernel`` in
-mun_amd/csrc/sem_kernels.h (tile -> element threads, halo column/row,
+mun_amd/csrc/sem_kernels.h (17-wide tiles of 15, 7 or 3 element rows ->
+element threads, halo column/row,
 owner-computes assembly through the 9-value exchange) and checks on the CPU
 that (1) every global node is written exactly once, (2) every read is in
 bounds, and (3) the assembled element contributions equal the oracle's
@@ -12,12 +13,13 @@ import pytest
 
 import oracle as O
 
-EX = EZ = 16
-TX, TZ = EX - 1, EZ - 1
+EX = 17
+TX = EX - 1
 P = 4
 
 
-def emulate_Ku(nex, nez, u, c2, dx, dz):
+def emulate_Ku(nex, nez, u, c2, dx, dz, EZ=15):
+    TZ = EZ - 1
     NX, NZ = nex * P + 1, nez * P + 1
     xi = O.gll_points(P)
     w = O.gll_weights(xi)
@@ -72,8 +74,9 @@ def emulate_Ku(nex, nez, u, c2, dx, dz):
     return out, writes
 
 
-@pytest.mark.parametrize("nex,nez", [(1, 1), (2, 3), (15, 15), (16, 15), (17, 31), (31, 2)])
-def test_tile_decomposition_matches_assembled_K(nex, nez):
+@pytest.mark.parametrize("ez", [15, 7, 3])
+@pytest.mark.parametrize("nex,nez", [(1, 1), (2, 3), (16, 14), (17, 15), (33, 29), (31, 2)])
+def test_tile_decomposition_matches_assembled_K(nex, nez, ez):
     rng = np.random.default_rng(nex * 100 + nez)
     mesh = O.StructuredMesh(0.0, 25.0 * nex, 0.0, 20.0 * nez, nex, nez, P)
     ops = O.sem_operators(mesh.dx, mesh.dz, P, lagrange_eps=0.0)
@@ -81,6 +84,6 @@ def test_tile_decomposition_matches_assembled_K(nex, nez):
     u = rng.standard_normal(mesh.N)
     K = O.stiffness_csr(c, mesh, ops)
     ref = K @ u
-    got, writes = emulate_Ku(nex, nez, u, c * c, mesh.dx, mesh.dz)
+    got, writes = emulate_Ku(nex, nez, u, c * c, mesh.dx, mesh.dz, ez)
     assert writes.min() == 1 and writes.max() == 1, "every node must be written exactly once"
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())

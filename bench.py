@@ -71,6 +71,65 @@ def cpu_baseline(wl, nsteps=24):
                       f"extrapolated to {m.nt} steps = {t_solve:.0f} s per forward+adjoint solve, 1 thread"}
 
 
+def extra_workloads(log):
+    """The other seeded workloads: 40x40 (batched 256 models vs the plain
+    one-solve path) and the 64-sample x 8-shot ELBO batch."""
+    import torch
+    from mun_amd import workloads as W
+    from mun_amd.batch import BatchSolver
+    dev = torch.device("cuda:0")
+    res = {}
+    # cpu40: 40x40 deg-4, 2000 steps; 256 models in one batch vs one at a time
+    wl = W.cpu40()
+    nb = 256
+    rng = np.random.default_rng(11)
+    coefs = wl.coef[0][None] + rng.normal(0.0, 50.0, (nb,) + wl.coef.shape[1:])
+    wl.coef = coefs
+    s = BatchSolver.from_workload(wl, precision="fp32", device=dev)
+    obs = s.forward(torch.as_tensor(wl.true_coef[None], device=dev))[0]
+    c = torch.as_tensor(coefs, device=dev)
+    s.loglik_grad(c, obs, wl.noise_std)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.loglik_grad(c, obs, wl.noise_std)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    wl1 = W.cpu40()
+    s1 = BatchSolver.from_workload(wl1, precision="fp32", device=dev)
+    s1.loglik_grad(c[:1], obs, wl.noise_std)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(4):
+        s1.loglik_grad(c[k:k + 1], obs, wl.noise_std)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / 4
+    res["cpu40"] = {"workload": "40x40 deg-4 (161x161 nodes), 2000 steps, fwd+adjoint, 8x8 B-spline nodes",
+                    "batched_solves_per_s": round(nb / tb, 2), "batch": nb,
+                    "tile_rows": s.plan.tile_rows, "plain_solves_per_s": round(1.0 / tp, 2),
+                    "batched_vs_plain": round(tp * nb / tb, 2)}
+    log(f"[bench] cpu40: batched {nb / tb:.1f}/s, plain {1 / tp:.1f}/s, ratio {tp * nb / tb:.1f}")
+    del s, s1
+    torch.cuda.empty_cache()
+    # elbo120: 64 samples x 8 shots, 120x120 deg-4, 4000 steps
+    wl = W.elbo120()
+    s = BatchSolver.from_workload(wl, precision="fp32", device=dev)
+    obs = s.forward(torch.as_tensor(wl.true_coef[None], device=dev))[0]
+    c = torch.as_tensor(wl.coef, device=dev)
+    t0 = time.perf_counter()
+    ll, g, _ = s.loglik_grad(c, obs, wl.noise_std)
+    elbo_grad = g.mean(0)
+    torch.cuda.synchronize()
+    te = time.perf_counter() - t0
+    res["elbo120"] = {"workload": "64 samples x 8 shots, 120x120 deg-4 (481x481 nodes), 4000 steps, "
+                                  "mean gradient over 12x12 B-spline nodes",
+                      "solves_per_s": round(wl.nmodel * wl.nshot / te, 2), "seconds": round(te, 2),
+                      "tile_rows": s.plan.tile_rows, "mean_grad_norm": float(elbo_grad.norm())}
+    log(f"[bench] elbo120: {te:.1f} s for {wl.nmodel * wl.nshot} solves")
+    del s
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +143,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-nt-scale", type=float, default=0.5)
+    ap.add_argument("--no-extra", action="store_true", help="skip the cpu40 and elbo120 workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -174,22 +234,32 @@ def main():
         out["plain_solves_per_s"] = round(1.0 / tp1, 4)
         out["batched_vs_plain"] = round(value * tp1, 3)
         log(f"[bench] plain path: {tp1:.2f} s per solve")
+    del solver, plan, coef, obs, c2, ll, grad
+    if not args.no_plain:
+        del plain
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        out["workloads"] = extra_workloads(log)
     if not args.no_serve:
         from mun_amd.serving import SEMServer, latency_stats, run_plain
         log(f"[bench] served stream: {args.serve_requests} requests, Poisson {args.serve_rate}/s, "
             f"meshes 20..160^2, nt scale {args.serve_nt_scale}")
-        del solver, plan
-        torch.cuda.empty_cache()
         reqs = W.served_solve_requests(args.serve_requests, args.serve_rate, nt_scale=args.serve_nt_scale)
         torch.cuda.synchronize()
         srv = SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000)
         served = latency_stats(srv.run(reqs, progress=log))
         plain = latency_stats(run_plain(reqs, progress=log))
+        # saturated: every request present at t = 0 (throughput, not latency)
+        for r in reqs:
+            r.arrival = 0.0
+        sat = latency_stats(SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000).run(reqs))
+        sat_plain = latency_stats(run_plain(reqs))
         out["served"] = {"workload": f"{args.serve_requests} Poisson({args.serve_rate}/s) forward+adjoint requests, "
                                      f"meshes {{20,40,80,160}}^2 deg-4, {int(1000 * args.serve_nt_scale)}-"
                                      f"{int(5000 * args.serve_nt_scale)} steps, 20-200 receivers",
                          "continuous": served, "plain_one_at_a_time": plain,
-                         "throughput_ratio": round(served["solves_per_s"] / plain["solves_per_s"], 3)}
+                         "saturated_continuous": sat, "saturated_plain": sat_plain,
+                         "saturated_throughput_ratio": round(sat["solves_per_s"] / sat_plain["solves_per_s"], 3)}
         out["latency_s"] = {"p50": round(served["p50_s"], 4), "p99": round(served["p99_s"], 4),
                             "note": "served stream (continuous batching), arrival to result"}
         log(f"[bench] served p50 {served['p50_s']:.3f}s p99 {served['p99_s']:.3f}s {served['solves_per_s']:.2f}/s; "

@@ -1,6 +1,6 @@
 """Profiling driver: runs the solver's kernels at a fixed shape under a profiler.
 
-    python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024]
+    python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -26,6 +26,7 @@ def main(argv=None):
     ap.add_argument("--nt", type=int, default=40)
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--tile-rows", type=int, default=15, help="step-kernel tile height (15, 7 or 3)")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     n = a.copy_mb * (1 << 20) // 4
@@ -37,7 +38,7 @@ def main(argv=None):
     print(f"[profile] copy {a.copy_mb} MiB x3 done", flush=True)
     wl = W.batch200(nmodel=a.batch, nt=a.nt)
     wl.mesh.checkpoint_every = a.checkpoint_every
-    s = BatchSolver.from_workload(wl, precision="fp32", device=dev)
+    s = BatchSolver.from_workload(wl, tile_rows=a.tile_rows, precision="fp32", device=dev)
     coef = torch.as_tensor(wl.coef, device=dev)
     _, c2 = s.velocity(coef)
     srcn, model_of = s._expand(a.batch)

@@ -48,7 +48,10 @@ constexpr int EZ_DEFAULT = 15;
 __host__ __device__ constexpr int tile_threads(int ez) { return (EX * ez + 63) / 64 * 64; }
 constexpr int NXCD = 8;           // XCDs of an MI355X
 
-enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2 };
+// MODE_ADJ: adjoint step with the gradient correlation fused (slot tables);
+// MODE_ADJS: adjoint step alone, the correlation runs later per chunk of steps
+// in corr_kernel (uniform batches).
+enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2, MODE_ADJS = 3 };
 
 // Global-memory qualified pointers: lets the compiler emit global_* instead of
 // flat_* instructions for pointers that come from the slot table.
@@ -360,11 +363,12 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
-    constexpr bool ADJ = (MODE == MODE_ADJ);
+    constexpr bool ADJ = (MODE == MODE_ADJ || MODE == MODE_ADJS);
+    constexpr bool CORR = (MODE == MODE_ADJ);
     __shared__ T xr_col[EZ][EX][NP];   // element's i = P column of contributions
     __shared__ T xr_row[EZ][EX][P];    // element's j = P row (i < P)
-    __shared__ T xg_col[ADJ ? EZ : 1][EX][NP];
-    __shared__ T xg_row[ADJ ? EZ : 1][EX][P];
+    __shared__ T xg_col[CORR ? EZ : 1][EX][NP];
+    __shared__ T xg_row[CORR ? EZ : 1][EX][P];
 
     // XCD-contiguous numbering of (tile, solve) pairs
     const int wgid = blockIdx.x;
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
         return;
     } else {
         T u[NP][NP], r[NP][NP];
-        T g[ADJ ? NP : 1][ADJ ? NP : 1];
+        T g[CORR ? NP : 1][CORR ? NP : 1];
         // update-phase operands of the owned 4 x 4 block, fetched before the
         // element math so their latency overlaps it
         T wv[P][P];
@@ -431,8 +435,8 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     load4(gm(a.Dz) + P * ez, dzv);
                 }
             }
-            T v[ADJ ? NP : 1][ADJ ? NP : 1];
-            if constexpr (ADJ) {
+            T v[CORR ? NP : 1][CORR ? NP : 1];
+            if constexpr (CORR) {
                 const GMEM T* __restrict__ ufp = V.uf + base;
 #pragma unroll
                 for (int j = 0; j < NP; ++j) load5(ufp + j * pitch, v[j]);
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                 for (int i = 0; i < NP; ++i) {
                     r[j][i] = T(0);
                     u[j][i] = T(0);
-                    if constexpr (ADJ) g[j][i] = T(0);
+                    if constexpr (CORR) g[j][i] = T(0);
                 }
         }
 
@@ -497,7 +501,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
             for (int j = 0; j < NP; ++j) xr_col[ezl][exl][j] = r[j][P];
 #pragma unroll
             for (int i = 0; i < P; ++i) xr_row[ezl][exl][i] = r[P][i];
-            if constexpr (ADJ) {
+            if constexpr (CORR) {
 #pragma unroll
                 for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
 #pragma unroll
@@ -515,7 +519,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                 if (i == 0) r[j][i] += xr_col[ezl][exl - 1][j];
                 if (j == 0) r[j][i] += (i < P) ? xr_row[ezl - 1][exl][i] : xr_col[ezl - 1][exl][P];
                 if (i == 0 && j == 0) r[j][i] += xr_col[ezl - 1][exl - 1][P];
-                if constexpr (ADJ) {
+                if constexpr (CORR) {
                     if (i == 0) g[j][i] += xg_col[ezl][exl - 1][j];
                     if (j == 0) g[j][i] += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
                     if (i == 0 && j == 0) g[j][i] += xg_col[ezl - 1][exl - 1][P];
@@ -575,7 +579,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     store4(V.ck_u + base + j * pitch, un);
                 }
             }
-            if constexpr (ADJ) {
+            if constexpr (CORR) {
                 GMEM double* Gp = V.G + base + j * pitch;
                 double gv[4];
                 load4nt(Gp, gv);
@@ -601,9 +605,150 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                             V.ck_u[base + o] = r[j][i];
                         }
                     }
-                    if constexpr (ADJ) V.G[base + o] += (double)g[j][i];
+                    if constexpr (CORR) V.G[base + o] += (double)g[j][i];
                 }
         }
+    }
+}
+
+// Element-local gradient correlation of (uf, l): the correlation half of
+// element_action with the same arithmetic (x part by rows, z part by columns).
+template <typename T, int P>
+__device__ __forceinline__ void element_corr(const ElemConst<T, P>& ec, const T (&l)[P + 1][P + 1],
+                                             const T (&uf)[P + 1][P + 1], T (&g)[P + 1][P + 1]) {
+    constexpr int NP = P + 1;
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            T d = T(0), du = T(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * l[j][k];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) du += ec.D[q][k] * uf[j][k];
+            g[j][q] = ec.gx[j][q] * (du * d);
+        }
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            T d = T(0), du = T(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * l[k][i];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) du += ec.D[q][k] * uf[k][i];
+            g[q][i] += ec.gz[q][i] * (du * d);
+        }
+}
+
+template <typename T, int P>
+struct CorrArgs {
+    ElemConst<T, P> ec;
+    int nex, nez, pitch;
+    int ntx, ntiles, total, nper;
+    long sb;
+    const T* uf;      // u^{c0+1} of solve 0; u^{it+1} at uf + (it - c0) * ustep
+    long ustep;
+    const T* ring;    // adjoint states: lambda_it in slot (it + 1) % ring_slots
+    int ring_slots;
+    long rstep;       // elements between ring slots
+    int c0, clen;     // chunk of adjoint steps it = c0 + clen - 1 .. c0
+    double* G;
+};
+
+// Gradient correlation of one chunk of adjoint steps, run after the chunk's
+// adjoint steps: for it = c0 + clen - 1 down to c0 every node adds its
+// assembled corr(u^{it+1}, lambda_it) to an FP64 running sum that starts from
+// G and is stored back at the end.  Per node this is the same sequence of FP64
+// additions as adding into G after every step, so the result does not depend
+// on how the steps are chunked (checkpoint spacing, full history).  The tile
+// decomposition, halo and LDS exchange are the step kernel's; the running sums
+// of the owned 4 x 4 block live in registers across the chunk.
+template <typename T, int P, int EZ>
+__global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T, P> a) {
+    constexpr int TZ = EZ - 1;
+    static_assert(P == 4, "vector loads assume 5 nodes per element edge");
+    constexpr int NP = P + 1;
+    __shared__ T xg_col[EZ][EX][NP];
+    __shared__ T xg_row[EZ][EX][P];
+
+    const int wgid = blockIdx.x;
+    const int lin = (wgid % NXCD) * a.nper + wgid / NXCD;
+    if (lin >= a.total) return;  // whole workgroup leaves together
+    const int tile = lin % a.ntiles;
+    const long b = lin / a.ntiles;
+    const int t = threadIdx.x;
+    const int exl = t % EX, ezl = t / EX;
+    const int tx = tile % a.ntx, tz = tile / a.ntx;
+    const int ex = tx * TX - 1 + exl;
+    const int ez = tz * TZ - 1 + ezl;
+    const bool valid = t < EX * EZ && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
+    const bool owned = valid && exl >= 1 && ezl >= 1;
+    const int pitch = a.pitch;
+    const int base = (P * ez) * pitch + P * ex;
+    const long soff = b * a.sb;
+    GMEM double* __restrict__ Gp = gm(a.G + soff) + base;
+    const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
+
+    double acc[P][P];
+    if (owned) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) load4nt(Gp + j * pitch, acc[j]);
+    }
+    for (int s = a.clen - 1; s >= 0; --s) {
+        const int it = a.c0 + s;
+        T g[NP][NP];
+        if (valid) {
+            const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base;
+            const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base;
+            T l[NP][NP], u[NP][NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                load5(lp + j * pitch, l[j]);
+                load5(up + j * pitch, u[j]);
+            }
+            element_corr<T, P>(a.ec, l, u, g);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) g[j][i] = T(0);
+        }
+        if (t < EX * EZ) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
+#pragma unroll
+            for (int i = 0; i < P; ++i) xg_row[ezl][exl][i] = g[P][i];
+        }
+        __syncthreads();
+        if (owned) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    if (i == 0) g[j][i] += xg_col[ezl][exl - 1][j];
+                    if (j == 0) g[j][i] += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
+                    if (i == 0 && j == 0) g[j][i] += xg_col[ezl - 1][exl - 1][P];
+                }
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+#pragma unroll
+                for (int i = 0; i < P; ++i) acc[j][i] += (double)g[j][i];
+            if (last_x || last_z) {
+#pragma unroll
+                for (int j = 0; j < NP; ++j)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        if (!((i == P && last_x && (j < P || last_z)) || (j == P && last_z && i < P))) continue;
+                        Gp[j * pitch + i] += (double)g[j][i];
+                    }
+            }
+        }
+        __syncthreads();  // exchange buffers are rewritten by the next step
+    }
+    if (owned) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) store4nt(Gp + j * pitch, acc[j]);
     }
 }
 

@@ -316,8 +316,18 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Layout {
     bool full_history = false;
     int S = 0, nseg = 0;
-    size_t C2P, U0, U1, W, CKU, CKW, SEG, L0, L1, OM, RES, G, NREC, RSET, total;
+    int C = 0;  // adjoint steps per gradient-correlation chunk
+    size_t C2P, U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, NREC, RSET, total;
 };
+
+// Correlation chunk: at most 32 adjoint steps, dividing a segment of S steps
+// into equal chunks (S = 100 -> 4 x 25).  The adjoint states of one chunk are
+// kept in a ring of C + 1 slots; the FP64 accumulator is read and written once
+// per chunk.
+int corr_chunk(int S) {
+    const int nchunk = (S + 31) / 32;
+    return (S + nchunk - 1) / nchunk;
+}
 
 // Workspace regions; every nodal array is [batch][sb] in the padded layout.
 Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
@@ -348,8 +358,8 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
         } else {
             L.SEG = take(state * (size_t)(nt + 1));
         }
-        L.L0 = take(state);
-        L.L1 = take(state);
+        L.C = corr_chunk(L.S);
+        L.RING = take(state * (size_t)(L.C + 1));
         L.OM = take(state);
         L.RES = take((size_t)B * nt * pitch * Ts);
         L.G = take((size_t)B * pl->sb * 8);
@@ -423,6 +433,19 @@ int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
         case 15: step_kernel<T, 4, MODE, 15><<<grid, tile_threads(15), 0, st>>>(a); break;
         case 7: step_kernel<T, 4, MODE, 7><<<grid, tile_threads(7), 0, st>>>(a); break;
         case 3: step_kernel<T, 4, MODE, 3><<<grid, tile_threads(3), 0, st>>>(a); break;
+        default: return fail(SEM_E_UNSUPPORTED, "tile height without a kernel instance");
+    }
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+template <typename T>
+int launch_corr(const sem_plan* pl, const CorrArgs<T, 4>& c, hipStream_t st) {
+    const unsigned grid = (unsigned)c.nper * NXCD;
+    switch (pl->ez) {
+        case 15: corr_kernel<T, 4, 15><<<grid, tile_threads(15), 0, st>>>(c); break;
+        case 7: corr_kernel<T, 4, 7><<<grid, tile_threads(7), 0, st>>>(c); break;
+        case 3: corr_kernel<T, 4, 3><<<grid, tile_threads(3), 0, st>>>(c); break;
         default: return fail(SEM_E_UNSUPPORTED, "tile height without a kernel instance");
     }
     LAUNCH_CHECK();
@@ -550,12 +573,15 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
                                           (long)nt * bt->rec_pitch);
     LAUNCH_CHECK();
 
-    // adjoint sweep
-    T* L0 = (T*)(ws + L.L0);
-    T* L1 = (T*)(ws + L.L1);
+    // adjoint sweep: per segment, recompute u into SEG, then chunks of up to C
+    // adjoint steps (states in a ring indexed by time) each followed by the
+    // gradient correlation of the chunk
+    T* RING = (T*)(ws + L.RING);
     T* OM = (T*)(ws + L.OM);
     double* G = (double*)(ws + L.G);
-    if ((rc = fill<T>(L0, T(0), state, st))) return rc;
+    const int ring_slots = L.C + 1;
+    auto ring_slot = [&](int t) { return RING + (long)((t + 1) % ring_slots) * state; };  // lambda_t
+    if ((rc = fill<T>(ring_slot(nt - 1), T(0), state, st))) return rc;
     if ((rc = fill<T>(OM, T(0), state, st))) return rc;
     if ((rc = fill<double>(G, 0.0, state, st))) return rc;
 
@@ -566,10 +592,23 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     a.src_node = nullptr;
     a.resid = RES;
     a.resid_bs = (long)nt * bt->rec_pitch;
-    a.G = G;
+    CorrArgs<T, 4> c;
+    std::memset(&c, 0, sizeof(c));
+    c.ec = a.ec;
+    c.nex = a.nex;
+    c.nez = a.nez;
+    c.pitch = a.pitch;
+    c.ntx = a.ntx;
+    c.ntiles = a.ntiles;
+    c.total = a.total;
+    c.nper = a.nper;
+    c.sb = a.sb;
+    c.ustep = state;
+    c.ring = RING;
+    c.ring_slots = ring_slots;
+    c.rstep = state;
+    c.G = G;
     T* SEG = (T*)(ws + L.SEG);
-    T* lam_in = L0;
-    T* lam_out = L1;
     for (int k = L.nseg - 1; k >= 0; --k) {
         const int s0 = k * L.S;
         const int len = std::min(L.S, nt - s0);
@@ -584,13 +623,21 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
                 if ((rc = launch_step<T, MODE_FWD>(pl, f, st))) return rc;
             }
         }
-        for (int it = s0 + len - 1; it >= s0; --it) {
-            a.u_in = lam_in;
-            a.u_out = lam_out;
-            a.uf = SEG + (long)(it - s0 + 1) * state;  // u^{it+1}
-            a.res_t = it;
-            if ((rc = launch_step<T, MODE_ADJ>(pl, a, st))) return rc;
-            std::swap(lam_in, lam_out);
+        const int useg0 = L.full_history ? 0 : s0;  // SEG slot j holds u^{useg0 + j}
+        for (int ce = s0 + len; ce > s0;) {
+            const int clen = std::min(L.C, ce - s0);
+            const int c0 = ce - clen;
+            for (int it = ce - 1; it >= c0; --it) {
+                a.u_in = ring_slot(it);
+                a.u_out = ring_slot(it - 1);
+                a.res_t = it;
+                if ((rc = launch_step<T, MODE_ADJS>(pl, a, st))) return rc;
+            }
+            c.uf = SEG + (long)(c0 + 1 - useg0) * state;  // u^{c0+1}
+            c.c0 = c0;
+            c.clen = clen;
+            if ((rc = launch_corr<T>(pl, c, st))) return rc;
+            ce = c0;
         }
     }
     if (grad_c || grad_c2) {

@@ -695,19 +695,40 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
 #pragma unroll
         for (int j = 0; j < P; ++j) load4nt(Gp + j * pitch, acc[j]);
     }
-    for (int s = a.clen - 1; s >= 0; --s) {
+    // FP32: software pipeline, the next step's u and lambda blocks are in
+    // flight while this step's correlation is computed (FP64 blocks would
+    // need twice the registers and load per step)
+    constexpr bool PIPE = std::is_same_v<T, float>;
+    T lc[NP][NP], uc[NP][NP];
+    auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
         const int it = a.c0 + s;
+        const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base;
+        const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            load5(lp + j * pitch, l[j]);
+            load5(up + j * pitch, u[j]);
+        }
+    };
+    if (PIPE && valid) fetch(a.clen - 1, lc, uc);
+    for (int s = a.clen - 1; s >= 0; --s) {
         T g[NP][NP];
         if (valid) {
-            const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base;
-            const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base;
-            T l[NP][NP], u[NP][NP];
+            if constexpr (PIPE) {
+                T ln[NP][NP], un[NP][NP];
+                fetch(s > 0 ? s - 1 : 0, ln, un);  // the last step re-reads its own (cached) blocks
+                element_corr<T, P>(a.ec, lc, uc, g);
 #pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                load5(lp + j * pitch, l[j]);
-                load5(up + j * pitch, u[j]);
+                for (int j = 0; j < NP; ++j)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        lc[j][i] = ln[j][i];
+                        uc[j][i] = un[j][i];
+                    }
+            } else {
+                fetch(s, lc, uc);
+                element_corr<T, P>(a.ec, lc, uc, g);
             }
-            element_corr<T, P>(a.ec, l, u, g);
         } else {
 #pragma unroll
             for (int j = 0; j < NP; ++j)

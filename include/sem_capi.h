@@ -78,6 +78,13 @@ typedef struct {
     double lagrange_eps;       /* +eps in Lagrange denominators (utils.py:36), 1e-12 */
     int tile_rows;             /* step-kernel tile height in element rows incl. the
                                   halo row: 15, 7 or 3 (0 = 15); see sem_autotune  */
+    /* Misfit window (CO2_plume/one_stage/single_cubic/sem_waveform/core.py:1433-1520):
+     * only samples t in [win_begin, win_end) with (t - win_begin) % win_decim == 0
+     * enter the misfit and the adjoint source.  win_end = 0 means nt, win_decim
+     * <= 1 means every sample.  SEM_ADJ_REFERENCE then runs the adjoint over the
+     * window only, t = win_end-1 .. win_begin, as the reference does (its
+     * gradient leaves out t < win_begin); SEM_ADJ_EXACT runs it back to t = 0. */
+    int win_begin, win_end, win_decim;
 } sem_desc;
 
 /* One batch of independent solves on a plan's mesh.  Solve b uses model
@@ -137,6 +144,27 @@ int sem_forward_adjoint(sem_plan* plan, const sem_batch* b, const void* obs, dou
                         double* misfit, double* grad_c, double* grad_c2,
                         void* workspace, size_t ws_bytes, void* stream);
 
+/* Forward solve that also keeps wavefield snapshots of a node-index box:
+ * snapshot k is u after step k*every (u^{k*every+1}), nodes ix0 <= ix < ix1,
+ * iz0 <= jz < iz1, stored [batch][nsnap][iz1-iz0][ix1-ix0] (real), nsnap =
+ * ceil(nt / every).  Replaces the save_wavefield path of run()
+ * (Bspline_Coupling/SEM/sem_waveform/core.py:318-325, 400-401, 649-651): the
+ * reference's ROI mask selects the nodes with x, z inside the ROI rectangle,
+ * which on the structured grid is this box.  Seismograms as sem_forward. */
+int sem_forward_snapshots(sem_plan* plan, const sem_batch* b, int every, int ix0, int ix1, int iz0, int iz1,
+                          void* snapshots, void* workspace, size_t ws_bytes, void* stream);
+
+/* Background velocity from a regular grid onto the nodes: vp [nzg][nxg] on
+ * increasing axes xg [nxg], zg [nzg], node coordinates clipped into the grid,
+ * bilinear (SEM_INTERP_LINEAR) or nearest grid value (SEM_INTERP_NEAREST,
+ * ties to the lower index).  Replaces build_background_from_grid
+ * (CO2_plume/one_stage/single_cubic/sem_waveform/velocity.py:212-274, scipy
+ * RegularGridInterpolator with clip=True).  All arrays device double; out [N]. */
+#define SEM_INTERP_LINEAR 0
+#define SEM_INTERP_NEAREST 1
+int sem_grid_to_nodes(sem_plan* plan, int nxg, int nzg, const double* xg, const double* zg, const double* vp,
+                      int method, double* out, void* stream);
+
 /* Tensor-product cubic B-spline velocity nodes (clamped uniform knots on the
  * domain; no reference counterpart -- nearest analogue is the gridded
  * background of CO2_plume/one_stage/single_cubic/sem_waveform/velocity.py:212-274):
@@ -167,6 +195,41 @@ int sem_curve_velocity(sem_plan* plan, int nmodel, const double* ctrl, double v_
  *   dL_dv device double [nmodel][N];  grad_ctrl device double [nmodel][12] */
 int sem_curve_project(sem_plan* plan, int nmodel, const double* dL_dv, const double* aux, double v_inside,
                       double v_outside, double tau, double* grad_ctrl, void* stream);
+
+/* Generalised closed-curve anomaly.  kind = SEM_CURVE_CLAMPED6 is the curve
+ * above; SEM_CURVE_PERIODIC is the closed periodic cubic B-spline through K
+ * control points of CO2_plume/one_stage/single_cubic/sem_waveform/velocity.py:9-152
+ * (controls extended by their first 3, knots 0, 1, ..., K+6, parameter interval
+ * [3, K+3), samples at 3 + kK/S, u* stored normalised to [0, 1)).  The anomaly
+ * is blended over a per-node background (velocity.py:277-332):
+ *   SEM_BLEND_SMOOTH : v = v_bg + (v_in - v_bg) sigmoid(-sd/tau)
+ *   SEM_BLEND_REPLACE: v = v_in where sd < 0, else v_bg
+ * with v_bg = background[node] (device double [N]) or v_outside if NULL.
+ *   ctrl device double [nmodel][nctrl][2];  c, c2, aux as sem_curve_velocity. */
+#define SEM_CURVE_CLAMPED6 0
+#define SEM_CURVE_PERIODIC 1
+#define SEM_BLEND_SMOOTH 0
+#define SEM_BLEND_REPLACE 1
+#define SEM_CURVE_MAX_CTRL 32
+typedef struct {
+    int kind;                  /* SEM_CURVE_*                                          */
+    int nctrl;                 /* 6 for CLAMPED6; 4 .. SEM_CURVE_MAX_CTRL for PERIODIC  */
+    int samples;               /* initial-guess curve samples, 2 .. 2048               */
+    int newton_steps;
+    double v_inside, v_outside, tau;
+    const double* background;  /* device double [N] or NULL                            */
+    int blend;                 /* SEM_BLEND_*                                          */
+} sem_curve_desc;
+int sem_curve_velocity_ex(sem_plan* plan, const sem_curve_desc* cd, int nmodel, const double* ctrl, double* c,
+                          void* c2, double* aux, void* stream);
+/* Projection of dL/dv onto the 2 nctrl control coordinates, interleaved
+ * [x0, z0, x1, z1, ...]: the clamped rule above for CLAMPED6 (threshold
+ * 1e-8 (v_bg - v_in)); for PERIODIC the rule of
+ * CO2_plume/one_stage/single_cubic/sem_waveform/core.py:1075-1185 (periodic
+ * basis with the 3 appended controls folded back, nodes kept where
+ * |dv/dsd| > 1e-8 max(1, |v_in - v_bg|)).  grad_ctrl device double [nmodel][2 nctrl]. */
+int sem_curve_project_ex(sem_plan* plan, const sem_curve_desc* cd, int nmodel, const double* dL_dv,
+                         const double* aux, double* grad_ctrl, void* stream);
 
 /* ---- continuous batching (slots) -------------------------------------------
  * A slot is one solve that advances through its own phases at its own time

@@ -23,7 +23,14 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_receiver_set_size", "sem_source_node", "sem_workspace_bytes", "sem_forward",
            "sem_forward_adjoint", "sem_bspline_map", "sem_bspline_project", "sem_curve_velocity",
            "sem_curve_project", "sem_plan_layout", "sem_slots_step", "sem_slots_record_final", "sem_pad_nodal",
-           "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize", "sem_last_error"]
+           "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize",
+           "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
+           "sem_last_error"]
+
+SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
+SEM_CURVE_CLAMPED6, SEM_CURVE_PERIODIC = 0, 1
+SEM_BLEND_SMOOTH, SEM_BLEND_REPLACE = 0, 1
+SEM_CURVE_MAX_CTRL = 32
 
 
 class SemDesc(C.Structure):
@@ -33,7 +40,13 @@ class SemDesc(C.Structure):
                 ("sponge_width", C.c_double), ("sponge_cmax", C.c_double), ("sponge_R", C.c_double),
                 ("sponge_sides", C.c_uint), ("precision", C.c_int), ("adjoint_mode", C.c_int),
                 ("record_mode", C.c_int), ("checkpoint_every", C.c_int), ("lagrange_eps", C.c_double),
-                ("tile_rows", C.c_int)]
+                ("tile_rows", C.c_int), ("win_begin", C.c_int), ("win_end", C.c_int), ("win_decim", C.c_int)]
+
+
+class SemCurveDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nctrl", C.c_int), ("samples", C.c_int), ("newton_steps", C.c_int),
+                ("v_inside", C.c_double), ("v_outside", C.c_double), ("tau", C.c_double),
+                ("background", C.c_void_p), ("blend", C.c_int)]
 
 
 class SemBatch(C.Structure):
@@ -95,6 +108,10 @@ def lib() -> C.CDLL:
     L.sem_receivers_sync.argtypes = [vp]
     L.sem_residual.argtypes = [vp, vp, vp, vp, vp, i, i, i, vp]
     L.sem_gradient_finalize.argtypes = [vp, vp, vp, d, vp, vp, vp]
+    L.sem_forward_snapshots.argtypes = [vp, C.POINTER(SemBatch), i, i, i, i, i, vp, vp, sz, vp]
+    L.sem_grid_to_nodes.argtypes = [vp, i, i, vp, vp, vp, i, vp, vp]
+    L.sem_curve_velocity_ex.argtypes = [vp, C.POINTER(SemCurveDesc), i, vp, vp, vp, vp, vp]
+    L.sem_curve_project_ex.argtypes = [vp, C.POINTER(SemCurveDesc), i, vp, vp, vp, vp]
     for name in EXPORTS:
         if name != "sem_last_error":
             getattr(L, name).restype = C.c_int

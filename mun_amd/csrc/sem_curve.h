@@ -1,32 +1,60 @@
-// Closed-curve velocity anomaly on the GPU (the reference's parameterisation).
+// Closed-curve velocity anomaly on the GPU (the reference's two parameterisations).
 //
-// Restates Bspline_Coupling/SEM/sem_waveform/velocity.py:8-202 and the
+// CLAMPED6 restates Bspline_Coupling/SEM/sem_waveform/velocity.py:8-202 and the
 // analytic projection of core.py:902-970:
 //   curve   C(u), u in [0,1]: cubic B-spline with control points
 //           P = [ctrl0..ctrl5, ctrl0, ctrl0, ctrl0] and clamped knots
-//           [0,0,0,0, 1/6, 2/6, 3/6, 4/6, 5/6, 1,1,1,1]
-//   sd      nearest of `samples` uniform curve samples, then `newton_steps`
-//           Newton steps on phi(u) = |C(u) - X|^2 / 2 (clipped to [0,1]);
-//           sign negative inside the sampled polygon (crossing-number test)
-//   v       = v_out + (v_in - v_out) sigmoid(-sd / tau)
-//   grad    g9_j = sum_nodes dL/dv * dv/dsd * N_j(u*) * n_out, then the three
-//           copies of P0 fold back into ctrl0; output interleaved [x0,z0,...].
+//           [0,0,0,0, 1/6, 2/6, 3/6, 4/6, 5/6, 1,1,1,1]; samples u_k = k/(S-1)
+//   grad    the three copies of P0 fold back into ctrl0; nodes kept where
+//           |dv/dsd| > 1e-8 (v_out - v_in)
+// PERIODIC restates CO2_plume/one_stage/single_cubic/sem_waveform/velocity.py:9-152
+// and core.py:1075-1185:
+//   curve   C(t), t in [3, K+3): uniform cubic B-spline with control points
+//           [ctrl0..ctrl_{K-1}, ctrl0, ctrl1, ctrl2] and knots 0, 1, ..., K+6;
+//           samples t_k = 3 + kK/S (endpoint excluded); u* stored as (t-3)/K
+//   grad    closed index j folds onto ctrl (j mod K); nodes kept where
+//           |dv/dsd| > 1e-8 max(1, |v_in - v_bg|)
+// Both:
+//   sd      nearest of S curve samples, then Newton steps on |C - X|^2 / 2
+//           (parameter clipped to the curve's interval); sign negative inside
+//           the sampled polygon (crossing-number test)
+//   v       smooth : v_bg + (v_in - v_bg) sigmoid(-sd / tau)
+//           replace: v_in where sd < 0, else v_bg
+//   grad    sum over nodes of dL/dv * dv/dsd * N_j(u*) * n_out, interleaved
+//           [x0, z0, x1, z1, ...]
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "../../include/sem_capi.h"
 
 namespace sem {
 
 constexpr int CURVE_MAX_SAMPLES = 2048;
+constexpr int CURVE_MAX_CLOSED = SEM_CURVE_MAX_CTRL + 3;
 
-__device__ inline void curve_knots(double* t) {
+struct CurveParams {
+    int kind, K, S, newton, blend;
+    double v_in, v_out, tau;
+    const double* bg;  // per-node background or null
+};
+
+// Curve geometry shared by a workgroup (LDS).
+struct CurveGeom {
+    int kind, K, ncl;         // ncl: closed control count (9 or K + 3)
+    double t[13];             // clamped knots (CLAMPED6)
+    double P[CURVE_MAX_CLOSED][2];
+    double lo, hi;            // parameter interval, Newton clip [lo, hi]
+};
+
+__device__ inline void clamped_knots(double* t) {
     for (int i = 0; i < 4; ++i) t[i] = 0.0;
     for (int i = 1; i <= 5; ++i) t[3 + i] = i * (1.0 / 6.0);
     for (int i = 9; i < 13; ++i) t[i] = 1.0;
 }
 
 // Basis values and first two derivatives of the degree-3 basis functions that
-// are non-zero on the span of u (The NURBS Book, algorithm A2.3).
-__device__ inline int curve_ders(double u, const double* t, double ders[3][4]) {
+// are non-zero on the span of u, clamped knots (The NURBS Book, algorithm A2.3).
+__device__ inline int clamped_ders(double u, const double* t, double ders[3][4]) {
     int s = 3;
     while (s < 8 && u >= t[s + 1]) ++s;
     double ndu[4][4], left[4], right[4];
@@ -75,19 +103,43 @@ __device__ inline int curve_ders(double u, const double* t, double ders[3][4]) {
         ders[1][j] *= 3.0;
         ders[2][j] *= 6.0;
     }
-    return s - 3;  // index of the first non-zero basis function
+    return s - 3;  // closed index of the first non-zero basis function
 }
 
-__device__ inline void curve_eval(const double (*P)[2], const double* t, double u, double* C, double* C1,
-                                  double* C2) {
+// Uniform cubic B-spline on unit knot spacing: span s = floor(t), tau = t - s,
+// basis functions of closed controls s-3 .. s and their derivatives in t.
+__device__ inline int uniform_ders(double t, double ders[3][4]) {
+    const double s = floor(t);
+    const double x = t - s, y = 1.0 - x;
+    const double x2 = x * x, x3 = x2 * x;
+    ders[0][0] = y * y * y / 6.0;
+    ders[0][1] = (3.0 * x3 - 6.0 * x2 + 4.0) / 6.0;
+    ders[0][2] = (-3.0 * x3 + 3.0 * x2 + 3.0 * x + 1.0) / 6.0;
+    ders[0][3] = x3 / 6.0;
+    ders[1][0] = -0.5 * y * y;
+    ders[1][1] = 1.5 * x2 - 2.0 * x;
+    ders[1][2] = -1.5 * x2 + x + 0.5;
+    ders[1][3] = 0.5 * x2;
+    ders[2][0] = y;
+    ders[2][1] = 3.0 * x - 2.0;
+    ders[2][2] = 1.0 - 3.0 * x;
+    ders[2][3] = x;
+    return (int)s - 3;
+}
+
+__device__ inline int curve_ders(const CurveGeom& g, double u, double d[3][4]) {
+    return g.kind == SEM_CURVE_PERIODIC ? uniform_ders(u, d) : clamped_ders(u, g.t, d);
+}
+
+__device__ inline void curve_eval(const CurveGeom& g, double u, double* C, double* C1, double* C2) {
     double d[3][4];
-    const int f = curve_ders(u, t, d);
+    const int f = curve_ders(g, u, d);
     for (int q = 0; q < 2; ++q) {
         double c0 = 0.0, c1 = 0.0, c2 = 0.0;
         for (int r = 0; r < 4; ++r) {
-            c0 += d[0][r] * P[f + r][q];
-            c1 += d[1][r] * P[f + r][q];
-            c2 += d[2][r] * P[f + r][q];
+            c0 += d[0][r] * g.P[f + r][q];
+            c1 += d[1][r] * g.P[f + r][q];
+            c2 += d[2][r] * g.P[f + r][q];
         }
         C[q] = c0;
         if (C1) C1[q] = c1;
@@ -95,34 +147,59 @@ __device__ inline void curve_eval(const double (*P)[2], const double* t, double 
     }
 }
 
-// One workgroup samples the curve into LDS, then every thread handles nodes in
-// a grid-stride loop.  aux = [sd | u* | n_out.x | n_out.z] planes of N values.
+// Thread 0 fills the geometry of model m (caller synchronises).
+__device__ inline void curve_setup(CurveGeom& g, const CurveParams& p, const double* ctrl, long m) {
+    g.kind = p.kind;
+    g.K = p.K;
+    g.ncl = p.K + 3;
+    const double* c = ctrl + m * 2 * p.K;
+    if (p.kind == SEM_CURVE_PERIODIC) {
+        for (int j = 0; j < g.ncl; ++j) {
+            g.P[j][0] = c[2 * (j % p.K)];
+            g.P[j][1] = c[2 * (j % p.K) + 1];
+        }
+        g.lo = 3.0;
+        g.hi = nextafter((double)(p.K + 3), 3.0);
+    } else {
+        clamped_knots(g.t);
+        for (int j = 0; j < g.ncl; ++j) {
+            const int k = j < 6 ? j : 0;
+            g.P[j][0] = c[2 * k];
+            g.P[j][1] = c[2 * k + 1];
+        }
+        g.lo = 0.0;
+        g.hi = 1.0;
+    }
+}
+
+__device__ inline double curve_sample_param(const CurveGeom& g, int k, int S) {
+    // numpy.linspace's rounding: k * step + start
+    if (g.kind == SEM_CURVE_PERIODIC) return (double)k * (((double)(g.K + 3) - 3.0) / (double)S) + 3.0;
+    return (k == S - 1) ? 1.0 : (S > 1 ? k * (1.0 / (S - 1)) : 0.0);
+}
+
+// One workgroup per (block, model) samples the curve into LDS, then every
+// thread handles nodes in a grid-stride loop.  aux = [sd | u* | n_out.x |
+// n_out.z] planes of N values.
 template <typename T>
-__global__ __launch_bounds__(256) void curve_velocity_kernel(const double* ctrl, int S, int newton, double v_in,
-                                                             double v_out, double tau, const double* x1d,
+__global__ __launch_bounds__(256) void curve_velocity_kernel(CurveParams p, const double* ctrl, const double* x1d,
                                                              const double* z1d, int NX, int NZ, double* c, T* c2,
                                                              double* aux) {
     __shared__ double smp[CURVE_MAX_SAMPLES][2];
-    __shared__ double t[13];
-    __shared__ double P[9][2];
+    __shared__ CurveGeom g;
     const long m = blockIdx.y;
     const long N = (long)NX * NZ;
-    if (threadIdx.x == 0) curve_knots(t);
-    if (threadIdx.x < 18) {
-        const int k = threadIdx.x / 2, q = threadIdx.x % 2;
-        P[k][q] = ctrl[m * 12 + (k < 6 ? k : 0) * 2 + q];
-    }
+    const int S = p.S;
+    if (threadIdx.x == 0) curve_setup(g, p, ctrl, m);
     __syncthreads();
-    const double step = S > 1 ? 1.0 / (S - 1) : 0.0;
     for (int k = threadIdx.x; k < S; k += blockDim.x) {
-        const double u = (k == S - 1) ? 1.0 : k * step;
         double C[2];
-        curve_eval(P, t, u, C, nullptr, nullptr);
+        curve_eval(g, curve_sample_param(g, k, S), C, nullptr, nullptr);
         smp[k][0] = C[0];
         smp[k][1] = C[1];
     }
     __syncthreads();
-    const double ts = tau > 1e-12 ? tau : 1e-12;
+    const double ts = p.tau > 1e-12 ? p.tau : 1e-12;
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
         const double X = x1d[k % NX], Z = z1d[k / NX];
         int best = 0;
@@ -135,89 +212,120 @@ __global__ __launch_bounds__(256) void curve_velocity_kernel(const double* ctrl,
                 best = q;
             }
         }
-        double u = (best == S - 1) ? 1.0 : best * step;
-        for (int it = 0; it < newton; ++it) {
+        double u = curve_sample_param(g, best, S);
+        for (int it = 0; it < p.newton; ++it) {
             double C[2], C1[2], C2[2];
-            curve_eval(P, t, u, C, C1, C2);
+            curve_eval(g, u, C, C1, C2);
             const double rx = C[0] - X, rz = C[1] - Z;
             const double f1 = rx * C1[0] + rz * C1[1];
             double f2 = C1[0] * C1[0] + C1[1] * C1[1] + rx * C2[0] + rz * C2[1];
             if (fabs(f2) < 1e-14) f2 = 1e-14;
-            u = fmin(fmax(u - f1 / f2, 0.0), 1.0);
+            u = fmin(fmax(u - f1 / f2, g.lo), g.hi);
         }
         double Cb[2];
-        curve_eval(P, t, u, Cb, nullptr, nullptr);
+        curve_eval(g, u, Cb, nullptr, nullptr);
         const double rx = X - Cb[0], rz = Z - Cb[1];
         const double dist = sqrt(rx * rx + rz * rz);
         // crossing-number point-in-polygon on the closed sample polygon
         bool inside = false;
-        for (int q = 0, p = S - 1; q < S; p = q++) {
-            const double xq = smp[q][0], zq = smp[q][1], xp = smp[p][0], zp = smp[p][1];
-            if ((zq > Z) != (zp > Z) && X < (xp - xq) * (Z - zq) / (zp - zq) + xq) inside = !inside;
+        for (int q = 0, r = S - 1; q < S; r = q++) {
+            const double xq = smp[q][0], zq = smp[q][1], xr = smp[r][0], zr = smp[r][1];
+            if ((zq > Z) != (zr > Z) && X < (xr - xq) * (Z - zq) / (zr - zq) + xq) inside = !inside;
         }
         const double sgn = inside ? -1.0 : 1.0;
         const double sd = sgn * dist;
-        const double x = fmin(fmax(sd / ts, -50.0), 50.0);
-        const double sig = 1.0 / (1.0 + exp(x));
-        const double v = v_out + (v_in - v_out) * sig;
+        const double vb = p.bg ? p.bg[k] : p.v_out;
+        double v;
+        if (p.blend == SEM_BLEND_REPLACE) {
+            v = sd < 0.0 ? p.v_in : vb;
+        } else {
+            const double x = fmin(fmax(sd / ts, -50.0), 50.0);
+            const double sig = 1.0 / (1.0 + exp(x));
+            v = vb + (p.v_in - vb) * sig;
+        }
         if (c) c[m * N + k] = v;
         c2[m * N + k] = (T)(v * v);
         double* A = aux + m * 4 * N;
         A[k] = sd;
-        A[N + k] = u;
+        A[N + k] = (g.kind == SEM_CURVE_PERIODIC) ? (u - 3.0) / (double)g.K : u;
         A[2 * N + k] = sgn * rx / (dist + 1e-15);
         A[3 * N + k] = sgn * rz / (dist + 1e-15);
     }
 }
 
-// Per-block partial sums of the 18 projection terms (9 basis functions x 2 axes).
-__global__ __launch_bounds__(256) void curve_project_kernel(const double* dLdv, const double* aux, double v_in,
-                                                            double v_out, double tau, long N, double* partial) {
-    __shared__ double red[256][18];
-    __shared__ double t[13];
-    if (threadIdx.x == 0) curve_knots(t);
-    __syncthreads();
+constexpr int CURVE_PROJ_THREADS = 64;
+
+// Per-block partial sums over the closed controls (2 axes x ncl): each thread
+// accumulates its nodes in its own LDS row, then rows are summed in a fixed
+// order (deterministic).
+__global__ __launch_bounds__(CURVE_PROJ_THREADS) void curve_project_kernel(CurveParams p, const double* dLdv,
+                                                                           const double* aux, long N,
+                                                                           double* partial) {
+    __shared__ double acc[CURVE_PROJ_THREADS][2 * CURVE_MAX_CLOSED];
+    __shared__ CurveGeom g;
     const long m = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int ncl = p.K + 3, w2 = 2 * ncl;
+    if (tid == 0) {
+        g.kind = p.kind;
+        g.K = p.K;
+        g.ncl = ncl;
+        if (p.kind != SEM_CURVE_PERIODIC) clamped_knots(g.t);
+    }
+    for (int j = 0; j < w2; ++j) acc[tid][j] = 0.0;
+    __syncthreads();
     const double* A = aux + m * 4 * N;
-    const double ts = tau > 1e-12 ? tau : 1e-12;
-    const double thr = 1e-8 * (v_out - v_in);
-    double acc[18];
-    for (int j = 0; j < 18; ++j) acc[j] = 0.0;
-    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
+    const double ts = p.tau > 1e-12 ? p.tau : 1e-12;
+    for (long k = blockIdx.x * (long)blockDim.x + tid; k < N; k += (long)gridDim.x * blockDim.x) {
+        const double vb = p.bg ? p.bg[k] : p.v_out;
         const double x = fmin(fmax(A[k] / ts, -50.0), 50.0);
         const double sig = 1.0 / (1.0 + exp(x));
-        const double dv = (v_in - v_out) * sig * (1.0 - sig) * (-1.0 / ts);
+        const double dv = (p.v_in - vb) * sig * (1.0 - sig) * (-1.0 / ts);
+        const double thr =
+            (p.kind == SEM_CURVE_PERIODIC) ? 1e-8 * fmax(1.0, fabs(p.v_in - vb)) : 1e-8 * (vb - p.v_in);
         if (!(fabs(dv) > thr)) continue;
         const double w = dLdv[m * N + k] * dv;
+        double u = A[N + k];
+        if (p.kind == SEM_CURVE_PERIODIC) {
+            const double un = u - floor(u);  // mod 1
+            u = fmin(fmax(3.0 + un * (double)p.K, 3.0), nextafter((double)(p.K + 3), 3.0));
+        }
         double d[3][4];
-        const int f = curve_ders(A[N + k], t, d);
+        const int f = curve_ders(g, u, d);
         for (int r = 0; r < 4; ++r) {
-            acc[f + r] += w * d[0][r] * A[2 * N + k];
-            acc[9 + f + r] += w * d[0][r] * A[3 * N + k];
+            acc[tid][f + r] += w * d[0][r] * A[2 * N + k];
+            acc[tid][ncl + f + r] += w * d[0][r] * A[3 * N + k];
         }
     }
-    for (int j = 0; j < 18; ++j) red[threadIdx.x][j] = acc[j];
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int j = 0; j < 18; ++j) red[threadIdx.x][j] += red[threadIdx.x + s][j];
+    for (int s = CURVE_PROJ_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int j = 0; j < w2; ++j) acc[tid][j] += acc[tid + s][j];
         __syncthreads();
     }
-    if (threadIdx.x < 18) partial[(m * gridDim.x + blockIdx.x) * 18 + threadIdx.x] = red[0][threadIdx.x];
+    for (int j = tid; j < w2; j += blockDim.x) partial[(m * gridDim.x + blockIdx.x) * w2 + j] = acc[0][j];
 }
 
-// Fold the partials in a fixed order, collapse P6..P8 into P0, interleave.
-__global__ void curve_project_finish_kernel(const double* partial, int nblk, double* grad) {
+// Fold the partials in a fixed order, collapse the appended controls, interleave.
+__global__ void curve_project_finish_kernel(CurveParams p, const double* partial, int nblk, double* grad) {
     const long m = blockIdx.x;
     if (threadIdx.x != 0) return;
-    double g[18];
-    for (int j = 0; j < 18; ++j) g[j] = 0.0;
+    const int K = p.K, ncl = K + 3, w2 = 2 * ncl;
+    double h[2 * CURVE_MAX_CLOSED];
+    for (int j = 0; j < w2; ++j) h[j] = 0.0;
     for (int b = 0; b < nblk; ++b)
-        for (int j = 0; j < 18; ++j) g[j] += partial[(m * nblk + b) * 18 + j];
+        for (int j = 0; j < w2; ++j) h[j] += partial[(m * nblk + b) * w2 + j];
     for (int axis = 0; axis < 2; ++axis) {
-        const double* h = g + 9 * axis;
-        grad[m * 12 + 0 + axis] = h[0] + h[6] + h[7] + h[8];
-        for (int j = 1; j < 6; ++j) grad[m * 12 + 2 * j + axis] = h[j];
+        const double* a = h + ncl * axis;
+        for (int j = 0; j < K; ++j) {
+            double s = a[j];
+            if (p.kind == SEM_CURVE_PERIODIC) {
+                if (j < 3) s += a[K + j];
+            } else if (j == 0) {
+                s = a[0] + a[6] + a[7] + a[8];
+            }
+            grad[m * 2 * K + 2 * j + axis] = s;
+        }
     }
 }
 

@@ -785,16 +785,20 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
 }
 
 // r = seis - obs (first nrec columns of each row), misfit[b] = 0.5 * sum r^2 in double.
+// Rows outside the misfit window [w0, w1) or off its decimation grid give r = 0.
 template <typename T>
 __global__ __launch_bounds__(256) void residual_kernel(const T* seis, const T* obs, T* res, double* misfit,
-                                                       const int* nrec_of, int nt, int pitch, long bs) {
+                                                       const int* nrec_of, int nt, int pitch, long bs, int w0,
+                                                       int w1, int decim) {
     const long b = blockIdx.x;
     const int nrec = nrec_of[b];
     double acc = 0.0;
     const long total = (long)nt * pitch;
     for (long k = threadIdx.x; k < total; k += blockDim.x) {
         const int c = (int)(k % pitch);
-        const T rv = (c < nrec) ? seis[b * bs + k] - obs[b * bs + k] : T(0);
+        const int row = (int)(k / pitch);
+        const bool in = row >= w0 && row < w1 && (row - w0) % decim == 0;
+        const T rv = (c < nrec && in) ? seis[b * bs + k] - obs[b * bs + k] : T(0);
         res[b * bs + k] = rv;
         acc += (double)rv * (double)rv;
     }
@@ -806,6 +810,54 @@ __global__ __launch_bounds__(256) void residual_kernel(const T* seis, const T* o
         __syncthreads();
     }
     if (threadIdx.x == 0) misfit[b] = 0.5 * red[0];
+}
+
+// Wavefield snapshot: box [iz0, iz0+nzr) x [ix0, ix0+nxr) of every solve's
+// padded state into out[b][nzr][nxr] (out_bs elements between solves).
+template <typename T>
+__global__ void snapshot_kernel(const T* state, long sb, int pitch, int ix0, int nxr, int iz0, int nzr, T* out,
+                                long out_bs) {
+    const long b = blockIdx.y;
+    const long n = (long)nxr * nzr;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const long jz = k / nxr, ix = k - jz * nxr;
+        out[b * out_bs + k] = state[b * sb + (iz0 + jz) * pitch + ix0 + ix];
+    }
+}
+
+// Regular-grid background onto the nodes (coordinates clipped into the grid):
+// cell i = largest index with g[i] <= x, limited to n-2; bilinear weights, or
+// the nearer grid value with ties to the lower index.
+__device__ inline int grid_cell(const double* g, int n, double x, double& f) {
+    int lo = 0, hi = n - 1;  // g[lo] <= x, invariant after clipping
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (g[mid] <= x) lo = mid; else hi = mid;
+    }
+    if (lo > n - 2) lo = n - 2;
+    f = (x - g[lo]) / (g[lo + 1] - g[lo]);
+    return lo;
+}
+
+__global__ void grid_to_nodes_kernel(const double* xg, int nxg, const double* zg, int nzg, const double* vp,
+                                     int method, const double* x1d, const double* z1d, int NX, int NZ,
+                                     double* out) {
+    const long N = (long)NX * NZ;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
+        const double x = fmin(fmax(x1d[k % NX], xg[0]), xg[nxg - 1]);
+        const double z = fmin(fmax(z1d[k / NX], zg[0]), zg[nzg - 1]);
+        double fx, fz;
+        const int i = grid_cell(xg, nxg, x, fx);
+        const int j = grid_cell(zg, nzg, z, fz);
+        const double* r0 = vp + (long)j * nxg;
+        const double* r1 = r0 + nxg;
+        if (method == SEM_INTERP_NEAREST) {
+            const int ii = fx <= 0.5 ? i : i + 1, jj = fz <= 0.5 ? j : j + 1;
+            out[k] = vp[(long)jj * nxg + ii];
+        } else {
+            out[k] = (1.0 - fz) * ((1.0 - fx) * r0[i] + fx * r0[i + 1]) + fz * ((1.0 - fx) * r1[i] + fx * r1[i + 1]);
+        }
+    }
 }
 
 template <typename T>

@@ -489,10 +489,17 @@ int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hi
     return SEM_OK;
 }
 
+// Wavefield snapshots of a node box every `every` steps (null: none).
+struct SnapSpec {
+    int every, ix0, nxr, iz0, nzr;
+    void* out;
+};
+
 // Forward time loop.  Checkpoint mode: ping-pong U0/U1 with (u, w) mirrored
 // into checkpoint k after step k*S-1.  Full-history mode: u^n in SEG slot n.
 template <typename T>
-int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, bool adj, hipStream_t st) {
+int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, bool adj, hipStream_t st,
+                 const SnapSpec* snap = nullptr) {
     const int B = bt->batch, nt = pl->d.nt;
     const long state = (long)B * pl->sb;
     StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);
@@ -532,6 +539,15 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
         }
         if ((rc = launch_step<T, MODE_FWD>(pl, a, st))) return rc;
         last = a.u_out;
+        if (snap && n % snap->every == 0) {
+            const long per = (long)snap->nxr * snap->nzr;
+            const long nsnap = (nt + snap->every - 1) / snap->every;
+            dim3 grid((unsigned)std::min<long>((per + 255) / 256, 1024), B);
+            snapshot_kernel<T><<<grid, 256, 0, st>>>(a.u_out, pl->sb, pl->pitch, snap->ix0, snap->nxr, snap->iz0,
+                                                     snap->nzr, (T*)snap->out + (n / snap->every) * per,
+                                                     nsnap * per);
+            LAUNCH_CHECK();
+        }
     }
     if (post) {
         a.u_in = last;
@@ -542,14 +558,15 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
 }
 
 template <typename T>
-int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, hipStream_t st) {
+int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, hipStream_t st,
+                 const SnapSpec* snap = nullptr) {
     Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, false);
     int rc;
     if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
     char* ws = (char*)wsv;
     if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
     if ((rc = fill<T>((T*)bt->seis, T(0), (long)bt->batch * pl->d.nt * bt->rec_pitch, st))) return rc;
-    return forward_loop<T>(pl, bt, ws, L, false, st);
+    return forward_loop<T>(pl, bt, ws, L, false, st, snap);
 }
 
 template <typename T>
@@ -570,7 +587,8 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     T* RES = (T*)(ws + L.RES);
     residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs, RES, misfit,
                                           (const int*)(ws + L.NREC), nt, bt->rec_pitch,
-                                          (long)nt * bt->rec_pitch);
+                                          (long)nt * bt->rec_pitch, pl->d.win_begin, pl->d.win_end,
+                                          pl->d.win_decim);
     LAUNCH_CHECK();
 
     // adjoint sweep: per segment, recompute u into SEG, then chunks of up to C
@@ -581,7 +599,11 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     double* G = (double*)(ws + L.G);
     const int ring_slots = L.C + 1;
     auto ring_slot = [&](int t) { return RING + (long)((t + 1) % ring_slots) * state; };  // lambda_t
-    if ((rc = fill<T>(ring_slot(nt - 1), T(0), state, st))) return rc;
+    // adjoint steps t_lo <= it < t_hi: lambda vanishes after the last windowed
+    // sample; the reference adjoint stops at the window start
+    const int t_hi = pl->d.win_end;
+    const int t_lo = pl->d.adjoint_mode == SEM_ADJ_REFERENCE ? pl->d.win_begin : 0;
+    if ((rc = fill<T>(ring_slot(t_hi - 1), T(0), state, st))) return rc;
     if ((rc = fill<T>(OM, T(0), state, st))) return rc;
     if ((rc = fill<double>(G, 0.0, state, st))) return rc;
 
@@ -611,7 +633,10 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     T* SEG = (T*)(ws + L.SEG);
     for (int k = L.nseg - 1; k >= 0; --k) {
         const int s0 = k * L.S;
-        const int len = std::min(L.S, nt - s0);
+        const int end = std::min(std::min(L.S, nt - s0) + s0, t_hi);  // last adjoint step + 1 in this segment
+        const int beg = std::max(s0, t_lo);
+        if (end <= beg) continue;
+        const int len = end - s0;
         if (!L.full_history) {
             // recompute u^{s0+1..s0+len} from checkpoint k into SEG[1..len]
             const T* slot0 = (T*)(ws + L.CKU) + (long)k * state;
@@ -624,8 +649,8 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
             }
         }
         const int useg0 = L.full_history ? 0 : s0;  // SEG slot j holds u^{useg0 + j}
-        for (int ce = s0 + len; ce > s0;) {
-            const int clen = std::min(L.C, ce - s0);
+        for (int ce = end; ce > beg;) {
+            const int clen = std::min(L.C, ce - beg);
             const int c0 = ce - clen;
             for (int it = ce - 1; it >= c0; --it) {
                 a.u_in = ring_slot(it);
@@ -716,8 +741,13 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     if (d->sponge_sides && !(d->sponge_width > 0)) return fail(SEM_E_INVALID, "sponge_width must be > 0");
     if (d->tile_rows != 0 && d->tile_rows != 15 && d->tile_rows != 7 && d->tile_rows != 3)
         return fail(SEM_E_UNSUPPORTED, "tile_rows must be 0 (default), 15, 7 or 3");
+    const int wend = d->win_end > 0 ? d->win_end : d->nt;
+    if (d->win_begin < 0 || wend > d->nt || d->win_begin >= wend)
+        return fail(SEM_E_INVALID, "misfit window must satisfy 0 <= win_begin < win_end <= nt");
     auto* pl = new sem_plan;
     pl->d = *d;
+    pl->d.win_end = wend;
+    pl->d.win_decim = d->win_decim > 1 ? d->win_decim : 1;
     const int P = 4;
     pl->NX = d->nelem_x * P + 1;
     pl->NZ = d->nelem_z * P + 1;
@@ -920,39 +950,104 @@ int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double
     return SEM_OK;
 }
 
-int sem_curve_velocity(sem_plan* pl, int nmodel, const double* ctrl, double v_in, double v_out, double tau,
-                       int samples, int newton_steps, double* c, void* c2, double* aux, void* stream) {
-    if (!pl || nmodel < 1 || nmodel > 65535 || !ctrl || !c2 || !aux)
-        return fail(SEM_E_INVALID, "sem_curve_velocity: bad argument");
-    if (samples < 2 || samples > CURVE_MAX_SAMPLES || newton_steps < 0)
-        return fail(SEM_E_INVALID, "sem_curve_velocity: samples must be in [2, 2048], newton_steps >= 0");
+static int curve_params(const sem_plan* pl, const sem_curve_desc* cd, int nmodel, CurveParams& p) {
+    if (!pl || !cd || nmodel < 1 || nmodel > 65535) return fail(SEM_E_INVALID, "sem_curve: bad argument");
+    if (cd->kind == SEM_CURVE_CLAMPED6) {
+        if (cd->nctrl != 6) return fail(SEM_E_INVALID, "sem_curve: the clamped curve has 6 control points");
+    } else if (cd->kind == SEM_CURVE_PERIODIC) {
+        if (cd->nctrl < 4 || cd->nctrl > SEM_CURVE_MAX_CTRL)
+            return fail(SEM_E_INVALID, "sem_curve: periodic curve needs 4..32 control points");
+    } else {
+        return fail(SEM_E_INVALID, "sem_curve: unknown curve kind");
+    }
+    if (cd->samples < 2 || cd->samples > CURVE_MAX_SAMPLES || cd->newton_steps < 0)
+        return fail(SEM_E_INVALID, "sem_curve: samples must be in [2, 2048], newton_steps >= 0");
+    if (cd->blend != SEM_BLEND_SMOOTH && cd->blend != SEM_BLEND_REPLACE)
+        return fail(SEM_E_INVALID, "sem_curve: unknown blend");
+    p.kind = cd->kind;
+    p.K = cd->nctrl;
+    p.S = cd->samples;
+    p.newton = cd->newton_steps;
+    p.blend = cd->blend;
+    p.v_in = cd->v_inside;
+    p.v_out = cd->v_outside;
+    p.tau = cd->tau;
+    p.bg = cd->background;
+    return SEM_OK;
+}
+
+int sem_curve_velocity_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, const double* ctrl, double* c,
+                          void* c2, double* aux, void* stream) {
+    CurveParams p;
+    int rc = curve_params(pl, cd, nmodel, p);
+    if (rc) return rc;
+    if (!ctrl || !c2 || !aux) return fail(SEM_E_INVALID, "sem_curve_velocity: null array");
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 1024), nmodel);
     if (pl->d.precision == SEM_FP64)
-        curve_velocity_kernel<double><<<grid, 256, 0, st>>>(ctrl, samples, newton_steps, v_in, v_out, tau,
-                                                            (double*)pl->d_x1d.p, (double*)pl->d_z1d.p, pl->NX,
-                                                            pl->NZ, c, (double*)c2, aux);
+        curve_velocity_kernel<double><<<grid, 256, 0, st>>>(p, ctrl, (double*)pl->d_x1d.p, (double*)pl->d_z1d.p,
+                                                            pl->NX, pl->NZ, c, (double*)c2, aux);
     else
-        curve_velocity_kernel<float><<<grid, 256, 0, st>>>(ctrl, samples, newton_steps, v_in, v_out, tau,
-                                                           (double*)pl->d_x1d.p, (double*)pl->d_z1d.p, pl->NX,
-                                                           pl->NZ, c, (float*)c2, aux);
+        curve_velocity_kernel<float><<<grid, 256, 0, st>>>(p, ctrl, (double*)pl->d_x1d.p, (double*)pl->d_z1d.p,
+                                                           pl->NX, pl->NZ, c, (float*)c2, aux);
     LAUNCH_CHECK();
     return SEM_OK;
 }
 
-int sem_curve_project(sem_plan* pl, int nmodel, const double* dL_dv, const double* aux, double v_in, double v_out,
-                      double tau, double* grad_ctrl, void* stream) {
-    if (!pl || nmodel < 1 || nmodel > 65535 || !dL_dv || !aux || !grad_ctrl)
-        return fail(SEM_E_INVALID, "sem_curve_project: bad argument");
+int sem_curve_project_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, const double* dL_dv,
+                         const double* aux, double* grad_ctrl, void* stream) {
+    CurveParams p;
+    int rc = curve_params(pl, cd, nmodel, p);
+    if (rc) return rc;
+    if (!dL_dv || !aux || !grad_ctrl) return fail(SEM_E_INVALID, "sem_curve_project: null array");
     hipStream_t st = (hipStream_t)stream;
-    const int nblk = (int)std::min<long>((pl->N + 255) / 256, 512);
+    const int nblk = (int)std::min<long>((pl->N + CURVE_PROJ_THREADS - 1) / CURVE_PROJ_THREADS, 1024);
+    const int w2 = 2 * (p.K + 3);
     double* partial = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * 18 * nblk * nmodel, st));
-    curve_project_kernel<<<dim3(nblk, nmodel), 256, 0, st>>>(dL_dv, aux, v_in, v_out, tau, pl->N, partial);
+    HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * w2 * nblk * nmodel, st));
+    curve_project_kernel<<<dim3(nblk, nmodel), CURVE_PROJ_THREADS, 0, st>>>(p, dL_dv, aux, pl->N, partial);
     LAUNCH_CHECK();
-    curve_project_finish_kernel<<<nmodel, 64, 0, st>>>(partial, nblk, grad_ctrl);
+    curve_project_finish_kernel<<<nmodel, 64, 0, st>>>(p, partial, nblk, grad_ctrl);
     LAUNCH_CHECK();
     HIP_TRY(hipFreeAsync(partial, st));
+    return SEM_OK;
+}
+
+int sem_curve_velocity(sem_plan* pl, int nmodel, const double* ctrl, double v_in, double v_out, double tau,
+                       int samples, int newton_steps, double* c, void* c2, double* aux, void* stream) {
+    sem_curve_desc cd{SEM_CURVE_CLAMPED6, 6, samples, newton_steps, v_in, v_out, tau, nullptr, SEM_BLEND_SMOOTH};
+    return sem_curve_velocity_ex(pl, &cd, nmodel, ctrl, c, c2, aux, stream);
+}
+
+int sem_curve_project(sem_plan* pl, int nmodel, const double* dL_dv, const double* aux, double v_in, double v_out,
+                      double tau, double* grad_ctrl, void* stream) {
+    sem_curve_desc cd{SEM_CURVE_CLAMPED6, 6, 2, 0, v_in, v_out, tau, nullptr, SEM_BLEND_SMOOTH};
+    return sem_curve_project_ex(pl, &cd, nmodel, dL_dv, aux, grad_ctrl, stream);
+}
+
+int sem_forward_snapshots(sem_plan* pl, const sem_batch* b, int every, int ix0, int ix1, int iz0, int iz1,
+                          void* snapshots, void* ws, size_t ws_bytes, void* stream) {
+    if (!pl || !snapshots) return fail(SEM_E_INVALID, "sem_forward_snapshots: null argument");
+    if (every < 1) return fail(SEM_E_INVALID, "sem_forward_snapshots: every must be >= 1");
+    if (ix0 < 0 || ix1 > pl->NX || ix0 >= ix1 || iz0 < 0 || iz1 > pl->NZ || iz0 >= iz1)
+        return fail(SEM_E_INVALID, "sem_forward_snapshots: node box outside the grid or empty");
+    SnapSpec sp{every, ix0, ix1 - ix0, iz0, iz1 - iz0, snapshots};
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? forward_impl<double>(pl, b, ws, ws_bytes, st, &sp)
+                                       : forward_impl<float>(pl, b, ws, ws_bytes, st, &sp);
+}
+
+int sem_grid_to_nodes(sem_plan* pl, int nxg, int nzg, const double* xg, const double* zg, const double* vp,
+                      int method, double* out, void* stream) {
+    if (!pl || !xg || !zg || !vp || !out) return fail(SEM_E_INVALID, "sem_grid_to_nodes: null argument");
+    if (nxg < 2 || nzg < 2) return fail(SEM_E_INVALID, "sem_grid_to_nodes: the grid needs >= 2 points per axis");
+    if (method != SEM_INTERP_LINEAR && method != SEM_INTERP_NEAREST)
+        return fail(SEM_E_INVALID, "sem_grid_to_nodes: unknown method");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned blocks = (unsigned)std::min<long>((pl->N + 255) / 256, 2048);
+    grid_to_nodes_kernel<<<blocks, 256, 0, st>>>(xg, nxg, zg, nzg, vp, method, (double*)pl->d_x1d.p,
+                                                (double*)pl->d_z1d.p, pl->NX, pl->NZ, out);
+    LAUNCH_CHECK();
     return SEM_OK;
 }
 
@@ -1062,10 +1157,10 @@ int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, dou
     HIP_TRY(hipStreamSynchronize(st));
     if (pl->d.precision == SEM_FP64)
         residual_kernel<double><<<1, 256, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
-                                                   (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch);
+                                                   (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     else
         residual_kernel<float><<<1, 256, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
-                                                  (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch);
+                                                  (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     LAUNCH_CHECK();
     return SEM_OK;
 }

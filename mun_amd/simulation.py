@@ -23,17 +23,30 @@ libmun_sem.so.  Differences a caller can see are opt-in config keys:
                           takes node values and run_forward_and_adjoint
                           accepts 'bspline_nodes' and returns
                           'grad_wrt_nodes'.
+  velocity.curve          'clamped6' | 'periodic': closed-curve
+                          parameterisation (default: 'periodic' when a
+                          gridded background is configured, else 'clamped6')
+
+Configurations of the CO2 variant of the solver
+(CO2_plume/one_stage/single_cubic/sem_waveform/core.py) are accepted as
+well: velocity.background_2d (or .background) gridded background with
+velocity.anomaly {enabled, control_points, perturbations, v_inside, blend},
+velocity.vmin/vmax, and method.misfit_window {enabled, tmin, tmax |
+window_len, decim}.  run() keeps ROI wavefield snapshots when
+output.save_wavefield is set (core.py:318-325, 400-401, 649-651, 1564-1581).
 Node-indexed attributes (``global_coords``, ``velocity_model``) use the
 reference's element-blocked node numbering (mesh.py:4-43).
 """
 from __future__ import annotations
 
 import math
+import time
 from typing import Optional
 
 import numpy as np
 import torch
 
+from . import io as sem_io
 from .solver import Plan, source_wavelet
 
 SIDES_ALL = ("xmin", "xmax", "zmin", "zmax")
@@ -118,49 +131,122 @@ class SEMSimulation:
         self.num_receivers = len(self.receiver_coords_sem)
         # velocity parameterisation
         self.grid = v.get("bspline_grid")
-        if self.grid is None:
-            self.vmin = float(v["inside_velocity"])
-            self.vmax = float(v["outside_velocity"])
-            cp = v.get("control_points")
-            self.ctrl_pts_original = None if cp is None else np.asarray(cp, dtype=np.float64).reshape(6, 2)
-            pt = v.get("perturbations")
-            self.true_offset = None if pt is None else np.asarray(pt, dtype=np.float64).reshape(6, 2)
+        bg_cfg = v.get("background_2d", v.get("background"))
+        self.background_cfg = bg_cfg if isinstance(bg_cfg, dict) else None
+        self.v_background = None      # reference node order (numpy)
+        self._bg_lex = None           # lexicographic (device)
+        anom = v.get("anomaly", {}) if isinstance(v.get("anomaly", {}), dict) else {}
+        if self.grid is None and self.background_cfg is not None:
+            # CO2 variant (C1S core.py:276-293, 353-447)
+            self.vmin = float(v.get("vmin", 2000.0))
+            self.vmax = float(v.get("vmax", 3000.0))
+            self.anomaly_enabled = bool(anom.get("enabled", True))
+            cp = anom.get("control_points", v.get("control_points"))
+            self.ctrl_pts_original = None if cp is None else np.asarray(cp, dtype=np.float64).reshape(-1, 2)
+            pt = anom.get("perturbations", v.get("perturbations"))
+            a = None if pt is None else np.asarray(pt, dtype=np.float64)
+            self.true_offset = None if (a is None or a.size == 0) else a.reshape(-1, 2)
+            self.v_inside = float(anom.get("v_inside", self.vmin))
+            self.blend = str(anom.get("blend", "smooth"))
+            self.curve_kind = str(v.get("curve", "periodic"))
             self.tau = float(v.get("tau", 10.0))
             self.spline_samples = int(v.get("spline_samples", 800))
-            self.c_max = self.vmax
+        elif self.grid is None:
+            self.vmin = float(v["inside_velocity"])
+            self.vmax = float(v["outside_velocity"])
+            self.anomaly_enabled = True
+            cp = v.get("control_points")
+            self.ctrl_pts_original = None if cp is None else np.asarray(cp, dtype=np.float64).reshape(-1, 2)
+            pt = v.get("perturbations")
+            self.true_offset = None if pt is None else np.asarray(pt, dtype=np.float64).reshape(-1, 2)
+            self.v_inside = self.vmin
+            self.blend = "smooth"
+            self.curve_kind = str(v.get("curve", "clamped6"))
+            self.tau = float(v.get("tau", 10.0))
+            self.spline_samples = int(v.get("spline_samples", 800))
         else:
             self.grid_coef = np.asarray(self.grid["coef"], dtype=np.float64)
             self.grid_base_ref = self.grid.get("base")  # node values in the reference numbering
-            self.c_max = float(v.get("outside_velocity", v.get("vmax", 3000.0)))
+        # output options (core.py:217-222, 318-325)
+        o = cfg["output"]
+        self.output_dir = o.get("output_dir", "sem_output")
+        self.save_wavefield = bool(o.get("save_wavefield", False))
+        self.save_seismograms = bool(o.get("save_seismograms", True))
+        self.roi_xmin = float(o.get("roi_xmin", self.xmin + 100.0))
+        self.roi_xmax = float(o.get("roi_xmax", self.xmax - 100.0))
+        self.roi_zmin = float(o.get("roi_zmin", self.zmin + 100.0))
+        self.roi_zmax = float(o.get("roi_zmax", self.zmax - 100.0))
+        self._snapshot_interval_cfg = o.get("snapshot_interval")
+        self.misfit_window_cfg = m.get("misfit_window")
         self._plans = {}
+        # the plan needs c_max for the sponge; a gridded background raises it
+        # to the background maximum (C1S core.py:417-418)
+        self.c_max = float(self.vmax) if self.grid is None else \
+            float(v.get("outside_velocity", v.get("vmax", 3000.0)))
+        if self.background_cfg is not None and self.grid is None:
+            xg, zg, vpg = sem_io.background_from_config(self.background_cfg, self.xmin, self.xmax, self.zmin,
+                                                        self.zmax)
+            self._bg_grid = (xg, zg, vpg, str(self.background_cfg.get("interp", "linear")))
+            self.vmin = float(min(self.vmin, np.nanmin(vpg)))
+            self.vmax = float(max(self.vmax, np.nanmax(vpg)))
+            self.c_max = self.vmax
         p0 = self._plan("post")
         self.npoints = p0.N
         self._perm = reference_node_order(self.nelem_x, self.nelem_z, self.npol)
         self.global_coords = p0.coords[self._perm]
         self.src_node = int(np.nonzero(self._perm == p0.source_node(self.src_x, self.src_z))[0][0])
         self.src_pos = self.global_coords[self.src_node]
+        if self.background_cfg is not None and self.grid is None:
+            xg, zg, vpg, meth = self._bg_grid
+            self._bg_lex = p0.grid_to_nodes(xg, zg, vpg, meth)
+            self.v_background = self._bg_lex.cpu().numpy()[self._perm]
         self.grid_base = None
         if self.grid is not None and self.grid_base_ref is not None:
             self.grid_base = np.empty(self.npoints)
             self.grid_base[self._perm] = np.asarray(self.grid_base_ref, dtype=np.float64).reshape(-1)
         # initial velocity and CFL adjustment, as __init__ does (core.py:241-282)
-        if self.grid is None and self.ctrl_pts_original is not None:
-            ctrl = self.ctrl_pts_original + (self.true_offset if self.true_offset is not None else 0.0)
-            c, _, _ = self._curve_velocity(ctrl)
-        elif self.grid is None:
-            c = torch.full((1, self.npoints), self.vmax, dtype=torch.float64, device=self.device)
+        if self.grid is None:
+            c, _, _ = self._model_velocity(self._ctrl_with_offset(None))
         else:
             c, _ = self._grid_velocity(self.grid_coef)
         self._set_velocity(c)
         self._adjust_dt(c)
 
-    def _plan(self, record: str) -> Plan:
-        key = (record, self.dt, self.nt)
+    def _ctrl_with_offset(self, offsets):
+        """Absolute control points: configured ones + offsets (or the configured
+        perturbations when offsets is None); None without an anomaly."""
+        if self.ctrl_pts_original is None or not self.anomaly_enabled:
+            return None
+        if offsets is None:
+            offsets = self.true_offset if self.true_offset is not None else 0.0
+        off = np.asarray(_np(offsets), dtype=np.float64)
+        return self.ctrl_pts_original + (off.reshape(-1, 2) if off.ndim else off)
+
+    def _window(self):
+        """Step window of method.misfit_window at the current dt/nt, or None
+        (C1S core.py:1440-1466)."""
+        mw = self.misfit_window_cfg
+        if not (mw and mw.get("enabled", False)):
+            return None
+        tmin = float(mw.get("tmin", 0.0) if mw.get("tmin") is not None else 0.0)
+        tmax = mw.get("tmax")
+        if tmax is None and mw.get("window_len") is not None:
+            tmax = tmin + float(mw["window_len"])
+        tmax = float(tmax) if tmax is not None else float(self.dt * (self.nt - 1))
+        if tmax <= tmin:
+            raise ValueError(f"misfit_window invalid: tmax({tmax}) <= tmin({tmin})")
+        decim = max(int(mw.get("decim", 1)), 1)
+        it0 = max(0, min(int(round(tmin / self.dt)), self.nt - 1))
+        it1 = max(it0 + 1, min(int(round(tmax / self.dt)) + 1, self.nt))
+        return it0, it1, decim
+
+    def _plan(self, record: str, window=None) -> Plan:
+        key = (record, self.dt, self.nt, window)
         if key not in self._plans:
             p = Plan(self.nelem_x, self.nelem_z, self.xmin, self.xmax, self.zmin, self.zmax, self.dt, self.nt,
                      self.npol, self.pml_thickness, self.c_max, 1e-8, self.sides, precision=self.precision,
                      adjoint_mode=self.adjoint_mode, record=record, checkpoint_every=self.checkpoint_every,
-                     device=self.device)
+                     device=self.device, misfit_window=window)
             p.add_receivers(np.asarray(self.receiver_coords_sem, dtype=np.float64))
             self._plans[key] = p
         return self._plans[key]
@@ -180,24 +266,33 @@ class SEMSimulation:
         self.velocity_model = c[0].cpu().numpy()[self._perm]
 
     # -------------------------------------------------------------- velocity
-    def _curve_velocity(self, ctrl6):
-        from . import _lib as L
-        plan = self._plan("post")
-        ctrl = torch.as_tensor(np.asarray(ctrl6, dtype=np.float64).reshape(1, 12), device=self.device)
-        c = torch.empty((1, plan.N), dtype=torch.float64, device=self.device)
-        c2 = torch.empty((1, plan.N), dtype=plan.dtype, device=self.device)
-        aux = torch.empty((1, 4, plan.N), dtype=torch.float64, device=self.device)
-        L.check(plan.lib.sem_curve_velocity(plan.h, 1, _p(ctrl), self.vmin, self.vmax, self.tau,
-                                            self.spline_samples, 7, _p(c), _p(c2), _p(aux),
-                                            _s(self.device)))
-        return c, c2, aux
+    def _curve_args(self):
+        bg = self._bg_lex
+        return dict(kind=self.curve_kind, v_in=self.v_inside, v_out=self.vmax, tau=self.tau, background=bg,
+                    blend=self.blend)
 
-    def _curve_project(self, dL_dv, aux):
-        from . import _lib as L
+    def _model_velocity(self, ctrl):
+        """Velocity of an absolute control-point set (None: background or
+        constant v_out); returns (c [1,N], c2 [1,N], aux or None)."""
         plan = self._plan("post")
-        g = torch.empty((1, 12), dtype=torch.float64, device=self.device)
-        L.check(plan.lib.sem_curve_project(plan.h, 1, _p(dL_dv.contiguous()), _p(aux), self.vmin, self.vmax,
-                                           self.tau, _p(g), _s(self.device)))
+        if ctrl is None:
+            if self._bg_lex is not None:
+                c = self._bg_lex[None].clone()
+            else:
+                c = torch.full((1, plan.N), self.vmax, dtype=torch.float64, device=self.device)
+            return c, (c * c).to(plan.dtype), None
+        a = self._curve_args()
+        return plan.curve_velocity(np.asarray(ctrl, dtype=np.float64).reshape(1, -1, 2), a["kind"], a["v_in"],
+                                   a["v_out"], a["tau"], self.spline_samples, 7, a["background"], a["blend"])
+
+    def _curve_velocity(self, ctrl):
+        return self._model_velocity(np.asarray(ctrl, dtype=np.float64).reshape(-1, 2))
+
+    def _curve_project(self, dL_dv, aux, nctrl):
+        plan = self._plan("post")
+        a = self._curve_args()
+        g = plan.curve_project(dL_dv, aux, nctrl, a["kind"], a["v_in"], a["v_out"], a["tau"], a["background"],
+                               a["blend"])
         return g[0].cpu().numpy()
 
     def _grid_velocity(self, coef):
@@ -225,59 +320,131 @@ class SEMSimulation:
         w = source_wavelet(self.nt, self.dt, self.src_f0, 0.1, 0.3, self.src_amp)
         return torch.as_tensor(w, dtype=plan.dtype, device=self.device)
 
+    def _snapshot_interval(self) -> int:
+        if self._snapshot_interval_cfg is not None:
+            return max(int(self._snapshot_interval_cfg), 1)
+        return max(200, self.nt // 100)  # core.py:400-401
+
+    def _roi(self, plan):
+        """ROI node box and the ROI nodes in the reference's order (the mask
+        of core.py:318-325 applied to its node numbering)."""
+        ix = np.nonzero((plan.x1d >= self.roi_xmin) & (plan.x1d <= self.roi_xmax))[0]
+        iz = np.nonzero((plan.z1d >= self.roi_zmin) & (plan.z1d <= self.roi_zmax))[0]
+        if ix.size == 0 or iz.size == 0:
+            return None, None
+        box = (int(ix[0]), int(ix[-1]) + 1, int(iz[0]), int(iz[-1]) + 1)
+        jz, jx = np.divmod(self._perm, plan.NX)
+        inside = (jx >= box[0]) & (jx < box[1]) & (jz >= box[2]) & (jz < box[3])
+        # position of each reference-ordered ROI node inside the box, row-major
+        local = (jz[inside] - box[2]) * (box[1] - box[0]) + (jx[inside] - box[0])
+        return box, local
+
     def run(self, ctrl_offsets_flat=None, use_torch: bool = True) -> dict:
         """Forward simulation; seismograms sampled before each update
-        (core.py:334-435, 598-659).  For a closed-curve model the argument is
-        the (12,) control-point offset; for a B-spline grid it is the node
-        values (nbz, nbx) (default: the configured ones)."""
+        (core.py:334-543, 598-659).  For a closed-curve model the argument is
+        the control-point offset (2K,) (default: the configured
+        perturbations); for a B-spline grid it is the node values (nbz, nbx)
+        (default: the configured ones).  With output.save_wavefield the ROI
+        snapshots are returned and written to wavefield_snapshots.npz."""
+        t0 = time.perf_counter()
         if self.grid is None:
-            if ctrl_offsets_flat is None and self.true_offset is not None:
-                ctrl_offsets_flat = self.true_offset
-            off = np.zeros((6, 2)) if ctrl_offsets_flat is None else \
-                np.asarray(_np(ctrl_offsets_flat), dtype=np.float64).reshape(6, 2)
-            c, c2, _ = self._curve_velocity(self.ctrl_pts_original + off)
+            c, c2, _ = self._model_velocity(self._ctrl_with_offset(ctrl_offsets_flat))
         else:
             coef = self.grid_coef if ctrl_offsets_flat is None else np.asarray(_np(ctrl_offsets_flat))
             c, c2 = self._grid_velocity(coef)
         self._adjust_dt(c)
         self._set_velocity(c)
         plan = self._plan("pre")
-        seis = plan.forward(c2, self._wavelet(plan), [plan.source_node(self.src_x, self.src_z)], [0])
-        return {"dt": float(self.dt), "nt": int(self.nt),
-                "receiver_data": seis[0].double().cpu().numpy(),
-                "receiver_coords": self.receiver_coords_sem}
+        src = [plan.source_node(self.src_x, self.src_z)]
+        snaps = None
+        if self.save_wavefield:
+            box, local = self._roi(plan)
+            if box is not None:
+                seis, s = plan.forward_snapshots(c2, self._wavelet(plan), src, self._snapshot_interval(), box, [0])
+                s = s[0].reshape(s.shape[1], -1).cpu().numpy()
+                snaps = [np.asarray(row[local], dtype=np.float32) for row in s]
+            else:
+                seis = plan.forward(c2, self._wavelet(plan), src, [0])
+                snaps = []
+        else:
+            seis = plan.forward(c2, self._wavelet(plan), src, [0])
+        data = seis[0].double().cpu().numpy()
+        results = {"receiver_data": data if self.save_seismograms else None,
+                   "wavefield_snapshots": snaps if self.save_wavefield else None,
+                   "computation_time": time.perf_counter() - t0,
+                   "velocity_model": self.velocity_model, "global_coords": self.global_coords,
+                   "dt": float(self.dt), "nt": int(self.nt), "receiver_coords": self.receiver_coords_sem}
+        if self.save_wavefield and snaps:
+            sem_io.save_wavefield_snapshots(self.output_dir, snaps, self.dt, self.nt, self.xmin, self.xmax,
+                                            self.zmin, self.zmax)
+        return results
+
+    def _ctrl_param(self, params_dict):
+        ctrl = np.asarray(_np(params_dict["bspline_ctrl"]), dtype=np.float64)
+        if ctrl.ndim == 1:
+            if ctrl.size % 2:
+                raise ValueError(f"bspline_ctrl 1D array length {ctrl.size} is not even; cannot reshape to (K,2)")
+            ctrl = ctrl.reshape(-1, 2)
+        elif not (ctrl.ndim == 2 and ctrl.shape[1] == 2):
+            raise ValueError(f"bspline_ctrl must be shape (2*K,) or (K,2), got {ctrl.shape}")
+        if self.curve_kind == "clamped6" and ctrl.shape[0] != 6:
+            raise ValueError(f"the clamped closed curve takes 6 control points, got {ctrl.shape[0]}")
+        return ctrl
+
+    def _observations(self, plan, y_obs, obs_dt, obs_nt, window):
+        """Observed data on the plan's nt rows.  Without a window: linear
+        resampling onto the simulation's time axis (core.py:109-128).  With a
+        window: y_obs must hold the windowed, decimated samples (dt * decim,
+        (it1 - it0 + decim - 1) // decim rows), the only case the windowed
+        reference compares directly (C1S core.py:1497-1503); they are placed on
+        rows it0, it0 + decim, ..."""
+        if window is None:
+            return self._align_observation_data(y_obs, obs_dt, obs_nt)
+        it0, it1, decim = window
+        nt_dec = (it1 - it0 + decim - 1) // decim
+        y = np.asarray(y_obs, dtype=np.float64)
+        if not (abs(self.dt * decim - obs_dt) < 1e-12 and nt_dec == obs_nt and y.shape[0] == nt_dec):
+            raise ValueError(f"misfit window: y_obs must have {nt_dec} rows at dt={self.dt * decim:g} "
+                             f"(got {y.shape[0]} rows, obs_dt={obs_dt:g}, obs_nt={obs_nt})")
+        out = np.zeros((self.nt, y.shape[1]))
+        out[it0:it1:decim] = y
+        return out
 
     def run_forward_and_adjoint(self, params_dict: dict) -> dict:
-        """Forward + adjoint log-likelihood and gradient (core.py:1067-1371).
+        """Forward + adjoint log-likelihood and gradient (core.py:1067-1371;
+        C1S core.py:1297-1705 with a misfit window).
 
-        Closed-curve model: params 'bspline_ctrl' (12,) or (6,2) absolute
-        control points -> 'grad_wrt_ctrl' (12,) interleaved [x0,z0,...].
+        Closed-curve model: params 'bspline_ctrl' (2K,) or (K,2) absolute
+        control points -> 'grad_wrt_ctrl' (2K,) interleaved [x0,z0,...].
         B-spline grid model: params 'bspline_nodes' (nbz, nbx) ->
         'grad_wrt_nodes' (nbz, nbx).  Both return 'loglik', 'clean_data'
-        (nt, nrec), 'dt', 'nt'."""
-        y_obs = np.asarray(params_dict["y_obs"], dtype=np.float64)
+        (nt or window rows, nrec), 'dt', 'nt'."""
+        y_obs = np.asarray(_np(params_dict["y_obs"]), dtype=np.float64)
         obs_dt = float(params_dict["obs_dt"])
         obs_nt = int(params_dict["obs_nt"])
         sigma = float(params_dict.get("noise_std", 0.0))
         aux = None
         if self.grid is None:
-            ctrl = np.asarray(_np(params_dict["bspline_ctrl"]), dtype=np.float64).reshape(6, 2)
-            c, c2, aux = self._curve_velocity(ctrl)
+            ctrl = self._ctrl_param(params_dict)
+            c, c2, aux = self._model_velocity(ctrl)
         else:
             c, c2 = self._grid_velocity(_np(params_dict.get("bspline_nodes", self.grid_coef)))
         self._adjust_dt(c)
         self._set_velocity(c)
-        plan = self._plan("post")
-        y = self._align_observation_data(y_obs, obs_dt, obs_nt)
+        window = self._window()
+        plan = self._plan("post", window)
+        y = self._observations(plan, y_obs, obs_dt, obs_nt, window)
         obs = torch.as_tensor(y, dtype=plan.dtype, device=self.device)[None]
         out = plan.forward_adjoint(c2, self._wavelet(plan), [plan.source_node(self.src_x, self.src_z)], obs,
                                    sigma, [0])
-        res = {"loglik": float(out["loglik"][0]), "clean_data": out["seis"][0].double().cpu().numpy(),
-               "dt": float(self.dt), "nt": int(self.nt)}
+        d_syn = out["seis"][0].double().cpu().numpy()[:, :self.num_receivers]
+        if window is not None:
+            d_syn = d_syn[window[0]:window[1]]
+        res = {"loglik": float(out["loglik"][0]), "clean_data": d_syn, "dt": float(self.dt), "nt": int(self.nt)}
         g_c = out["grad_c"]  # d loglik / dc
         if self.grid is None:
             # the reference hands dL/dv = -dJ_dv to the projection (core.py:1278-1287)
-            res["grad_wrt_ctrl"] = self._curve_project(-g_c, aux)
+            res["grad_wrt_ctrl"] = self._curve_project(-g_c, aux, ctrl.shape[0])
         else:
             nbz, nbx = self.grid_coef.shape
             res["grad_wrt_nodes"] = plan.bspline_project(g_c, nbz, nbx)[0].cpu().numpy()
@@ -288,7 +455,7 @@ class SEMSimulation:
         """log-likelihood of a forward run with pre-update sampling, as the
         reference's finite-difference check computes it (core.py:1440-1492)."""
         if self.grid is None:
-            c, c2, _ = self._curve_velocity(np.asarray(_np(ctrl12), dtype=np.float64).reshape(6, 2))
+            c, c2, _ = self._model_velocity(np.asarray(_np(ctrl12), dtype=np.float64).reshape(-1, 2))
         else:
             c, c2 = self._grid_velocity(_np(ctrl12))
         plan = self._plan("pre")

@@ -17,6 +17,8 @@ from . import _lib as L
 PRECISIONS = {"fp32": (L.SEM_FP32, torch.float32), "fp64": (L.SEM_FP64, torch.float64)}
 ADJ_MODES = {"reference": L.SEM_ADJ_REFERENCE, "exact": L.SEM_ADJ_EXACT}
 RECORD = {"post": L.SEM_RECORD_POST, "pre": L.SEM_RECORD_PRE}
+CURVE_KINDS = {"clamped6": L.SEM_CURVE_CLAMPED6, "periodic": L.SEM_CURVE_PERIODIC}
+BLENDS = {"smooth": L.SEM_BLEND_SMOOTH, "replace": L.SEM_BLEND_REPLACE}
 
 
 def ricker(t, f0, t0=0.1):
@@ -58,7 +60,7 @@ class Plan:
                  sponge_R: float = 1e-8, sponge_sides: Sequence[str] = ("xmin", "xmax", "zmin", "zmax"),
                  precision: str = "fp32", adjoint_mode: str = "reference", record: str = "post",
                  checkpoint_every: Optional[int] = 100, lagrange_eps: float = 1e-12, device=None,
-                 tile_rows: int = 0):
+                 tile_rows: int = 0, misfit_window: Optional[Sequence[int]] = None):
         if not torch.cuda.is_available():
             raise RuntimeError("mun_amd.Plan needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device if device is not None else "cuda")
@@ -67,9 +69,12 @@ class Plan:
         sides = 0
         for s in sponge_sides:
             sides |= L.SIDE_BITS[s]
+        wb, we, wd = (0, 0, 1) if misfit_window is None else (int(q) for q in misfit_window)
         d = L.SemDesc(nelem_x, nelem_z, order, xmin, xmax, zmin, zmax, dt, nt, sponge_width, sponge_cmax,
                       sponge_R, sides, code, ADJ_MODES[adjoint_mode], RECORD[record],
-                      0 if checkpoint_every is None else int(checkpoint_every), lagrange_eps, int(tile_rows))
+                      0 if checkpoint_every is None else int(checkpoint_every), lagrange_eps, int(tile_rows),
+                      wb, we, wd)
+        self.misfit_window = None if misfit_window is None else (wb, we or int(nt), max(wd, 1))
         self.desc = d
         self.precision, self.adjoint_mode, self.record = precision, adjoint_mode, record
         self.tile_rows = int(tile_rows) or 15
@@ -142,6 +147,64 @@ class Plan:
                                          _stream_ptr(self.device)))
         return c, c2
 
+    def grid_to_nodes(self, xg, zg, vp, method: str = "linear") -> torch.Tensor:
+        """Regular-grid background vp[nzg, nxg] on increasing axes xg, zg onto
+        the nodes (bilinear or nearest, coordinates clipped into the grid);
+        float64 [N] in lexicographic node order."""
+        dev = self.device
+        xg = torch.as_tensor(np.asarray(xg, dtype=np.float64).reshape(-1), device=dev)
+        zg = torch.as_tensor(np.asarray(zg, dtype=np.float64).reshape(-1), device=dev)
+        vp = torch.as_tensor(np.asarray(vp, dtype=np.float64), device=dev).contiguous()
+        if vp.shape != (zg.numel(), xg.numel()):
+            raise ValueError(f"vp grid shape {tuple(vp.shape)} must be (len(zg), len(xg))")
+        if bool((xg[1:] <= xg[:-1]).any()) or bool((zg[1:] <= zg[:-1]).any()):
+            raise ValueError("grid axes must be strictly increasing")
+        meth = {"linear": L.SEM_INTERP_LINEAR, "nearest": L.SEM_INTERP_NEAREST}[method]
+        out = torch.empty(self.N, dtype=torch.float64, device=dev)
+        L.check(self.lib.sem_grid_to_nodes(self.h, xg.numel(), zg.numel(), _ptr(xg), _ptr(zg), _ptr(vp), meth,
+                                           _ptr(out), _stream_ptr(dev)))
+        return out
+
+    def _curve_desc(self, kind, nctrl, v_in, v_out, tau, samples, newton, background, blend):
+        cd = L.SemCurveDesc(CURVE_KINDS[kind], int(nctrl), int(samples), int(newton), float(v_in), float(v_out),
+                            float(tau), _ptr(background), BLENDS[blend])
+        cd._keep = background
+        return cd
+
+    def curve_velocity(self, ctrl, kind: str = "clamped6", v_in: float = 2000.0, v_out: float = 3000.0,
+                       tau: float = 10.0, samples: int = 800, newton_steps: int = 7,
+                       background: Optional[torch.Tensor] = None, blend: str = "smooth"):
+        """Closed-curve anomaly velocity for ctrl [M, K, 2] (absolute control
+        points); returns (c float64 [M,N], c2 [M,N], aux float64 [M,4,N])."""
+        ctrl = np.asarray(ctrl.detach().cpu() if isinstance(ctrl, torch.Tensor) else ctrl, dtype=np.float64)
+        ctrl = ctrl.reshape(1, -1, 2) if ctrl.ndim < 3 else ctrl
+        M, K = ctrl.shape[0], ctrl.shape[1]
+        ctrl = torch.as_tensor(np.ascontiguousarray(ctrl), device=self.device)
+        if background is not None:
+            background = torch.as_tensor(background, dtype=torch.float64, device=self.device).contiguous()
+        cd = self._curve_desc(kind, K, v_in, v_out, tau, samples, newton_steps, background, blend)
+        c = torch.empty((M, self.N), dtype=torch.float64, device=self.device)
+        c2 = torch.empty((M, self.N), dtype=self.dtype, device=self.device)
+        aux = torch.empty((M, 4, self.N), dtype=torch.float64, device=self.device)
+        L.check(self.lib.sem_curve_velocity_ex(self.h, C.byref(cd), M, _ptr(ctrl), _ptr(c), _ptr(c2), _ptr(aux),
+                                               _stream_ptr(self.device)))
+        return c, c2, aux
+
+    def curve_project(self, dL_dv, aux, nctrl: int, kind: str = "clamped6", v_in: float = 2000.0,
+                      v_out: float = 3000.0, tau: float = 10.0, background: Optional[torch.Tensor] = None,
+                      blend: str = "smooth") -> torch.Tensor:
+        """Analytic projection of dL/dv [M,N] onto the control points: [M, 2K]
+        interleaved [x0, z0, x1, z1, ...]."""
+        dL_dv = torch.as_tensor(dL_dv, dtype=torch.float64, device=self.device).reshape(-1, self.N).contiguous()
+        M = dL_dv.shape[0]
+        if background is not None:
+            background = torch.as_tensor(background, dtype=torch.float64, device=self.device).contiguous()
+        cd = self._curve_desc(kind, nctrl, v_in, v_out, tau, 2, 0, background, blend)
+        g = torch.empty((M, 2 * int(nctrl)), dtype=torch.float64, device=self.device)
+        L.check(self.lib.sem_curve_project_ex(self.h, C.byref(cd), M, _ptr(dL_dv), _ptr(aux.contiguous()), _ptr(g),
+                                              _stream_ptr(self.device)))
+        return g
+
     def bspline_project(self, g_c: torch.Tensor, nbz: int, nbx: int) -> torch.Tensor:
         """g_coef = Bz^T g_c Bx (chain rule of ``bspline_map``)."""
         g_c = g_c.to(device=self.device, dtype=torch.float64).contiguous()
@@ -213,6 +276,25 @@ class Plan:
         bt = self._batch(c2, wavelet, src, rec_sets, model_of, seis, pitch)
         L.check(self.lib.sem_forward(self.h, C.byref(bt), _ptr(ws), ws.numel(), _stream_ptr(self.device)))
         return seis
+
+    def forward_snapshots(self, c2, wavelet, src_nodes, every: int, box, rec_sets=None, model_of=None):
+        """Forward solve that also keeps u after steps 0, every, 2 every, ...
+        on the node box (ix0, ix1, iz0, iz1); returns (seis [B, nt, nrec_max],
+        snapshots [B, nsnap, iz1-iz0, ix1-ix0])."""
+        c2, wavelet, src, rec_sets, model_of, pitch = self._prep(c2, wavelet, src_nodes, rec_sets, model_of)
+        B = len(rec_sets)
+        ix0, ix1, iz0, iz1 = (int(q) for q in box)
+        every = int(every)
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        nsnap = (self.nt + every - 1) // every
+        seis = torch.empty((B, self.nt, pitch), dtype=self.dtype, device=self.device)
+        snaps = torch.empty((B, nsnap, max(iz1 - iz0, 0), max(ix1 - ix0, 0)), dtype=self.dtype, device=self.device)
+        ws = self.workspace(B, pitch, False)
+        bt = self._batch(c2, wavelet, src, rec_sets, model_of, seis, pitch)
+        L.check(self.lib.sem_forward_snapshots(self.h, C.byref(bt), every, ix0, ix1, iz0, iz1, _ptr(snaps),
+                                               _ptr(ws), ws.numel(), _stream_ptr(self.device)))
+        return seis, snaps
 
     def forward_adjoint(self, c2, wavelet, src_nodes, obs, noise_std, rec_sets=None, model_of=None,
                         want_grad_c: bool = True, want_grad_c2: bool = False):

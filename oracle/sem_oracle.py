@@ -58,6 +58,8 @@ __all__ = [
     "sensitivity_dc2", "misfit_loglik", "forward_adjoint",
     "bspline_basis_matrix", "tensor_bspline_velocity", "layered_background", "greville", "clamped_knots",
     "closed_curve_velocity", "closed_curve_projection",
+    "forward_snapshots", "roi_box", "window_mask", "window_from_times",
+    "grid_background", "increasing_grid", "periodic_curve_velocity", "periodic_curve_projection",
 ]
 
 
@@ -492,7 +494,7 @@ class _History:
 
 def forward(disc: Discretization, c: np.ndarray, record: str = "post",
             history: Optional[str] = "full", checkpoint_every: Optional[int] = None,
-            K: Optional[sp.csr_matrix] = None):
+            K: Optional[sp.csr_matrix] = None, after_step=None):
     """Damped leapfrog with point source.
 
     u~^{n+1} = D o [2u~^n - D o u~^{n-1} + dt^2 M^-1 (s_n M_src e_src - K u~^n)]
@@ -502,6 +504,7 @@ def forward(disc: Discretization, c: np.ndarray, record: str = "post",
     ``record='post'`` samples R u~^{n+1} after the rotation as
     run_forward_and_adjoint does (S/core.py:1211,1218-1219); ``'pre'`` samples
     R u~^n before the update as run() does (S/core.py:633-639; quirk Q4).
+    ``after_step(it, u)`` (optional) sees u~^{it+1} after every step.
     Returns (seismograms (nt, nrec), history or None)."""
     p = disc.prob
     if K is None:
@@ -538,7 +541,37 @@ def forward(disc: Discretization, c: np.ndarray, record: str = "post",
             seis[it] = R @ u_curr
         if hist is not None:
             hist.put(it, u_prev, u_curr)
+        if after_step is not None:
+            after_step(it, u_curr)
     return seis, hist
+
+
+def forward_snapshots(disc: Discretization, c: np.ndarray, every: int, box, record: str = "pre"):
+    """Wavefield snapshots of run() (S/core.py:318-325, 400-401, 649-651):
+    after step it with it % every == 0 the state u~^{it+1} restricted to the
+    ROI.  The ROI mask (x, z inside a rectangle) is a node box on the
+    structured grid: box = (ix0, ix1, iz0, iz1), half-open.  Returns
+    (seis, snapshots (nsnap, iz1-iz0, ix1-ix0))."""
+    ix0, ix1, iz0, iz1 = box
+    NX, NZ = disc.mesh.NX, disc.mesh.NZ
+    snaps = []
+
+    def grab(it, u):
+        if it % every == 0:
+            snaps.append(u.reshape(NZ, NX)[iz0:iz1, ix0:ix1].copy())
+
+    seis, _ = forward(disc, c, record=record, history=None, after_step=grab)
+    return seis, np.stack(snaps)
+
+
+def roi_box(mesh: StructuredMesh, xmin: float, xmax: float, zmin: float, zmax: float):
+    """Node box of the ROI mask (x >= xmin) & (x <= xmax) & (z >= zmin) &
+    (z <= zmax) (S/core.py:318-325); None if empty."""
+    ix = np.nonzero((mesh.x1d >= xmin) & (mesh.x1d <= xmax))[0]
+    iz = np.nonzero((mesh.z1d >= zmin) & (mesh.z1d <= zmax))[0]
+    if ix.size == 0 or iz.size == 0:
+        return None
+    return int(ix[0]), int(ix[-1]) + 1, int(iz[0]), int(iz[-1]) + 1
 
 
 def sensitivity_dc2(u: np.ndarray, lam: np.ndarray, mesh: StructuredMesh, ops) -> np.ndarray:
@@ -564,7 +597,7 @@ def misfit_loglik(seis: np.ndarray, y_obs: np.ndarray, noise_std: float):
 
 
 def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.ndarray,
-            sigma2: float, mode: str = "reference", K: Optional[sp.csr_matrix] = None):
+            sigma2: float, mode: str = "reference", K: Optional[sp.csr_matrix] = None, stop: int = 0):
     """Reverse-time adjoint and gradient correlation.
 
     reference mode (S/core.py:1247-1275; quirk Q5, no sponge):
@@ -573,6 +606,8 @@ def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.nd
         lam_{k-1} = D o [2 lam_k - D o lam_{k+1} + dt^2 M^-1 (R^T r_k/sigma^2 - K lam_k)]
     both with lam_{nt-1} = lam_nt = 0 and
         dJ_dv = 2c o sum_k sens(u~^{k+1}, lam_k)      (S/core.py:1265-1270).
+    The loop runs it = nt-1 .. stop; ``stop`` > 0 is the misfit-window
+    truncation of C1S/core.py:1530-1560, whose adjoint covers the window only.
     Returns dJ_dv, which equals d loglik / dc (node order of ``disc.mesh``)."""
     p = disc.prob
     if K is None:
@@ -587,7 +622,7 @@ def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.nd
     lam_next = np.zeros(N)
     g = np.zeros(N)
     exact = (mode == "exact")
-    for it in range(p.nt - 1, -1, -1):
+    for it in range(p.nt - 1, stop - 1, -1):
         rt = RT @ rs[it]
         rhs = rt - K @ lam_curr
         if exact:
@@ -600,14 +635,49 @@ def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.nd
     return 2.0 * c * g
 
 
+def window_mask(nt: int, window) -> np.ndarray:
+    """Rows of the misfit window (C1S/core.py:1433-1470): it0 <= t < it1 and
+    (t - it0) % decim == 0; window = (it0, it1, decim) or None (all rows)."""
+    m = np.zeros(nt, dtype=bool)
+    if window is None:
+        m[:] = True
+        return m
+    it0, it1, decim = window
+    m[it0:it1:max(int(decim), 1)] = True
+    return m
+
+
+def window_from_times(dt: float, nt: int, tmin: float = 0.0, tmax: Optional[float] = None,
+                      window_len: Optional[float] = None, decim: int = 1):
+    """Step window of method.misfit_window (C1S/core.py:1440-1466)."""
+    if tmax is None and window_len is not None:
+        tmax = tmin + float(window_len)
+    tmax = float(tmax) if tmax is not None else float(dt * (nt - 1))
+    if tmax <= tmin:
+        raise ValueError(f"misfit_window invalid: tmax({tmax}) <= tmin({tmin})")
+    decim = max(int(decim), 1)
+    it0 = int(round(tmin / dt))
+    it1 = int(round(tmax / dt)) + 1
+    it0 = max(0, min(it0, nt - 1))
+    it1 = max(it0 + 1, min(it1, nt))
+    return it0, it1, decim
+
+
 def forward_adjoint(disc: Discretization, c: np.ndarray, y_obs: np.ndarray, noise_std: float,
-                    mode: str = "reference", checkpoint_every: Optional[int] = None):
+                    mode: str = "reference", checkpoint_every: Optional[int] = None, window=None):
     """run_forward_and_adjoint's node-level part (S/core.py:1067-1278):
-    returns dict(loglik, dloglik_dc (N,), seis (nt,nrec), residual)."""
+    returns dict(loglik, dloglik_dc (N,), seis (nt,nrec), residual).
+
+    ``window`` = (it0, it1, decim) restricts the misfit to the windowed,
+    decimated rows (C1S/core.py:1433-1520; y_obs is given on all nt rows and
+    read on the windowed ones); in the reference mode the adjoint then stops
+    at it0 as the reference's does."""
     K = stiffness_csr(c, disc.mesh, disc.ops)
     seis, hist = forward(disc, c, record="post", checkpoint_every=checkpoint_every, K=K)
-    loglik, r, sigma2 = misfit_loglik(seis, y_obs, noise_std)
-    g = adjoint(disc, c, hist, r, sigma2, mode=mode, K=K)
+    keep = window_mask(disc.prob.nt, window)
+    loglik, r, sigma2 = misfit_loglik(seis * keep[:, None], y_obs * keep[:, None], noise_std)
+    stop = window[0] if (window is not None and mode == "reference") else 0
+    g = adjoint(disc, c, hist, r, sigma2, mode=mode, K=K, stop=stop)
     return {"loglik": loglik, "dloglik_dc": g, "seis": seis, "residual": r, "sigma2": sigma2}
 
 
@@ -749,4 +819,126 @@ def closed_curve_projection(dL_dv: np.ndarray, sd, ust, nout, v_in, v_out, tau):
     gz6 = np.concatenate([[gz[0] + gz[6] + gz[7] + gz[8]], gz[1:6]])
     out[0::2] = gx6
     out[1::2] = gz6
+    return out
+
+
+# -- gridded background (C1S/velocity.py:212-274, C1S/core.py:353-415) -------
+def grid_background(coords: np.ndarray, xg: np.ndarray, zg: np.ndarray, vp_grid: np.ndarray,
+                    method: str = "linear", clip: bool = True) -> np.ndarray:
+    """Background velocity on nodes from vp_grid[iz, ix] on increasing axes:
+    scipy RegularGridInterpolator over (z, x) at the node coordinates, clipped
+    into the grid (build_background_from_grid, C1S/velocity.py:212-274)."""
+    from scipy.interpolate import RegularGridInterpolator
+    X = np.asarray(coords, dtype=np.float64).reshape(-1, 2)
+    xg = np.asarray(xg, dtype=np.float64).reshape(-1)
+    zg = np.asarray(zg, dtype=np.float64).reshape(-1)
+    vp_grid = np.asarray(vp_grid, dtype=np.float64)
+    if vp_grid.shape != (zg.size, xg.size):
+        raise ValueError(f"vp_grid shape {vp_grid.shape} must be (len(zg), len(xg))")
+    x, z = X[:, 0].copy(), X[:, 1].copy()
+    if clip:
+        x = np.clip(x, xg[0], xg[-1])
+        z = np.clip(z, zg[0], zg[-1])
+    rgi = RegularGridInterpolator((zg, xg), vp_grid, method=str(method), bounds_error=False, fill_value=None)
+    return rgi(np.column_stack([z, x])).astype(np.float64)
+
+
+def increasing_grid(xg, zg, vp_grid):
+    """Flip decreasing axes together with the grid (C1S/core.py:396-403)."""
+    xg = np.asarray(xg, dtype=np.float64).reshape(-1)
+    zg = np.asarray(zg, dtype=np.float64).reshape(-1)
+    vp_grid = np.asarray(vp_grid, dtype=np.float64)
+    if xg[0] > xg[-1]:
+        xg, vp_grid = xg[::-1], vp_grid[:, ::-1]
+    if zg[0] > zg[-1]:
+        zg, vp_grid = zg[::-1], vp_grid[::-1, :]
+    return xg, zg, vp_grid
+
+
+# -- periodic closed curve over a background (C1S/velocity.py:9-152, 277-332) --
+def _periodic_curve(ctrl: np.ndarray, k: int = 3):
+    """ctrl extended by its first k points, knots 0, 1, ..., len + k
+    (C1S/velocity.py:9-35)."""
+    from scipy.interpolate import BSpline
+    ctrl = np.asarray(ctrl, dtype=np.float64).reshape(-1, 2)
+    P = np.vstack([ctrl, ctrl[:k]])
+    knots = np.arange(0.0, len(P) + k + 1, dtype=np.float64)
+    return BSpline(knots, P, k, extrapolate=False), knots, P
+
+
+def periodic_curve_velocity(coords: np.ndarray, ctrl: np.ndarray, v_in: float, v_bg, tau: float,
+                            samples: int = 800, newton_steps: int = 7, blend: str = "smooth"):
+    """Signed distance to the periodic closed cubic B-spline by nearest of
+    `samples` samples on [3, K+3) (endpoint excluded) + Newton (clipped to
+    [3, nextafter(K+3, 3)]), sign from the sampled polygon, u* normalised to
+    [0, 1) (C1S/velocity.py:54-152); blended over the background v_bg (scalar
+    or per node): smooth v = v_bg + (v_in - v_bg) sigmoid(-sd/tau), replace
+    v = v_in inside (C1S/velocity.py:277-332).  Returns (v, sd, u_star, n_out)."""
+    from matplotlib.path import Path
+    spl, knots, _ = _periodic_curve(ctrl)
+    s1, s2 = spl.derivative(1), spl.derivative(2)
+    k = 3
+    lo, hi = float(knots[k]), float(knots[-(k + 1)])
+    hi_safe = np.nextafter(hi, lo)
+    u0 = np.linspace(lo, hi, samples, endpoint=False, dtype=np.float64)
+    C0 = spl(u0)
+    path = Path(C0, closed=True)
+    X = np.asarray(coords, dtype=np.float64).reshape(-1, 2)
+    n = len(X)
+    sd, ust, nout = np.empty(n), np.empty(n), np.empty((n, 2))
+    for s in range(0, n, 20000):
+        Xc = X[s:s + 20000]
+        d2 = ((Xc[:, None, :] - C0[None, :, :]) ** 2).sum(-1)
+        u = u0[np.argmin(d2, axis=1)]
+        for _ in range(newton_steps):
+            r = spl(u) - Xc
+            cp, cpp = s1(u), s2(u)
+            f1 = (r * cp).sum(-1)
+            f2 = (cp * cp).sum(-1) + (r * cpp).sum(-1)
+            f2 = np.where(np.abs(f2) < 1e-14, 1e-14, f2)
+            u = np.clip(u - f1 / f2, lo, hi_safe)
+        rv = Xc - spl(u)
+        dist = np.linalg.norm(rv, axis=1)
+        sign = np.where(path.contains_points(Xc, radius=1e-12), -1.0, 1.0)
+        sd[s:s + 20000] = sign * dist
+        ust[s:s + 20000] = (u - lo) / max(hi - lo, 1e-300)
+        nout[s:s + 20000] = sign[:, None] * rv / (dist[:, None] + 1e-15)
+    v_bg = np.broadcast_to(np.asarray(v_bg, dtype=np.float64), (n,))
+    if blend == "replace":
+        v = v_bg.copy()
+        v[sd < 0.0] = float(v_in)
+    else:
+        x = np.clip(sd / max(float(tau), 1e-12), -50.0, 50.0)
+        sig = 1.0 / (1.0 + np.exp(x))
+        v = v_bg + (float(v_in) - v_bg) * sig
+    return v, sd, ust, nout
+
+
+def periodic_curve_projection(dL_dv: np.ndarray, sd, ust, nout, v_in: float, v_bg, tau: float, K: int):
+    """Projection onto the K control points of the periodic curve, interleaved
+    [x0, z0, ...]: periodic basis at t = 3 + (u* mod 1) K, the 3 appended
+    controls folded back onto ctrl0..ctrl2, nodes kept where |dv/dsd| > 1e-8
+    max(1, |v_in - v_bg|)  (C1S/core.py:1075-1185)."""
+    from scipy.interpolate import BSpline
+    n = len(sd)
+    v_bg = np.broadcast_to(np.asarray(v_bg, dtype=np.float64), (n,))
+    ts = max(float(tau), 1e-12)
+    x = np.clip(np.asarray(sd) / ts, -50.0, 50.0)
+    sig = 1.0 / (1.0 + np.exp(x))
+    dv = (float(v_in) - v_bg) * sig * (1.0 - sig) * (-1.0 / ts)
+    mask = np.abs(dv) > 1e-8 * np.maximum(1.0, np.abs(float(v_in) - v_bg))
+    out = np.zeros(2 * K)
+    if not np.any(mask):
+        return out
+    w = (np.asarray(dL_dv)[mask] * dv[mask])[:, None]
+    knots = np.arange(0.0, K + 3 + 3 + 1, dtype=np.float64)
+    lo, hi = float(knots[3]), float(knots[-4])
+    t = np.clip(lo + np.mod(np.asarray(ust)[mask], 1.0) * (hi - lo), lo, np.nextafter(hi, lo))
+    Bc = np.stack([BSpline(knots, np.eye(K + 3)[j], 3, extrapolate=False)(t) for j in range(K + 3)], axis=1)
+    B = Bc[:, :K].copy()
+    B[:, :3] += Bc[:, K:]
+    gx = (w * B * nout[mask, 0:1]).sum(0)
+    gz = (w * B * nout[mask, 1:2]).sum(0)
+    out[0::2] = gx
+    out[1::2] = gz
     return out

@@ -1,0 +1,218 @@
+"""GPU parity of the wider drop-in surface against the FP64 oracle: wavefield
+snapshots, misfit window + decimation (both adjoint modes), gridded
+background, the periodic closed curve over a background, and the CO2-variant
+configuration through SEMSimulation (through the C ABI)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tests.gpu_helpers import gpu_setup, make_plan, rel_l2, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prob_small():
+    return small_problem(nt=300)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_snapshots_match_oracle(prob_small, precision, tol):
+    prob, disc, coef, c = prob_small
+    print(f"[snapshots] {precision}", flush=True)
+    box = O.roi_box(disc.mesh, 60.0, 430.0, 30.0, 360.0)
+    seis_o, snaps_o = O.forward_snapshots(disc, c, 37, box, record="pre")
+    plan = make_plan(prob, precision, record="pre")
+    rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
+    seis, snaps = plan.forward_snapshots(c2, w, [src], 37, box, [rs])
+    torch.cuda.synchronize()
+    assert snaps.shape[1:] == snaps_o.shape
+    assert rel_l2(seis[0].cpu().numpy(), seis_o) < tol
+    assert rel_l2(snaps[0].cpu().numpy(), snaps_o) < tol
+
+
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_misfit_window_matches_oracle(prob_small, mode, precision):
+    prob, disc, coef, c = prob_small
+    print(f"[window] {mode} {precision}", flush=True)
+    rng = np.random.default_rng(11)
+    cobs, _, _ = O.tensor_bspline_velocity(disc.mesh, coef + 40.0 * rng.standard_normal(coef.shape))
+    yobs, _ = O.forward(disc, cobs, history=None)
+    win = (70, 251, 3)
+    ref = O.forward_adjoint(disc, c, yobs, 1e-9, mode=mode, window=win)
+    from mun_amd.solver import Plan
+    plan = Plan(prob.nelem_x, prob.nelem_z, prob.xmin, prob.xmax, prob.zmin, prob.zmax, prob.dt, prob.nt, 4,
+                prob.sponge_width, prob.sponge_cmax, prob.sponge_R, prob.sponge_sides, precision=precision,
+                adjoint_mode=mode, checkpoint_every=64, misfit_window=win)
+    rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
+    obs = torch.as_tensor(yobs, dtype=plan.dtype, device=plan.device)[None]
+    out = plan.forward_adjoint(c2, w, [src], obs, 1e-9, [rs])
+    torch.cuda.synchronize()
+    e_ll = abs(float(out["loglik"][0]) - ref["loglik"]) / abs(ref["loglik"])
+    e_g = rel_l2(out["grad_c"][0].cpu().numpy(), ref["dloglik_dc"])
+    print(f"  loglik {e_ll:.2e} grad {e_g:.2e}")
+    tol = 1e-9 if precision == "fp64" else 1e-4
+    assert e_ll < tol and e_g < tol
+
+
+def test_grid_to_nodes_matches_scipy(prob_small):
+    prob, disc, coef, c = prob_small
+    print("[grid]", flush=True)
+    plan = make_plan(prob, "fp64")
+    rng = np.random.default_rng(3)
+    xg = np.sort(rng.uniform(40.0, 460.0, 23))   # narrower than the mesh: clipping is exercised
+    zg = np.sort(rng.uniform(-30.0, 400.0, 17))
+    vp = rng.uniform(1500.0, 3500.0, (zg.size, xg.size))
+    for method in ("linear", "nearest"):
+        got = plan.grid_to_nodes(xg, zg, vp, method).cpu().numpy()
+        ref = O.grid_background(disc.mesh.coords, xg, zg, vp, method)
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-10)
+
+
+def _periodic_ctrl(K, R=150.0, cx=240.0, cz=210.0, seed=5):
+    rng = np.random.default_rng(seed)
+    th = 2 * np.pi * np.arange(K) / K
+    return np.stack([cx + R * np.cos(th), cz + R * np.sin(th)], axis=1) + rng.normal(0.0, 9.0, (K, 2)) \
+        + np.array([0.37, -0.21])
+
+
+@pytest.mark.parametrize("K,blend", [(6, "smooth"), (11, "smooth"), (8, "replace")])
+def test_periodic_curve_velocity_and_projection(prob_small, K, blend):
+    prob, disc, coef, c = prob_small
+    print(f"[periodic curve] K={K} {blend}", flush=True)
+    plan = make_plan(prob, "fp64")
+    X = disc.mesh.coords
+    bg = 2300.0 + 1.5 * X[:, 1]
+    ctrl = _periodic_ctrl(K)
+    v, sd, us, no = O.periodic_curve_velocity(X, ctrl, 1700.0, bg, 12.0, 900, 7, blend)
+    cg, c2g, aux = plan.curve_velocity(ctrl[None], "periodic", 1700.0, 3000.0, 12.0, 900, 7,
+                                       torch.as_tensor(bg), blend)
+    A = aux[0].cpu().numpy()
+    np.testing.assert_allclose(cg[0].cpu().numpy(), v, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(A[0], sd, atol=1e-8)
+    np.testing.assert_allclose(A[1], us, atol=1e-10)
+    np.testing.assert_allclose(A[2:].T, no, atol=1e-9)
+    rng = np.random.default_rng(K)
+    dL = rng.standard_normal(X.shape[0])
+    ref = O.periodic_curve_projection(dL, sd, us, no, 1700.0, bg, 12.0, K)
+    got = plan.curve_project(torch.as_tensor(dL)[None], aux, K, "periodic", 1700.0, 3000.0, 12.0,
+                             torch.as_tensor(bg), blend)[0].cpu().numpy()
+    assert rel_l2(got, ref) < 1e-10
+
+
+def test_clamped_curve_over_background_replace_blend(prob_small):
+    prob, disc, coef, c = prob_small
+    print("[clamped curve + background]", flush=True)
+    plan = make_plan(prob, "fp64")
+    X = disc.mesh.coords
+    ctrl = np.array([[150.0, 120.0], [300.0, 110.0], [360.0, 180.0], [330.0, 300.0], [200.0, 320.0],
+                     [120.0, 230.0]]) + np.array([0.31, -0.17])
+    bg = 2500.0 + 0.8 * X[:, 0]
+    v_c, sd, us, no = O.closed_curve_velocity(X, ctrl, 1800.0, 3000.0, 10.0, 800, 7)
+    ref = np.where(sd < 0.0, 1800.0, bg)
+    cg, _, aux = plan.curve_velocity(ctrl[None], "clamped6", 1800.0, 3000.0, 10.0, 800, 7, torch.as_tensor(bg),
+                                     "replace")
+    np.testing.assert_array_equal(cg[0].cpu().numpy(), ref)
+
+
+def co2_config(tmp_path, precision="fp64", mode="reference", window=True, K=8):
+    """A configuration in the CO2 variant's schema
+    (CO2_plume/one_stage/single_cubic/sem_waveform/core.py:240-447)."""
+    xg = np.linspace(-700.0, 700.0, 57)
+    zg = np.linspace(700.0, -700.0, 43)  # decreasing: the loader flips it
+    Zg, Xg = np.meshgrid(zg, xg, indexing="ij")
+    vp = 2400.0 + 0.4 * (Zg + 700.0) + 60.0 * np.sin(Xg / 150.0)
+    vp_path = os.path.join(str(tmp_path), "vp_background_pml.npy")
+    np.save(vp_path, vp)
+    ctrl = _periodic_ctrl(K, R=180.0, cx=20.0, cz=-30.0, seed=9)
+    mw = {"enabled": True, "tmin": 0.03, "tmax": 0.17, "decim": 2} if window else None
+    return {
+        "domain": {"xmin": -600, "xmax": 600, "zmin": -600, "zmax": 600, "nelem_x": 20, "nelem_z": 20},
+        "time": {"total_time": 0.2, "dt": 2.5e-4},
+        "source": {"position": (0.0, -400.0), "frequency": 12.0, "amplitude": 1.0},
+        "receivers": {"positions": np.stack([np.linspace(-450, 450, 19), np.full(19, 420.0)], axis=1)},
+        "method": {"polynomial_order": 4, "pml_thickness": 150.0, "precision": precision, "adjoint_mode": mode,
+                   "checkpoint_every": 40, "cfl_adjust": False, "misfit_window": mw},
+        "velocity": {"vmin": 1800.0, "vmax": 3000.0, "tau": 12.0, "spline_samples": 800,
+                     "background_2d": {"vp_path": vp_path, "x": xg, "z": zg, "interp": "linear"},
+                     "anomaly": {"control_points": ctrl, "v_inside": 1900.0, "blend": "smooth"}},
+        "output": {"save_wavefield": True, "snapshot_interval": 25, "output_dir": str(tmp_path),
+                   "roi_xmin": -300.0, "roi_xmax": 250.0, "roi_zmin": -200.0, "roi_zmax": 350.0},
+    }, ctrl, (xg, zg, vp)
+
+
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_co2_config_forward_adjoint_matches_oracle(tmp_path, mode):
+    from mun_amd.simulation import SEMSimulation
+    print(f"[co2 facade] {mode}", flush=True)
+    cfg, ctrl, (xg, zg, vp) = co2_config(tmp_path, mode=mode)
+    sim = SEMSimulation(cfg)
+    d = cfg["domain"]
+    rec = np.asarray(cfg["receivers"]["positions"])
+    prob = O.Problem(d["xmin"], d["xmax"], d["zmin"], d["zmax"], 20, 20, 4, sim.dt, sim.nt, (0.0, -400.0), 12.0,
+                     rec, sponge_width=150.0, sponge_cmax=sim.c_max)
+    disc = O.discretize(prob)
+    x2, z2, v2 = O.increasing_grid(xg, zg, vp)
+    bg = O.grid_background(disc.mesh.coords, x2, z2, v2, "linear")
+    assert abs(sim.c_max - max(3000.0, vp.max())) < 1e-9
+    np.testing.assert_allclose(sim.v_background, bg[O.reference_node_order(20, 20, 4)], rtol=1e-13)
+    # observations on the window's decimated grid, from a perturbed curve
+    ctrl_true = ctrl + np.array([6.0, -4.0])
+    v_true, _, _, _ = O.periodic_curve_velocity(disc.mesh.coords, ctrl_true, 1900.0, bg, 12.0, 800, 7)
+    y_full, _ = O.forward(disc, v_true, history=None)
+    win = O.window_from_times(sim.dt, sim.nt, 0.03, 0.17, decim=2)
+    y_dec = y_full[win[0]:win[1]:win[2]]
+    out = sim.run_forward_and_adjoint({"bspline_ctrl": ctrl.reshape(-1), "y_obs": y_dec, "obs_dt": sim.dt * 2,
+                                       "obs_nt": y_dec.shape[0], "noise_std": 1e-9})
+    v, sd, us, no = O.periodic_curve_velocity(disc.mesh.coords, ctrl, 1900.0, bg, 12.0, 800, 7)
+    y_on_rows = np.zeros_like(y_full)
+    y_on_rows[win[0]:win[1]:win[2]] = y_dec
+    ref = O.forward_adjoint(disc, v, y_on_rows, 1e-9, mode=mode, window=win)
+    g_ref = O.periodic_curve_projection(-ref["dloglik_dc"], sd, us, no, 1900.0, bg, 12.0, ctrl.shape[0])
+    assert abs(out["loglik"] - ref["loglik"]) < 1e-9 * abs(ref["loglik"])
+    assert out["grad_wrt_ctrl"].shape == (2 * ctrl.shape[0],)
+    assert rel_l2(out["grad_wrt_ctrl"], g_ref) < 1e-8
+    assert out["clean_data"].shape == (win[1] - win[0], rec.shape[0])
+    assert rel_l2(out["clean_data"], ref["seis"][win[0]:win[1]]) < 1e-9
+
+
+def test_co2_config_run_writes_snapshots(tmp_path):
+    from mun_amd.simulation import SEMSimulation
+    from mun_amd import io as sem_io
+    print("[co2 run snapshots]", flush=True)
+    cfg, ctrl, (xg, zg, vp) = co2_config(tmp_path, window=False)
+    sim = SEMSimulation(cfg)
+    res = sim.run(np.zeros(2 * ctrl.shape[0]))
+    d = cfg["domain"]
+    rec = np.asarray(cfg["receivers"]["positions"])
+    prob = O.Problem(d["xmin"], d["xmax"], d["zmin"], d["zmax"], 20, 20, 4, sim.dt, sim.nt, (0.0, -400.0), 12.0,
+                     rec, sponge_width=150.0, sponge_cmax=sim.c_max)
+    disc = O.discretize(prob)
+    x2, z2, v2 = O.increasing_grid(xg, zg, vp)
+    bg = O.grid_background(disc.mesh.coords, x2, z2, v2, "linear")
+    v, _, _, _ = O.periodic_curve_velocity(disc.mesh.coords, ctrl, 1900.0, bg, 12.0, 800, 7)
+    box = O.roi_box(disc.mesh, -300.0, 250.0, -200.0, 350.0)
+    seis_o, snaps_o = O.forward_snapshots(disc, v, 25, box, record="pre")
+    assert rel_l2(res["receiver_data"], seis_o) < 1e-9
+    # reference order: the ROI mask over the element-blocked numbering
+    perm = O.reference_node_order(20, 20, 4)
+    coords_ref = disc.mesh.coords[perm]
+    mask = ((coords_ref[:, 0] >= -300.0) & (coords_ref[:, 0] <= 250.0) & (coords_ref[:, 1] >= -200.0)
+            & (coords_ref[:, 1] <= 350.0))
+    full = []
+    O.forward(disc, v, record="pre", history=None,
+              after_step=lambda it, u: full.append(u[perm][mask].astype(np.float32)) if it % 25 == 0 else None)
+    snaps = res["wavefield_snapshots"]
+    assert len(snaps) == len(full) == snaps_o.shape[0]
+    for a, b in zip(snaps, full):
+        assert a.dtype == np.float32 and a.shape == b.shape
+        assert rel_l2(a, b) < 1e-6
+    f = np.load(os.path.join(str(tmp_path), "wavefield_snapshots.npz"), allow_pickle=False)
+    assert f["snapshots"].shape == (len(full), int(mask.sum())) and float(f["dt"]) == sim.dt
+    path = sem_io.save_forward_results(str(tmp_path), res)
+    back = sem_io.load_forward_results(path)
+    np.testing.assert_array_equal(back["receiver_data"], res["receiver_data"])

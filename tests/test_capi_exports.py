@@ -55,3 +55,44 @@ def test_null_arguments_fail_cleanly():
     assert lib.sem_plan_dims(None, None, None, None) == -1
     assert lib.sem_forward(None, None, None, 0, None) == -1
     assert b"null" in lib.sem_last_error()
+
+
+@pytest.mark.parametrize("kw", [dict(win_begin=5, win_end=3), dict(win_end=11), dict(win_begin=-1),
+                                dict(win_begin=10)])
+def test_plan_create_rejects_bad_windows(kw):
+    lib = L.lib()
+    h = C.c_void_p()
+    assert lib.sem_plan_create(C.byref(_desc(**kw)), C.byref(h)) == -1
+    assert "window" in lib.sem_last_error().decode()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof / offsetof of every ABI struct as a C compiler lays them out
+    equals the ctypes mirror."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"sem_desc": L.SemDesc, "sem_batch": L.SemBatch, "sem_slot": L.SemSlot,
+               "sem_curve_desc": L.SemCurveDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{name} {fname} %zu\\n", offsetof({name}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln.strip():
+            parts = ln.split()
+            got[tuple(parts[:-1])] = int(parts[-1])
+    for name, cls in structs.items():
+        assert got[(name, "size")] == C.sizeof(cls), name
+        for fname, _ in cls._fields_:
+            assert got[(name, fname)] == getattr(cls, fname).offset, (name, fname)

@@ -1,6 +1,7 @@
 """Profiling driver: runs the solver's kernels at a fixed shape under a profiler.
 
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
+    python -m mun_amd.profile_kernels --stream-ceiling
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -27,7 +28,16 @@ def main(argv=None):
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--tile-rows", type=int, default=15, help="step-kernel tile height (15, 7 or 3)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="run the HBM streaming-ceiling probe (mun_amd/stream_ceiling) instead")
     a = ap.parse_args(argv)
+    if a.stream_ceiling:
+        import os
+        import subprocess
+        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stream_ceiling")
+        if not os.path.exists(exe):
+            raise FileNotFoundError(f"{exe} is missing: build it with `python -m mun_amd.build`")
+        return subprocess.run([exe], check=True).returncode
     dev = torch.device("cuda:0")
     n = a.copy_mb * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device=dev)

@@ -220,7 +220,7 @@ def main():
                       "batch": args.batch, "nt": m.nt, "nodes": m.N, "parallelism": "none (1 GPU)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                        "kernel": "step_kernel<float,4,MODE_FWD,15>",
+                        "kernel": f"step_kernel<float,4,MODE_FWD,{plan.tile_rows}>",
                         "launch_us": round(t_launch * 1e6, 1),
                         "algorithmic_bytes_per_launch": bytes_launch, "bytes_per_node": FWD_BYTES_PER_NODE,
                         "achieved_16B_basis": round(STD_BYTES_PER_NODE * m.N * args.batch / t_launch / 1e9, 1)},

@@ -451,6 +451,83 @@ class SEMSimulation:
         self.last_grad_c = g_c[0].cpu().numpy()[self._perm]
         return res
 
+    def run_batch(self, params) -> list:
+        """Forward runs of M parameter sets in batched launches, sampled like
+        run(): closed-curve models take control-point offsets [M, 2K] (added
+        to the configured points), B-spline grids node values [M, nbz, nbx].
+        Models whose CFL-adjusted (dt, nt) coincide share one batch (the
+        reference adjusts dt per call, Q1).  Returns a list of M dicts with
+        'receiver_data' (nt, nrec), 'dt', 'nt'.  Replaces the per-chain calls
+        of SEMSimulation(config).run() in the MCMC likelihood
+        (Bspline_Coupling/MCMC/MCMC.py:192-213, 426-433)."""
+        M, c2, groups = self._batch_models(params)
+        out = [None] * M
+        for (dt, nt), idx, plan, seis in self._forward_groups(groups, c2):
+            data = seis.double().cpu().numpy()[:, :, :self.num_receivers]
+            for k, m in enumerate(idx):
+                out[m] = {"receiver_data": data[k], "dt": float(dt), "nt": int(nt)}
+        return out
+
+    def _forward_groups(self, groups, c2):
+        """Batched pre-update forward runs per (dt, nt) group; yields
+        ((dt, nt), model indices, plan, seismograms on the device)."""
+        dt0, nt0 = self.dt, self.nt
+        try:
+            for (dt, nt), idx in groups.items():
+                self.dt, self.nt = dt, nt
+                plan = self._plan("pre")
+                src = plan.source_node(self.src_x, self.src_z)
+                seis = plan.forward(c2[idx].to(plan.dtype), self._wavelet(plan), [src] * len(idx), [0] * len(idx))
+                yield (dt, nt), idx, plan, seis
+        finally:
+            self.dt, self.nt = dt0, nt0
+
+    def _batch_models(self, params):
+        params = np.asarray(_np(params), dtype=np.float64)
+        M = params.shape[0]
+        plan0 = self._plan("post")
+        if self.grid is None:
+            if self.ctrl_pts_original is None or not self.anomaly_enabled:
+                raise ValueError("batched runs need a closed-curve anomaly or a B-spline grid")
+            ctrl = self.ctrl_pts_original[None] + params.reshape(M, -1, 2)
+            a = self._curve_args()
+            c, c2, _ = plan0.curve_velocity(ctrl, a["kind"], a["v_in"], a["v_out"], a["tau"], self.spline_samples,
+                                            7, a["background"], a["blend"])
+        else:
+            base = None if self.grid_base is None else torch.as_tensor(self.grid_base, device=self.device)
+            c, c2 = plan0.bspline_map(torch.as_tensor(params.reshape(M, *self.grid_coef.shape)), base)
+        if not bool(torch.isfinite(c).all()):
+            raise ValueError("[CFL] velocity_model contains non-finite values (NaN/Inf).")
+        groups = {}
+        cmax = c.max(dim=1).values.cpu().numpy()
+        for m in range(M):
+            dt, nt = self.dt, self.nt
+            if self.cfl_adjust:
+                dtm = cfl_dt(float(cmax[m]), self.npol, self.dx_elem, self.dz_elem)
+                if dt > dtm:
+                    dt, nt = float(dtm), int(np.ceil(self.total_time / dtm))
+            groups.setdefault((dt, nt), []).append(m)
+        return M, c2, groups
+
+    def log_likelihood_batch(self, params, y_obs, sigma: float) -> np.ndarray:
+        """sum Normal(simulated, sigma).log_prob(y_obs) for M parameter sets
+        (the likelihood of Bspline_Coupling/MCMC/MCMC.py:426-433) from one
+        batched forward; y_obs (nt, nrec) on the simulation's time axis.  The
+        squared residual is reduced on the GPU in double."""
+        M, c2, groups = self._batch_models(params)
+        y = np.asarray(_np(y_obs), dtype=np.float64)
+        out = np.empty(M)
+        const = math.log(float(sigma)) + 0.5 * math.log(2 * math.pi)
+        for (dt, nt), idx, plan, seis in self._forward_groups(groups, c2):
+            if y.shape != (nt, self.num_receivers):
+                raise ValueError(f"y_obs shape {y.shape} != simulated {(nt, self.num_receivers)}")
+            obs = torch.zeros((1,) + tuple(seis.shape[1:]), dtype=plan.dtype, device=self.device)
+            obs[0, :, :self.num_receivers] = torch.as_tensor(y, dtype=plan.dtype)
+            mis = plan.misfit(seis, obs, self.num_receivers).cpu().numpy()
+            for k, m in enumerate(idx):
+                out[m] = -mis[k] / float(sigma) ** 2 - nt * self.num_receivers * const
+        return out
+
     def run_forward_only_for_loglik(self, ctrl12, sigma2, obs_dt, obs_nt, y_obs) -> float:
         """log-likelihood of a forward run with pre-update sampling, as the
         reference's finite-difference check computes it (core.py:1440-1492)."""

@@ -296,6 +296,19 @@ class Plan:
                                                _ptr(ws), ws.numel(), _stream_ptr(self.device)))
         return seis, snaps
 
+    def misfit(self, seis: torch.Tensor, obs: torch.Tensor, nrec: int) -> torch.Tensor:
+        """0.5 * ||seis - obs||^2 per solve over the first nrec columns, in
+        double (seis, obs: [B, nt, pitch] in the plan's precision)."""
+        seis = seis.to(device=self.device, dtype=self.dtype).contiguous()
+        obs = obs.to(device=self.device, dtype=self.dtype).expand_as(seis).contiguous()
+        B, nt, pitch = seis.shape
+        res = torch.empty_like(seis[0])
+        out = torch.empty(B, dtype=torch.float64, device=self.device)
+        for b in range(B):
+            L.check(self.lib.sem_residual(self.h, _ptr(seis[b]), _ptr(obs[b]), _ptr(res), _ptr(out[b:b + 1]),
+                                          int(nrec), nt, pitch, _stream_ptr(self.device)))
+        return out
+
     def forward_adjoint(self, c2, wavelet, src_nodes, obs, noise_std, rec_sets=None, model_of=None,
                         want_grad_c: bool = True, want_grad_c2: bool = False):
         """Forward + misfit + adjoint gradient for a batch.

@@ -216,3 +216,24 @@ def test_co2_config_run_writes_snapshots(tmp_path):
     path = sem_io.save_forward_results(str(tmp_path), res)
     back = sem_io.load_forward_results(path)
     np.testing.assert_array_equal(back["receiver_data"], res["receiver_data"])
+
+
+def test_mcmc_batched_likelihood_matches_sequential_runs():
+    """run_batch / log_likelihood_batch (the MCMC likelihood of
+    Bspline_Coupling/MCMC/MCMC.py:426-433 for several chains at once) give the
+    bits of one run() per parameter set and torch's Normal log-density."""
+    from mun_amd.simulation import SEMSimulation
+    from tests.test_gpu_dropin import reference_config
+    print("[mcmc batch]", flush=True)
+    sim = SEMSimulation(reference_config("fp64", nelem=16))
+    rng = np.random.default_rng(8)
+    offs = rng.normal(0.0, 15.0, (3, 12))
+    runs = sim.run_batch(offs)
+    y = sim.run(np.zeros(12))["receiver_data"] + rng.normal(0.0, 1e-9, runs[0]["receiver_data"].shape)
+    ll = sim.log_likelihood_batch(offs, y, 2e-9)
+    for m in range(3):
+        seq = sim.run(offs[m])
+        np.testing.assert_array_equal(runs[m]["receiver_data"], seq["receiver_data"])
+        ref = torch.distributions.Normal(torch.as_tensor(seq["receiver_data"]), 2e-9).log_prob(
+            torch.as_tensor(y)).sum().item()
+        assert abs(ll[m] - ref) <= 1e-10 * abs(ref)

@@ -860,6 +860,14 @@ __global__ void grid_to_nodes_kernel(const double* xg, int nxg, const double* zg
     }
 }
 
+// flag[0] = 1 if any of the n values is NaN or infinite (plain stores of the
+// same value: no atomics needed).
+template <typename T>
+__global__ void finite_check_kernel(const T* p, long n, int* flag) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        if (!isfinite(p[k])) flag[0] = 1;
+}
+
 template <typename T>
 __global__ void fill_kernel(T* p, T v, long n) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) p[k] = v;

@@ -317,7 +317,7 @@ struct Layout {
     bool full_history = false;
     int S = 0, nseg = 0;
     int C = 0;  // adjoint steps per gradient-correlation chunk
-    size_t C2P, U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, NREC, RSET, total;
+    size_t C2P, U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, NREC, RSET, FLAG, total;
 };
 
 // Correlation chunk: at most 32 adjoint steps, dividing a segment of S steps
@@ -350,6 +350,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
     L.W = take(state);
     L.NREC = take((size_t)B * 4);
     L.RSET = take((size_t)B * 4);
+    L.FLAG = take(4);
     if (adj) {
         if (!L.full_history) {
             L.CKU = take(state * L.nseg);
@@ -455,8 +456,9 @@ int launch_corr(const sem_plan* pl, const CorrArgs<T, 4>& c, hipStream_t st) {
 int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout& L) {
     if (!bt || bt->batch <= 0) return fail(SEM_E_INVALID, "batch must be >= 1");
     if ((long)pl->ntiles * bt->batch >= (1L << 31) - 64) return fail(SEM_E_INVALID, "batch too large for this mesh");
-    if (!bt->c2 || !bt->wavelet || !bt->src_node || !bt->rec_set || !bt->seis)
+    if (!bt->c2 || !bt->wavelet || !bt->src_node || !bt->rec_set || (!bt->seis && bt->rec_pitch > 0))
         return fail(SEM_E_INVALID, "sem_batch: null pointer in c2/wavelet/src_node/rec_set/seis");
+    if (bt->rec_pitch < 0) return fail(SEM_E_INVALID, "rec_pitch must be >= 0");
     if (bt->nmodel < 1 || bt->nmodel > bt->batch) return fail(SEM_E_INVALID, "nmodel must be in [1, batch]");
     if (!bt->model_of && bt->nmodel != bt->batch) return fail(SEM_E_INVALID, "model_of is NULL but nmodel != batch");
     if (ws_bytes < L.total)
@@ -484,8 +486,20 @@ int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hi
     dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), bt->nmodel);
     pad_rows_kernel<T><<<grid, 256, 0, st>>>((const T*)bt->c2, (T*)(ws + L.C2P), pl->NX, pl->NZ, pl->pitch, pl->sb);
     LAUNCH_CHECK();
-    // the pageable source buffers above must stay valid until the copies land
+    // a non-finite model is rejected before any time step (the reference's CFL
+    // check raises on it, utils.py:64-93)
+    int* flag = (int*)(ws + L.FLAG);
+    fill_kernel<int><<<1, 64, 0, st>>>(flag, 0, 1);
+    LAUNCH_CHECK();
+    const long nc2 = (long)bt->nmodel * pl->N;
+    finite_check_kernel<T><<<(unsigned)std::min<long>((nc2 + 255) / 256, 4096), 256, 0, st>>>((const T*)bt->c2, nc2,
+                                                                                            flag);
+    LAUNCH_CHECK();
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    // the pageable buffers above must stay valid until the copies land
     HIP_TRY(hipStreamSynchronize(st));
+    if (bad) return fail(SEM_E_NONFINITE, "squared velocity contains NaN or Inf");
     return SEM_OK;
 }
 
@@ -575,7 +589,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, true);
     int rc;
     if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
-    if (!obs || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
+    if ((!obs && bt->rec_pitch > 0) || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
     char* ws = (char*)wsv;
     const int B = bt->batch, nt = pl->d.nt;
     const long state = (long)B * pl->sb;

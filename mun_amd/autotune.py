@@ -87,7 +87,7 @@ def best_tile_rows(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp3
             hit = _MEM[key]
         else:
             hit = _load().get(key)
-        if hit:
+        if hit and int(hit.get("tile_rows", -1)) in TILE_ROWS:  # entries of other kernel builds are re-timed
             _MEM[key] = hit
             if report is not None:
                 report.update(hit)

@@ -216,26 +216,6 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
 typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef double vd2 __attribute__((ext_vector_type(2)));
 
-// five consecutive values starting at a 16-byte aligned address
-template <typename T>
-__device__ __forceinline__ void load5(const GMEM T* __restrict__ p, T (&v)[5]) {
-    if constexpr (std::is_same_v<T, float>) {
-        const vf4 a = *reinterpret_cast<const GMEM vf4*>(p);
-        v[0] = a.x;
-        v[1] = a.y;
-        v[2] = a.z;
-        v[3] = a.w;
-    } else {
-        const vd2 a = *reinterpret_cast<const GMEM vd2*>(p);
-        const vd2 c = *reinterpret_cast<const GMEM vd2*>(p + 2);
-        v[0] = a.x;
-        v[1] = a.y;
-        v[2] = c.x;
-        v[3] = c.y;
-    }
-    v[4] = p[4];
-}
-
 template <typename T>
 __device__ __forceinline__ void load4(const GMEM T* __restrict__ p, T (&v)[4]) {
     if constexpr (std::is_same_v<T, float>) {
@@ -251,6 +231,39 @@ __device__ __forceinline__ void load4(const GMEM T* __restrict__ p, T (&v)[4]) {
         v[1] = a.y;
         v[2] = c.x;
         v[3] = c.y;
+    }
+}
+
+// An element's 5 x 5 block in two halves.  fetch_block issues one 16-byte load
+// per row for the first four nodes and, only in lanes whose right neighbour
+// element is not held by lane + 1, a load of the fifth; finish_block takes the
+// fifth node of the other lanes from lane + 1, whose first node it is (a
+// cross-lane read instead of a memory instruction).  finish_block must run in
+// converged control flow.
+template <typename T>
+__device__ __forceinline__ void fetch_block(const GMEM T* __restrict__ p, int pitch, bool own_fifth,
+                                            T (&v)[5][5]) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        T q[4];
+        load4(p + j * pitch, q);
+        v[j][0] = q[0];
+        v[j][1] = q[1];
+        v[j][2] = q[2];
+        v[j][3] = q[3];
+    }
+    if (own_fifth) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) v[j][4] = p[j * pitch + 4];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void finish_block(bool from_right, T (&v)[5][5]) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const T r = __shfl_down(v[j][0], 1, 64);
+        if (from_right) v[j][4] = r;
     }
 }
 
@@ -413,18 +426,43 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
     } else {
         T u[NP][NP], r[NP][NP];
         T g[CORR ? NP : 1][CORR ? NP : 1];
-        // update-phase operands of the owned 4 x 4 block, fetched before the
-        // element math so their latency overlaps it
+        // update-phase operands of the owned 4 x 4 block, issued right after
+        // the element math (earlier they would hold registers through it) so
+        // their latency overlaps the exchange
         T wv[P][P];
         T dxv[P], dzv[P];
-        if (valid) {
-            const GMEM T* __restrict__ uin = V.u_in + base;
-            const GMEM T* __restrict__ c2p = V.c2 + base;
+        {
+            // every lane loads a block (elements outside the domain read a
+            // clamped in-range one) so that the cross-lane fifth-node reads
+            // below see their neighbours' data
+            const int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
+            const int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+            const int base_c = (P * ezc) * pitch + P * exc;
+            // lane t + 1 holds the right neighbour (same tile row and wave, inside the domain)
+            const bool from_right = exl < EX - 1 && (t & 63) != 63 && ex + 1 < a.nex;
             T c2[NP][NP];
+            fetch_block(V.u_in + base_c, pitch, valid && !from_right, u);
+            fetch_block(V.c2 + base_c, pitch, valid && !from_right, c2);
+            T v[CORR ? NP : 1][CORR ? NP : 1];
+            if constexpr (CORR) fetch_block(V.uf + base_c, pitch, valid && !from_right, v);
+            finish_block(from_right, u);
+            finish_block(from_right, c2);
+            if constexpr (CORR) finish_block(from_right, v);
+            // the element math runs in every lane (branch-free: fewer live
+            // registers); lanes outside the domain then contribute zero
+            if constexpr (CORR)
+                element_action<T, P, true>(a.ec, u, c2, v, r, g);
+            else
+                element_action<T, P, false>(a.ec, u, c2, u, r, r);
+            if (!valid) {
 #pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                load5(uin + j * pitch, u[j]);
-                load5(c2p + j * pitch, c2[j]);
+                for (int j = 0; j < NP; ++j)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        r[j][i] = T(0);
+                        u[j][i] = T(0);
+                        if constexpr (CORR) g[j][i] = T(0);
+                    }
             }
             if (owned) {
                 const GMEM T* __restrict__ wq = V.w + base;
@@ -435,24 +473,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     load4(gm(a.Dz) + P * ez, dzv);
                 }
             }
-            T v[CORR ? NP : 1][CORR ? NP : 1];
-            if constexpr (CORR) {
-                const GMEM T* __restrict__ ufp = V.uf + base;
-#pragma unroll
-                for (int j = 0; j < NP; ++j) load5(ufp + j * pitch, v[j]);
-                element_action<T, P, true>(a.ec, u, c2, v, r, g);
-            } else {
-                element_action<T, P, false>(a.ec, u, c2, u, r, r);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int i = 0; i < NP; ++i) {
-                    r[j][i] = T(0);
-                    u[j][i] = T(0);
-                    if constexpr (CORR) g[j][i] = T(0);
-                }
         }
 
         if constexpr (MODE == MODE_FWD) {
@@ -479,7 +499,8 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
         }
         if constexpr (ADJ) {
             // adjoint source R^T res, injected into the receiver element's slots
-            if (valid) {
+            // (entries name in-domain elements only, so lanes outside never match)
+            {
                 const int set = V.set;
                 const int beg = a.rec_ptr[set * (a.ntiles + 1) + tile];
                 const int end = a.rec_ptr[set * (a.ntiles + 1) + tile + 1];
@@ -700,40 +721,45 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
     // need twice the registers and load per step)
     constexpr bool PIPE = std::is_same_v<T, float>;
     T lc[NP][NP], uc[NP][NP];
+    // every lane loads a block (clamped into the domain) so that the
+    // cross-lane fifth-node reads see their neighbours' data
+    const int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
+    const int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+    const int base_c = (P * ezc) * pitch + P * exc;
+    const bool from_right = exl < EX - 1 && (t & 63) != 63 && ex + 1 < a.nex;
+    const bool own5 = valid && !from_right;
     auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
         const int it = a.c0 + s;
-        const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base;
-        const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            load5(lp + j * pitch, l[j]);
-            load5(up + j * pitch, u[j]);
-        }
+        const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base_c;
+        const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base_c;
+        fetch_block(lp, pitch, own5, l);
+        fetch_block(up, pitch, own5, u);
     };
-    if (PIPE && valid) fetch(a.clen - 1, lc, uc);
+    if constexpr (PIPE) fetch(a.clen - 1, lc, uc);
     for (int s = a.clen - 1; s >= 0; --s) {
         T g[NP][NP];
-        if (valid) {
-            if constexpr (PIPE) {
-                T ln[NP][NP], un[NP][NP];
-                fetch(s > 0 ? s - 1 : 0, ln, un);  // the last step re-reads its own (cached) blocks
-                element_corr<T, P>(a.ec, lc, uc, g);
-#pragma unroll
-                for (int j = 0; j < NP; ++j)
-#pragma unroll
-                    for (int i = 0; i < NP; ++i) {
-                        lc[j][i] = ln[j][i];
-                        uc[j][i] = un[j][i];
-                    }
-            } else {
-                fetch(s, lc, uc);
-                element_corr<T, P>(a.ec, lc, uc, g);
-            }
-        } else {
+        T ln[PIPE ? NP : 1][PIPE ? NP : 1], un[PIPE ? NP : 1][PIPE ? NP : 1];
+        if constexpr (PIPE)
+            fetch(s > 0 ? s - 1 : 0, ln, un);  // the last step re-reads its own (cached) blocks
+        else
+            fetch(s, lc, uc);
+        finish_block(from_right, lc);
+        finish_block(from_right, uc);
+        element_corr<T, P>(a.ec, lc, uc, g);
+        if (!valid) {
 #pragma unroll
             for (int j = 0; j < NP; ++j)
 #pragma unroll
                 for (int i = 0; i < NP; ++i) g[j][i] = T(0);
+        }
+        if constexpr (PIPE) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    lc[j][i] = ln[j][i];
+                    uc[j][i] = un[j][i];
+                }
         }
         if (t < EX * EZ) {
 #pragma unroll

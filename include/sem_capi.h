@@ -244,7 +244,7 @@ int sem_curve_project_ex(sem_plan* plan, const sem_curve_desc* cd, int nmodel, c
 #define SEM_SLOT_IDLE 0
 #define SEM_SLOT_FORWARD 1    /* u0/u1 ping-pong (by step parity), w increment       */
 #define SEM_SLOT_RECOMPUTE 2  /* u0 = checkpoint u of the segment, w = its w, seg    */
-#define SEM_SLOT_ADJOINT 3    /* u0/u1 = lambda ping-pong by step parity, w = omega  */
+#define SEM_SLOT_ADJOINT 3    /* lambda in the ring, w = omega                        */
 
 typedef struct {
     int phase;             /* SEM_SLOT_*                                              */
@@ -264,6 +264,9 @@ typedef struct {
     long seg_stride;       /* elements between segment slots                          */
     const void* resid;     /* adjoint source [nt][rec_pitch]                          */
     double* G;             /* gradient accumulator (padded layout, double)            */
+    void* ring;            /* adjoint: lambda_t in slot (t+1) % ring_slots, stride seg_stride */
+    int ring_slots;        /* >= chunk_len + 1 of sem_slots_correlate                 */
+    int reserved;
 } sem_slot;
 
 /* Padded nodal layout: row pitch and per-array element count. */
@@ -274,6 +277,13 @@ int sem_plan_layout(const sem_plan* plan, int* pitch, long* array_elems);
  * current segments; slots with m >= len skip.  rec_pitch: seismogram row length. */
 int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
                    int rec_pitch, void* stream);
+/* Gradient correlation of chunk `chunk` (segment steps chunk*chunk_len ..
+ * min((chunk+1)*chunk_len, len) - 1) of every ADJOINT slot, added to its G in
+ * FP64; enqueue after those adjoint steps and before the ring slots are reused
+ * (ring_slots >= chunk_len + 1).  Replaces the per-step correlation of
+ * accumulate_sensitivity_dJdc_torch (core.py:49-85, 1265-1270). */
+int sem_slots_correlate(sem_plan* plan, const sem_slot* slots_device, int nslots, int chunk, int chunk_len,
+                        void* stream);
 /* Receiver samples of u (final post-update sample) for slots in phase_mask:
  * forward slots sample u[(n0+len)&1] into seis row n0+len-1. */
 int sem_slots_record_final(sem_plan* plan, const sem_slot* slots_device, int nslots, unsigned phase_mask,

@@ -25,6 +25,7 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_curve_project", "sem_plan_layout", "sem_slots_step", "sem_slots_record_final", "sem_pad_nodal",
            "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize",
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
+           "sem_slots_correlate",
            "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
@@ -63,7 +64,8 @@ class SemSlot(C.Structure):
                 ("rec_set", C.c_int), ("record", C.c_int), ("u0", C.c_void_p), ("u1", C.c_void_p),
                 ("w", C.c_void_p), ("c2", C.c_void_p), ("wavelet", C.c_void_p), ("seis", C.c_void_p),
                 ("ck_u", C.c_void_p), ("ck_w", C.c_void_p), ("seg", C.c_void_p), ("seg_stride", C.c_long),
-                ("resid", C.c_void_p), ("G", C.c_void_p)]
+                ("resid", C.c_void_p), ("G", C.c_void_p), ("ring", C.c_void_p), ("ring_slots", C.c_int),
+                ("reserved", C.c_int)]
 
 
 class SemError(RuntimeError):
@@ -103,6 +105,7 @@ def lib() -> C.CDLL:
     L.sem_plan_layout.argtypes = [vp, ip, C.POINTER(C.c_long)]
     L.sem_slots_step.argtypes = [vp, vp, i, i, C.c_uint, i, vp]
     L.sem_slots_record_final.argtypes = [vp, vp, i, C.c_uint, i, vp]
+    L.sem_slots_correlate.argtypes = [vp, vp, i, i, i, vp]
     L.sem_pad_nodal.argtypes = [vp, vp, vp, i, vp]
     L.sem_receivers_reserve.argtypes = [vp, C.c_long, i]
     L.sem_receivers_sync.argtypes = [vp]

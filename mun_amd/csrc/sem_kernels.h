@@ -48,10 +48,9 @@ constexpr int EZ_DEFAULT = 15;
 __host__ __device__ constexpr int tile_threads(int ez) { return (EX * ez + 63) / 64 * 64; }
 constexpr int NXCD = 8;           // XCDs of an MI355X
 
-// MODE_ADJ: adjoint step with the gradient correlation fused (slot tables);
-// MODE_ADJS: adjoint step alone, the correlation runs later per chunk of steps
-// in corr_kernel (uniform batches).
-enum StepMode { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2, MODE_ADJS = 3 };
+// MODE_ADJS: adjoint step; its gradient correlation runs later per chunk of
+// steps in corr_kernel.
+enum StepMode { MODE_FWD = 0, MODE_REC = 2, MODE_ADJS = 3 };
 
 // Global-memory qualified pointers: lets the compiler emit global_* instead of
 // flat_* instructions for pointers that come from the slot table.
@@ -111,10 +110,8 @@ struct StepArgs {
     T* seis;         long seis_bs; int rec_pitch; int rec_t;
     // checkpoint mirrors of (u^{n+1}, w^{n+1}), may be null
     T* ck_u;         T* ck_w;
-    // adjoint: residual source, forward wavefield, gradient accumulator
+    // adjoint: residual source
     const T* resid;  long resid_bs; int res_t; T invJ;
-    const T* uf;
-    double* G;
     // slot mode: per-solve pointers and steps come from a device slot table
     const sem_slot* slots;  // null: uniform batch mode
     int m;                  // step index within the slots' current segments
@@ -129,8 +126,6 @@ struct View {
     GMEM T* u_out;
     GMEM T* w;
     const GMEM T* c2;
-    const GMEM T* uf;
-    GMEM double* G;
     GMEM T* ck_u;
     GMEM T* ck_w;
     GMEM T* seis;
@@ -149,8 +144,6 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
         v.u_out = gm(a.u_out + soff);
         v.w = gm(a.w + soff);
         v.c2 = gm(a.c2 + mdl * a.sb);
-        v.uf = a.uf ? gm(a.uf + soff) : nullptr;
-        v.G = a.G ? gm(a.G + soff) : nullptr;
         v.ck_u = a.ck_u ? gm(a.ck_u + soff) : nullptr;
         v.ck_w = a.ck_w ? gm(a.ck_w + soff) : nullptr;
         v.seis = a.seis ? gm(a.seis + b * a.seis_bs) : nullptr;
@@ -166,11 +159,9 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     if (!((a.phase_mask >> d.phase) & 1u)) return false;
     v.c2 = gm((const T*)d.c2);
     v.set = d.rec_set;
-    v.G = gm(d.G);
     v.seis = gm((T*)d.seis);
     v.resid = gm((const T*)d.resid);
     v.ck_u = v.ck_w = nullptr;
-    v.uf = nullptr;
     v.wav = nullptr;
     v.src = -1;
     v.rec_t = v.res_t = -1;
@@ -203,14 +194,13 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
         }
         return MODE == MODE_FWD;
     }
-    // adjoint: lambda_it in u[it & 1], lambda_{it-1} into the other
+    // adjoint: lambda_t in ring slot (t + 1) % ring_slots
     const int it = d.n0 + d.len - 1 - a.m;
-    v.u_in = gm((const T*)((it & 1) ? d.u1 : d.u0));
-    v.u_out = gm((T*)((it & 1) ? d.u0 : d.u1));
+    v.u_in = gm((const T*)d.ring + (long)((it + 1) % d.ring_slots) * d.seg_stride);
+    v.u_out = gm((T*)d.ring + (long)(it % d.ring_slots) * d.seg_stride);
     v.w = gm((T*)d.w);
-    v.uf = gm((const T*)d.seg + (long)(it - d.n0 + 1) * d.seg_stride);
     v.res_t = it;
-    return MODE == MODE_ADJ;
+    return MODE == MODE_ADJS;
 }
 
 typedef float vf4 __attribute__((ext_vector_type(4)));
@@ -312,12 +302,10 @@ __device__ __forceinline__ void load4nt(const GMEM T* __restrict__ p, T (&v)[4])
     }
 }
 
-// Element contributions r = Ke u / J (x and z parts with folded constants);
-// in the adjoint mode also the gradient correlation g of (uf, u).
-template <typename T, int P, bool CORR>
+// Element contributions r = Ke u / J (x and z parts with folded constants).
+template <typename T, int P>
 __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const T (&u)[P + 1][P + 1],
-                                               const T (&c2)[P + 1][P + 1], const T (&v)[P + 1][P + 1],
-                                               T (&r)[P + 1][P + 1], T (&g)[P + 1][P + 1]) {
+                                               const T (&c2)[P + 1][P + 1], T (&r)[P + 1][P + 1]) {
     constexpr int NP = P + 1;
     // x direction, row by row
 #pragma unroll
@@ -329,12 +317,6 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
 #pragma unroll
             for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[j][k];
             f[q] = c2[j][q] * d;
-            if constexpr (CORR) {
-                T du = T(0);
-#pragma unroll
-                for (int k = 0; k < NP; ++k) du += ec.D[q][k] * v[j][k];
-                g[j][q] = ec.gx[j][q] * (du * d);
-            }
         }
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
@@ -354,12 +336,6 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
 #pragma unroll
             for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[k][i];
             f[q] = c2[q][i] * d;
-            if constexpr (CORR) {
-                T du = T(0);
-#pragma unroll
-                for (int k = 0; k < NP; ++k) du += ec.D[q][k] * v[k][i];
-                g[q][i] += ec.gz[q][i] * (du * d);
-            }
         }
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
@@ -376,12 +352,9 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
-    constexpr bool ADJ = (MODE == MODE_ADJ || MODE == MODE_ADJS);
-    constexpr bool CORR = (MODE == MODE_ADJ);
+    constexpr bool ADJ = (MODE == MODE_ADJS);
     __shared__ T xr_col[EZ][EX][NP];   // element's i = P column of contributions
     __shared__ T xr_row[EZ][EX][P];    // element's j = P row (i < P)
-    __shared__ T xg_col[CORR ? EZ : 1][EX][NP];
-    __shared__ T xg_row[CORR ? EZ : 1][EX][P];
 
     // XCD-contiguous numbering of (tile, solve) pairs
     const int wgid = blockIdx.x;
@@ -425,7 +398,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
         return;
     } else {
         T u[NP][NP], r[NP][NP];
-        T g[CORR ? NP : 1][CORR ? NP : 1];
         // update-phase operands of the owned 4 x 4 block, issued right after
         // the element math (earlier they would hold registers through it) so
         // their latency overlaps the exchange
@@ -443,17 +415,11 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
             T c2[NP][NP];
             fetch_block(V.u_in + base_c, pitch, valid && !from_right, u);
             fetch_block(V.c2 + base_c, pitch, valid && !from_right, c2);
-            T v[CORR ? NP : 1][CORR ? NP : 1];
-            if constexpr (CORR) fetch_block(V.uf + base_c, pitch, valid && !from_right, v);
             finish_block(from_right, u);
             finish_block(from_right, c2);
-            if constexpr (CORR) finish_block(from_right, v);
             // the element math runs in every lane (branch-free: fewer live
             // registers); lanes outside the domain then contribute zero
-            if constexpr (CORR)
-                element_action<T, P, true>(a.ec, u, c2, v, r, g);
-            else
-                element_action<T, P, false>(a.ec, u, c2, u, r, r);
+            element_action<T, P>(a.ec, u, c2, r);
             if (!valid) {
 #pragma unroll
                 for (int j = 0; j < NP; ++j)
@@ -461,7 +427,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     for (int i = 0; i < NP; ++i) {
                         r[j][i] = T(0);
                         u[j][i] = T(0);
-                        if constexpr (CORR) g[j][i] = T(0);
                     }
             }
             if (owned) {
@@ -522,12 +487,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
             for (int j = 0; j < NP; ++j) xr_col[ezl][exl][j] = r[j][P];
 #pragma unroll
             for (int i = 0; i < P; ++i) xr_row[ezl][exl][i] = r[P][i];
-            if constexpr (CORR) {
-#pragma unroll
-                for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
-#pragma unroll
-                for (int i = 0; i < P; ++i) xg_row[ezl][exl][i] = g[P][i];
-            }
         }
         __syncthreads();
         if (!owned) return;
@@ -540,11 +499,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                 if (i == 0) r[j][i] += xr_col[ezl][exl - 1][j];
                 if (j == 0) r[j][i] += (i < P) ? xr_row[ezl - 1][exl][i] : xr_col[ezl - 1][exl][P];
                 if (i == 0 && j == 0) r[j][i] += xr_col[ezl - 1][exl - 1][P];
-                if constexpr (CORR) {
-                    if (i == 0) g[j][i] += xg_col[ezl][exl - 1][j];
-                    if (j == 0) g[j][i] += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
-                    if (i == 0 && j == 0) g[j][i] += xg_col[ezl - 1][exl - 1][P];
-                }
             }
 
         // point source: at most one element per solve owns the source node
@@ -600,14 +554,6 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     store4(V.ck_u + base + j * pitch, un);
                 }
             }
-            if constexpr (CORR) {
-                GMEM double* Gp = V.G + base + j * pitch;
-                double gv[4];
-                load4nt(Gp, gv);
-#pragma unroll
-                for (int i = 0; i < P; ++i) gv[i] += (double)g[j][i];
-                store4nt(Gp, gv);
-            }
         }
         // domain edges: right column (i = P) of the last element column, top row (j = P)
         if (last_x || last_z) {
@@ -626,14 +572,13 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                             V.ck_u[base + o] = r[j][i];
                         }
                     }
-                    if constexpr (CORR) V.G[base + o] += (double)g[j][i];
                 }
         }
     }
 }
 
-// Element-local gradient correlation of (uf, l): the correlation half of
-// element_action with the same arithmetic (x part by rows, z part by columns).
+// Element-local gradient correlation of (uf, l): x part by rows, z part by
+// columns, g = gx (D_x uf)(D_x l) + gz (D_z uf)(D_z l).
 template <typename T, int P>
 __device__ __forceinline__ void element_corr(const ElemConst<T, P>& ec, const T (&l)[P + 1][P + 1],
                                              const T (&uf)[P + 1][P + 1], T (&g)[P + 1][P + 1]) {
@@ -675,6 +620,10 @@ struct CorrArgs {
     long rstep;       // elements between ring slots
     int c0, clen;     // chunk of adjoint steps it = c0 + clen - 1 .. c0
     double* G;
+    // slot mode: per-solve buffers from the slot table; chunk `chunk` of
+    // `chunk_len` steps of each ADJOINT slot's current segment
+    const sem_slot* slots;
+    int chunk, chunk_len;
 };
 
 // Gradient correlation of one chunk of adjoint steps, run after the chunk's
@@ -707,8 +656,37 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
     const bool owned = valid && exl >= 1 && ezl >= 1;
     const int pitch = a.pitch;
     const int base = (P * ez) * pitch + P * ex;
-    const long soff = b * a.sb;
-    GMEM double* __restrict__ Gp = gm(a.G + soff) + base;
+    // this solve's chunk: forward states u^{c0+1..}, adjoint ring, accumulator
+    const T* ufb;
+    const T* ringb;
+    double* Gb;
+    long ustep, rstep;
+    int c0, clen, ring_slots;
+    if (!a.slots) {
+        const long soff = b * a.sb;
+        ufb = a.uf + soff;
+        ringb = a.ring + soff;
+        Gb = a.G + soff;
+        ustep = a.ustep;
+        rstep = a.rstep;
+        c0 = a.c0;
+        clen = a.clen;
+        ring_slots = a.ring_slots;
+    } else {
+        const sem_slot& d = a.slots[b];
+        const int m_lo = a.chunk * a.chunk_len;
+        if (d.phase != SEM_SLOT_ADJOINT || m_lo >= d.len) return;  // uniform per workgroup
+        const int m_hi = m_lo + a.chunk_len < d.len ? m_lo + a.chunk_len : d.len;
+        // segment steps m_lo .. m_hi-1 are adjoint steps it = n0+len-1-m
+        c0 = d.n0 + d.len - m_hi;
+        clen = m_hi - m_lo;
+        ustep = rstep = d.seg_stride;
+        ufb = (const T*)d.seg + (long)(c0 + 1 - d.n0) * d.seg_stride;
+        ringb = (const T*)d.ring;
+        Gb = d.G;
+        ring_slots = d.ring_slots;
+    }
+    GMEM double* __restrict__ Gp = gm(Gb) + base;
     const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
 
     double acc[P][P];
@@ -729,14 +707,14 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
     const bool from_right = exl < EX - 1 && (t & 63) != 63 && ex + 1 < a.nex;
     const bool own5 = valid && !from_right;
     auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
-        const int it = a.c0 + s;
-        const GMEM T* __restrict__ up = gm(a.uf + soff + (long)s * a.ustep) + base_c;
-        const GMEM T* __restrict__ lp = gm(a.ring + soff + (long)((it + 1) % a.ring_slots) * a.rstep) + base_c;
+        const int it = c0 + s;
+        const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep) + base_c;
+        const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep) + base_c;
         fetch_block(lp, pitch, own5, l);
         fetch_block(up, pitch, own5, u);
     };
-    if constexpr (PIPE) fetch(a.clen - 1, lc, uc);
-    for (int s = a.clen - 1; s >= 0; --s) {
+    if constexpr (PIPE) fetch(clen - 1, lc, uc);
+    for (int s = clen - 1; s >= 0; --s) {
         T g[NP][NP];
         T ln[PIPE ? NP : 1][PIPE ? NP : 1], un[PIPE ? NP : 1][PIPE ? NP : 1];
         if constexpr (PIPE)

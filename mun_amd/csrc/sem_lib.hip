@@ -1080,6 +1080,30 @@ static int prepare_receivers(sem_plan* pl) {
 }
 
 template <typename T>
+static int slots_corr(sem_plan* pl, const sem_slot* slots, int nslots, int chunk, int chunk_len, hipStream_t st) {
+    sem_batch dummy;
+    std::memset(&dummy, 0, sizeof(dummy));
+    dummy.batch = nslots;
+    Layout L{};
+    const StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, L);
+    CorrArgs<T, 4> c;
+    std::memset(&c, 0, sizeof(c));
+    c.ec = a.ec;
+    c.nex = a.nex;
+    c.nez = a.nez;
+    c.pitch = a.pitch;
+    c.ntx = a.ntx;
+    c.ntiles = a.ntiles;
+    c.total = a.total;
+    c.nper = a.nper;
+    c.sb = a.sb;
+    c.slots = slots;
+    c.chunk = chunk;
+    c.chunk_len = chunk_len;
+    return launch_corr<T>(pl, c, st);
+}
+
+template <typename T>
 static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
                         bool final_record, hipStream_t st) {
     int rc = prepare_receivers(pl);
@@ -1105,7 +1129,7 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
     }
     if (mask & adj) {
         a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
-        if ((rc = launch_step<T, MODE_ADJ>(pl, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_ADJS>(pl, a, st))) return rc;
     }
     return SEM_OK;
 }
@@ -1119,6 +1143,15 @@ int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsig
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, m, mask, rec_pitch, false, st)
                                        : slots_launch<float>(pl, slots, nslots, m, mask, rec_pitch, false, st);
+}
+
+int sem_slots_correlate(sem_plan* pl, const sem_slot* slots, int nslots, int chunk, int chunk_len, void* stream) {
+    if (!pl || !slots || nslots < 1 || chunk < 0 || chunk_len < 1)
+        return fail(SEM_E_INVALID, "sem_slots_correlate: bad argument");
+    if ((long)pl->ntiles * nslots >= (1L << 31) - 64) return fail(SEM_E_INVALID, "too many slots for this mesh");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? slots_corr<double>(pl, slots, nslots, chunk, chunk_len, st)
+                                       : slots_corr<float>(pl, slots, nslots, chunk, chunk_len, st);
 }
 
 int sem_slots_record_final(sem_plan* pl, const sem_slot* slots, int nslots, unsigned mask, int rec_pitch,

@@ -82,11 +82,20 @@ class SolveResult:
         return self.finished - self.arrival
 
 
+def corr_chunk(S: int) -> int:
+    """Adjoint steps per gradient-correlation chunk in a lane (<= 10, dividing
+    a segment of S steps into equal chunks)."""
+    n = -(-S // 10)
+    return -(-S // n)
+
+
 def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4):
-    """HBM bytes one solve holds while resident."""
+    """HBM bytes one solve holds while resident: c2, u0, u1, w, 2K checkpoint
+    states, S+1 segment states, the adjoint ring and omega, FP64 accumulator,
+    traces."""
     E = pitch_elems
     K = max(1, math.ceil(nt / S))
-    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + 3) + E * 8
+    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + (corr_chunk(S) + 1) + 1) + E * 8
     traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
     return nodal + traces
 
@@ -113,6 +122,7 @@ class Lane:
         self.key = key
         self.C = capacity
         self.S = server.S
+        self.chunk = corr_chunk(self.S)
         self.rec_pitch = rec_pitch
         tile_rows = 15
         if server.autotune:
@@ -175,7 +185,8 @@ class Lane:
         elif a.phase == L.SLOT_RECOMPUTE:
             d.u0, d.w = b["cku"][a.k].data_ptr(), b["ckw"][a.k].data_ptr()
         elif a.phase == L.SLOT_ADJOINT:
-            d.u0, d.u1, d.w = b["l0"].data_ptr(), b["l1"].data_ptr(), b["om"].data_ptr()
+            d.w = b["om"].data_ptr()
+            d.ring, d.ring_slots = b["ring"].data_ptr(), self.chunk + 1
         return d
 
     def write_table(self):
@@ -192,6 +203,9 @@ class Lane:
         tab = C.c_void_p(self.table_dev.data_ptr())
         for m in range(self.S):
             L.check(lib.sem_slots_step(h, tab, self.C, m, mask, self.rec_pitch, sp))
+            if (m + 1) % self.chunk == 0 or m == self.S - 1:
+                # gradient correlation of the adjoint slots' chunk that just ended
+                L.check(lib.sem_slots_correlate(h, tab, self.C, m // self.chunk, self.chunk, sp))
 
     def launch_round(self, use_graph: bool):
         with torch.cuda.stream(self.stream):
@@ -263,7 +277,7 @@ class SEMServer:
         with torch.cuda.stream(lane.stream):
             z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
             b = {"u0": z(E), "u1": z(E), "w": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E),
-                 "l0": z(E), "l1": z(E), "om": z(E), "G": z(E, dt=torch.float64),
+                 "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(E, dt=torch.float64),
                  "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
                  "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
             if req.obs is not None:

@@ -790,12 +790,13 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
 
 // r = seis - obs (first nrec columns of each row), misfit[b] = 0.5 * sum r^2 in double.
 // Rows outside the misfit window [w0, w1) or off its decimation grid give r = 0.
+// nrec_of: per-solve receiver counts on the device, or null to use nrec.
 template <typename T>
 __global__ __launch_bounds__(256) void residual_kernel(const T* seis, const T* obs, T* res, double* misfit,
-                                                       const int* nrec_of, int nt, int pitch, long bs, int w0,
-                                                       int w1, int decim) {
+                                                       const int* nrec_of, int nrec, int nt, int pitch, long bs,
+                                                       int w0, int w1, int decim) {
     const long b = blockIdx.x;
-    const int nrec = nrec_of[b];
+    if (nrec_of) nrec = nrec_of[b];
     double acc = 0.0;
     const long total = (long)nt * pitch;
     for (long k = threadIdx.x; k < total; k += blockDim.x) {

@@ -600,7 +600,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     // residual and misfit
     T* RES = (T*)(ws + L.RES);
     residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs, RES, misfit,
-                                          (const int*)(ws + L.NREC), nt, bt->rec_pitch,
+                                          (const int*)(ws + L.NREC), 0, nt, bt->rec_pitch,
                                           (long)nt * bt->rec_pitch, pl->d.win_begin, pl->d.win_end,
                                           pl->d.win_decim);
     LAUNCH_CHECK();
@@ -1198,16 +1198,12 @@ int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, dou
     if (!pl || !seis || !obs || !res || !misfit || nrec < 0 || nrec > rec_pitch || nt < 1)
         return fail(SEM_E_INVALID, "sem_residual: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    // the kernel reads the receiver count from device memory: stage it in the misfit slot
-    static_assert(sizeof(double) >= sizeof(int), "");
-    HIP_TRY(hipMemcpyAsync(misfit, &nrec, sizeof(int), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
     if (pl->d.precision == SEM_FP64)
         residual_kernel<double><<<1, 256, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
-                                                   (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
+                                                   nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     else
         residual_kernel<float><<<1, 256, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
-                                                  (const int*)misfit, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
+                                                  nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     LAUNCH_CHECK();
     return SEM_OK;
 }

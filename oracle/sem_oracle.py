@@ -36,8 +36,20 @@ Pinning (what checks each function; tests live in ``tests/test_oracle_*.py``):
         of the full misfit in the sponge-aware ('exact') adjoint mode, dot
         product test of the adjoint recurrence.
     tensor B-spline map: partition of unity, linear reproduction, FD.
-    closed-curve velocity map (signed distance): parity unpinned beyond
-        self-consistency (distance to sampled curve, sign on simple shapes).
+    closed_curve_velocity / closed_curve_projection (clamped 6-point curve):
+        unsigned distance against brute force on a dense, independently
+        evaluated curve (Cox-de Boor), sign against its winding number, u*
+        on a nearest point, n_out = grad_X sd by FD, mean velocity on the
+        curve, projection = -d/dctrl (a^T v) by central differences.
+    periodic_curve_velocity / periodic_curve_projection: knot points of a
+        regular K-gon at radius R (2 + cos 2 pi/K) / 3 with sd, u* and normal
+        along the ray through them; projection = -d/dctrl (a^T v) by FD.
+    grid_background / increasing_grid: bilinear fields reproduced exactly,
+        nearest-neighbour ties, axis flips carry the values.
+    window_mask / window_from_times / forward_adjoint windows: rounding and
+        clamps, central FD of the windowed misfit (exact mode), the reference
+        mode's truncation before the window start.
+    forward_snapshots: each snapshot equals the state after its step.
 """
 from __future__ import annotations
 

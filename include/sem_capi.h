@@ -263,7 +263,7 @@ typedef struct {
     void* seg;             /* recompute: slots 1..len of u;  adjoint: u^{it+1} at slot it-n0+1 */
     long seg_stride;       /* elements between segment slots                          */
     const void* resid;     /* adjoint source [nt][rec_pitch]                          */
-    double* G;             /* gradient accumulator (padded layout, double)            */
+    double* G;             /* gradient accumulator, sem_plan_gradient_elems() doubles */
     void* ring;            /* adjoint: lambda_t in slot (t+1) % ring_slots, stride seg_stride */
     int ring_slots;        /* >= chunk_len + 1 of sem_slots_correlate                 */
     int reserved;
@@ -271,6 +271,12 @@ typedef struct {
 
 /* Padded nodal layout: row pitch and per-array element count. */
 int sem_plan_layout(const sem_plan* plan, int* pitch, long* array_elems);
+/* Doubles in one solve's gradient accumulator (sem_slot.G, sem_gradient_finalize):
+ * the padded nodal array followed by 9 per-element planes, so that every element
+ * keeps its own running sums for its 25 nodes and assembly happens once per solve.
+ * Replaces the per-step nodal assembly of accumulate_sensitivity_dJdc_torch
+ * (core.py:49-85); zero-fill before the first adjoint chunk. */
+int sem_plan_gradient_elems(const sem_plan* plan, long* elems);
 /* One step of every slot whose phase is in phase_mask (bit 1 << SEM_SLOT_*):
  * forward/recompute slots run the forward step, adjoint slots the adjoint step
  * (a mask mixing both launches both kernels).  m = step index within the
@@ -303,7 +309,8 @@ int sem_pad_nodal(sem_plan* plan, const void* src, void* dst_padded, int nmodel,
 /* res = seis - obs on the first nrec columns of nt rows, misfit = 0.5 sum res^2 (double) */
 int sem_residual(sem_plan* plan, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
                  int rec_pitch, void* stream);
-/* grad_c2 = scale * G, grad_c = 2 sqrt(c2) grad_c2 for one solve (G, c2 padded; outputs [N]) */
+/* grad_c2 = scale * G, grad_c = 2 sqrt(c2) grad_c2 for one solve, G assembled per
+ * node from the accumulator (sem_plan_gradient_elems doubles), c2 padded; outputs [N] */
 int sem_gradient_finalize(sem_plan* plan, const double* G, const void* c2_padded, double scale, double* grad_c,
                           double* grad_c2, void* stream);
 

@@ -25,7 +25,7 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_curve_project", "sem_plan_layout", "sem_slots_step", "sem_slots_record_final", "sem_pad_nodal",
            "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize",
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
-           "sem_slots_correlate",
+           "sem_slots_correlate", "sem_plan_gradient_elems",
            "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
     L.sem_curve_velocity.argtypes = [vp, i, vp, d, d, d, i, i, vp, vp, vp, vp]
     L.sem_curve_project.argtypes = [vp, i, vp, vp, d, d, d, vp, vp]
     L.sem_plan_layout.argtypes = [vp, ip, C.POINTER(C.c_long)]
+    L.sem_plan_gradient_elems.argtypes = [vp, C.POINTER(C.c_long)]
     L.sem_slots_step.argtypes = [vp, vp, i, i, C.c_uint, i, vp]
     L.sem_slots_record_final.argtypes = [vp, vp, i, C.c_uint, i, vp]
     L.sem_slots_correlate.argtypes = [vp, vp, i, i, i, vp]

@@ -577,42 +577,52 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
     }
 }
 
-// Element-local gradient correlation of (uf, l): x part by rows, z part by
-// columns, g = gx (D_x uf)(D_x l) + gz (D_z uf)(D_z l).
+// Element-local gradient correlation of (uf, l) added node by node into FP64
+// running sums: g(j, i) = gx(j, i) (D_x uf)(D_x l) + gz(j, i) (D_z uf)(D_z l)
+// at node (j, i), with no 5 x 5 temporary held across the element.
 template <typename T, int P>
-__device__ __forceinline__ void element_corr(const ElemConst<T, P>& ec, const T (&l)[P + 1][P + 1],
-                                             const T (&uf)[P + 1][P + 1], T (&g)[P + 1][P + 1]) {
+__device__ __forceinline__ void element_corr_add(const ElemConst<T, P>& ec, const T (&l)[P + 1][P + 1],
+                                                 const T (&uf)[P + 1][P + 1], double (&acc)[P + 1][P + 1]) {
     constexpr int NP = P + 1;
 #pragma unroll
     for (int j = 0; j < NP; ++j)
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            T d = T(0), du = T(0);
+        for (int i = 0; i < NP; ++i) {
+            T dxl = T(0), dxu = T(0), dzl = T(0), dzu = T(0);
 #pragma unroll
-            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * l[j][k];
+            for (int k = 0; k < NP; ++k) dxl += ec.D[i][k] * l[j][k];
 #pragma unroll
-            for (int k = 0; k < NP; ++k) du += ec.D[q][k] * uf[j][k];
-            g[j][q] = ec.gx[j][q] * (du * d);
-        }
+            for (int k = 0; k < NP; ++k) dxu += ec.D[i][k] * uf[j][k];
 #pragma unroll
-    for (int i = 0; i < NP; ++i)
+            for (int k = 0; k < NP; ++k) dzl += ec.D[j][k] * l[k][i];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            T d = T(0), du = T(0);
-#pragma unroll
-            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * l[k][i];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) du += ec.D[q][k] * uf[k][i];
-            g[q][i] += ec.gz[q][i] * (du * d);
+            for (int k = 0; k < NP; ++k) dzu += ec.D[j][k] * uf[k][i];
+            T g = ec.gx[j][i] * (dxu * dxl);
+            g += ec.gz[j][i] * (dzu * dzl);
+            acc[j][i] += (double)g;
         }
 }
 
+// Correlation tiles: CORR_X elements along x (one 16-lane group of a wave)
+// by CORR_Z element rows, one wave per workgroup, no halo.
+constexpr int CORR_X = 16;
+constexpr int CORR_Z = 4;
+
+// Gradient accumulator of one solve: `sb` doubles in the padded nodal layout
+// (each element's 16 owned nodes, node (4 ez + j, 4 ex + i), i, j < 4),
+// followed by 9 element planes of `gplane` = nez * nexp doubles (element
+// (ex, ez) at ez * nexp + ex): planes 0..4 hold the element's contributions to
+// its right-column nodes (i = 4, j = 0..4), planes 5..8 those to its top-row
+// nodes (j = 4, i = 0..3).  grad_finalize_kernel adds the (up to 4) element
+// contributions of each node in a fixed order.
 template <typename T, int P>
 struct CorrArgs {
     ElemConst<T, P> ec;
     int nex, nez, pitch;
-    int ntx, ntiles, total, nper;
-    long sb;
+    int ctx, ctiles, total, nper;  // correlation tiles per row, per solve; grid
+    long sb;                       // elements per nodal array of one solve
+    long gsb, gplane;              // accumulator doubles per solve, per element plane
+    int nexp;                      // element-plane row length
     const T* uf;      // u^{c0+1} of solve 0; u^{it+1} at uf + (it - c0) * ustep
     long ustep;
     const T* ring;    // adjoint states: lambda_it in slot (it + 1) % ring_slots
@@ -627,35 +637,32 @@ struct CorrArgs {
 };
 
 // Gradient correlation of one chunk of adjoint steps, run after the chunk's
-// adjoint steps: for it = c0 + clen - 1 down to c0 every node adds its
-// assembled corr(u^{it+1}, lambda_it) to an FP64 running sum that starts from
-// G and is stored back at the end.  Per node this is the same sequence of FP64
-// additions as adding into G after every step, so the result does not depend
-// on how the steps are chunked (checkpoint spacing, full history).  The tile
-// decomposition, halo and LDS exchange are the step kernel's; the running sums
-// of the owned 4 x 4 block live in registers across the chunk.
-template <typename T, int P, int EZ>
-__global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T, P> a) {
-    constexpr int TZ = EZ - 1;
+// adjoint steps: for it = c0 + clen - 1 down to c0 each element adds its 25
+// nodal contributions corr(u^{it+1}, lambda_it) to FP64 running sums that
+// start from the accumulator and are stored back at the end.  Every
+// contribution has a sum of its own (assembly happens once, in
+// grad_finalize_kernel), so each is the same sequence of FP64 additions
+// however the steps are chunked (checkpoint spacing, full history, slots),
+// and no step needs an exchange between threads: no LDS, no barrier, no halo
+// elements.  Without a software pipeline the kernel holds 146 VGPRs (3 waves
+// per SIMD), which hides the load latency as well as prefetching the next
+// step's blocks at 2 waves per SIMD does.
+template <typename T, int P>
+__global__ __launch_bounds__(CORR_X * CORR_Z) void corr_kernel(const CorrArgs<T, P> a) {
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
-    __shared__ T xg_col[EZ][EX][NP];
-    __shared__ T xg_row[EZ][EX][P];
-
     const int wgid = blockIdx.x;
     const int lin = (wgid % NXCD) * a.nper + wgid / NXCD;
     if (lin >= a.total) return;  // whole workgroup leaves together
-    const int tile = lin % a.ntiles;
-    const long b = lin / a.ntiles;
+    const int tile = lin % a.ctiles;
+    const long b = lin / a.ctiles;
     const int t = threadIdx.x;
-    const int exl = t % EX, ezl = t / EX;
-    const int tx = tile % a.ntx, tz = tile / a.ntx;
-    const int ex = tx * TX - 1 + exl;
-    const int ez = tz * TZ - 1 + ezl;
-    const bool valid = t < EX * EZ && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
-    const bool owned = valid && exl >= 1 && ezl >= 1;
+    const int exl = t % CORR_X, ezl = t / CORR_X;
+    const int tx = tile % a.ctx, tz = tile / a.ctx;
+    const int ex = tx * CORR_X + exl;
+    const int ez = tz * CORR_Z + ezl;
+    const bool valid = ex < a.nex && ez < a.nez;
     const int pitch = a.pitch;
-    const int base = (P * ez) * pitch + P * ex;
     // this solve's chunk: forward states u^{c0+1..}, adjoint ring, accumulator
     const T* ufb;
     const T* ringb;
@@ -666,7 +673,7 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
         const long soff = b * a.sb;
         ufb = a.uf + soff;
         ringb = a.ring + soff;
-        Gb = a.G + soff;
+        Gb = a.G + b * a.gsb;
         ustep = a.ustep;
         rstep = a.rstep;
         c0 = a.c0;
@@ -686,94 +693,61 @@ __global__ __launch_bounds__(tile_threads(EZ)) void corr_kernel(const CorrArgs<T
         Gb = d.G;
         ring_slots = d.ring_slots;
     }
-    GMEM double* __restrict__ Gp = gm(Gb) + base;
-    const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
-
-    double acc[P][P];
-    if (owned) {
-#pragma unroll
-        for (int j = 0; j < P; ++j) load4nt(Gp + j * pitch, acc[j]);
-    }
-    // FP32: software pipeline, the next step's u and lambda blocks are in
-    // flight while this step's correlation is computed (FP64 blocks would
-    // need twice the registers and load per step)
-    constexpr bool PIPE = std::is_same_v<T, float>;
-    T lc[NP][NP], uc[NP][NP];
     // every lane loads a block (clamped into the domain) so that the
     // cross-lane fifth-node reads see their neighbours' data
-    const int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
-    const int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+    const int exc = ex < a.nex ? ex : a.nex - 1;
+    const int ezc = ez < a.nez ? ez : a.nez - 1;
     const int base_c = (P * ezc) * pitch + P * exc;
-    const bool from_right = exl < EX - 1 && (t & 63) != 63 && ex + 1 < a.nex;
-    const bool own5 = valid && !from_right;
+    const bool from_right = exl < CORR_X - 1 && ex + 1 < a.nex;
+    GMEM double* __restrict__ Gp = gm(Gb) + base_c;
+    GMEM double* __restrict__ Ep = gm(Gb) + a.sb + (long)ezc * a.nexp + exc;
+
+    double acc[NP][NP];
+    if (valid) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const vd2 lo = __builtin_nontemporal_load(reinterpret_cast<const GMEM vd2*>(Gp + j * pitch));
+            const vd2 hi = __builtin_nontemporal_load(reinterpret_cast<const GMEM vd2*>(Gp + j * pitch + 2));
+            acc[j][0] = lo.x;
+            acc[j][1] = lo.y;
+            acc[j][2] = hi.x;
+            acc[j][3] = hi.y;
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) acc[j][P] = __builtin_nontemporal_load(Ep + j * a.gplane);
+#pragma unroll
+        for (int i = 0; i < P; ++i) acc[P][i] = __builtin_nontemporal_load(Ep + (NP + i) * a.gplane);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) acc[j][i] = 0.0;
+    }
+    T lc[NP][NP], uc[NP][NP];
     auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
         const int it = c0 + s;
         const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep) + base_c;
         const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep) + base_c;
-        fetch_block(lp, pitch, own5, l);
-        fetch_block(up, pitch, own5, u);
+        fetch_block(lp, pitch, !from_right, l);
+        fetch_block(up, pitch, !from_right, u);
     };
-    if constexpr (PIPE) fetch(clen - 1, lc, uc);
     for (int s = clen - 1; s >= 0; --s) {
-        T g[NP][NP];
-        T ln[PIPE ? NP : 1][PIPE ? NP : 1], un[PIPE ? NP : 1][PIPE ? NP : 1];
-        if constexpr (PIPE)
-            fetch(s > 0 ? s - 1 : 0, ln, un);  // the last step re-reads its own (cached) blocks
-        else
-            fetch(s, lc, uc);
+        fetch(s, lc, uc);
         finish_block(from_right, lc);
         finish_block(from_right, uc);
-        element_corr<T, P>(a.ec, lc, uc, g);
-        if (!valid) {
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int i = 0; i < NP; ++i) g[j][i] = T(0);
-        }
-        if constexpr (PIPE) {
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int i = 0; i < NP; ++i) {
-                    lc[j][i] = ln[j][i];
-                    uc[j][i] = un[j][i];
-                }
-        }
-        if (t < EX * EZ) {
-#pragma unroll
-            for (int j = 0; j < NP; ++j) xg_col[ezl][exl][j] = g[j][P];
-#pragma unroll
-            for (int i = 0; i < P; ++i) xg_row[ezl][exl][i] = g[P][i];
-        }
-        __syncthreads();
-        if (owned) {
-#pragma unroll
-            for (int j = 0; j < NP; ++j)
-#pragma unroll
-                for (int i = 0; i < NP; ++i) {
-                    if (i == 0) g[j][i] += xg_col[ezl][exl - 1][j];
-                    if (j == 0) g[j][i] += (i < P) ? xg_row[ezl - 1][exl][i] : xg_col[ezl - 1][exl][P];
-                    if (i == 0 && j == 0) g[j][i] += xg_col[ezl - 1][exl - 1][P];
-                }
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-#pragma unroll
-                for (int i = 0; i < P; ++i) acc[j][i] += (double)g[j][i];
-            if (last_x || last_z) {
-#pragma unroll
-                for (int j = 0; j < NP; ++j)
-#pragma unroll
-                    for (int i = 0; i < NP; ++i) {
-                        if (!((i == P && last_x && (j < P || last_z)) || (j == P && last_z && i < P))) continue;
-                        Gp[j * pitch + i] += (double)g[j][i];
-                    }
-            }
-        }
-        __syncthreads();  // exchange buffers are rewritten by the next step
+        element_corr_add<T, P>(a.ec, lc, uc, acc);
     }
-    if (owned) {
+    if (valid) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) store4nt(Gp + j * pitch, acc[j]);
+        for (int j = 0; j < P; ++j) {
+            const vd2 lo = {acc[j][0], acc[j][1]}, hi = {acc[j][2], acc[j][3]};
+            __builtin_nontemporal_store(lo, reinterpret_cast<GMEM vd2*>(Gp + j * pitch));
+            __builtin_nontemporal_store(hi, reinterpret_cast<GMEM vd2*>(Gp + j * pitch + 2));
+        }
+#pragma unroll
+        for (int j = 0; j < NP; ++j) __builtin_nontemporal_store(acc[j][P], Ep + j * a.gplane);
+#pragma unroll
+        for (int i = 0; i < P; ++i) __builtin_nontemporal_store(acc[P][i], Ep + (NP + i) * a.gplane);
     }
 }
 
@@ -878,18 +852,36 @@ __global__ void fill_kernel(T* p, T v, long n) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) p[k] = v;
 }
 
-// grad_c2 = scale * G ; grad_c = 2 sqrt(c2) grad_c2   (G padded, outputs [b][N])
+// Assembled sensitivity of node (ix, jz) from the gradient accumulator (layout
+// at CorrArgs): the owning element's own sum, then the left, lower and
+// lower-left elements' sums for the node, in that order.
+__device__ __forceinline__ double assemble_node(const double* Gb, long sb, long gplane, int nexp, int nex, int nez,
+                                                int pitch, int ix, int jz) {
+    const double* E = Gb + sb;
+    const int ex = ix >> 2, i = ix & 3, ez = jz >> 2, j = jz & 3;
+    const bool inx = ex < nex, inz = ez < nez;
+    double s = (inx && inz) ? Gb[(long)jz * pitch + ix] : 0.0;
+    if (i == 0 && ex > 0 && inz) s += E[j * gplane + (long)ez * nexp + (ex - 1)];
+    if (j == 0 && ez > 0 && inx) s += E[(5 + i) * gplane + (long)(ez - 1) * nexp + ex];
+    if (i == 0 && j == 0 && ex > 0 && ez > 0) s += E[4 * gplane + (long)(ez - 1) * nexp + (ex - 1)];
+    return s;
+}
+
+// grad_c2 = scale * G ; grad_c = 2 sqrt(c2) grad_c2   (G assembled from the
+// accumulator, c2 padded, outputs [b][N])
 template <typename T>
-__global__ void grad_finalize_kernel(const double* G, long sb, int NX, int NZ, int pitch, const T* c2,
-                                     const int* model_of, double scale, double* grad_c, double* grad_c2) {
+__global__ void grad_finalize_kernel(const double* G, long gsb, long sb, long gplane, int nexp, int nex, int nez,
+                                     int NX, int NZ, int pitch, const T* c2, const int* model_of, double scale,
+                                     double* grad_c, double* grad_c2) {
     const long b = blockIdx.y;
     const long m = model_of ? model_of[b] : b;
     const long N = (long)NX * NZ;
+    const double* Gb = G + b * gsb;
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
-        const long jz = k / NX, ix = k - jz * NX;
-        const double g2 = scale * G[b * sb + jz * pitch + ix];
+        const int jz = (int)(k / NX), ix = (int)(k - (long)jz * NX);
+        const double g2 = scale * assemble_node(Gb, sb, gplane, nexp, nex, nez, pitch, ix, jz);
         if (grad_c2) grad_c2[b * N + k] = g2;
-        if (grad_c) grad_c[b * N + k] = 2.0 * sqrt((double)c2[m * sb + jz * pitch + ix]) * g2;
+        if (grad_c) grad_c[b * N + k] = 2.0 * sqrt((double)c2[m * sb + (long)jz * pitch + ix]) * g2;
     }
 }
 

@@ -222,6 +222,10 @@ struct sem_plan {
     int ntx = 0, ntz = 0, ntiles = 0;
     int ez = EZ_DEFAULT;  // tile height in element rows incl. halo (15, 7 or 3)
     int tz = EZ_DEFAULT - 1;
+    // gradient correlation: tiles of CORR_X x CORR_Z elements; accumulator of
+    // gsb doubles per solve = sb nodal + 9 element planes of gplane doubles
+    int ctx = 0, ctiles = 0, nexp = 0;
+    long gplane = 0, gsb = 0;
     std::vector<double> xi, wq, Dm, x1d, z1d;
     std::vector<double> Dxv, Dzv;
     DevBuf d_Dx, d_Dz, d_x1d, d_z1d;
@@ -363,7 +367,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
         L.RING = take(state * (size_t)(L.C + 1));
         L.OM = take(state);
         L.RES = take((size_t)B * nt * pitch * Ts);
-        L.G = take((size_t)B * pl->sb * 8);
+        L.G = take((size_t)B * pl->gsb * 8);
     }
     L.total = o;
     return L;
@@ -441,21 +445,37 @@ int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
 }
 
 template <typename T>
+CorrArgs<T, 4> corr_args(const sem_plan* pl, const StepArgs<T, 4>& a, int B) {
+    CorrArgs<T, 4> c;
+    std::memset(&c, 0, sizeof(c));
+    c.ec = a.ec;
+    c.nex = a.nex;
+    c.nez = a.nez;
+    c.pitch = a.pitch;
+    c.ctx = pl->ctx;
+    c.ctiles = pl->ctiles;
+    c.total = pl->ctiles * B;
+    c.nper = (c.total + NXCD - 1) / NXCD;
+    c.sb = a.sb;
+    c.gsb = pl->gsb;
+    c.gplane = pl->gplane;
+    c.nexp = pl->nexp;
+    return c;
+}
+
+template <typename T>
 int launch_corr(const sem_plan* pl, const CorrArgs<T, 4>& c, hipStream_t st) {
+    (void)pl;
     const unsigned grid = (unsigned)c.nper * NXCD;
-    switch (pl->ez) {
-        case 15: corr_kernel<T, 4, 15><<<grid, tile_threads(15), 0, st>>>(c); break;
-        case 7: corr_kernel<T, 4, 7><<<grid, tile_threads(7), 0, st>>>(c); break;
-        case 3: corr_kernel<T, 4, 3><<<grid, tile_threads(3), 0, st>>>(c); break;
-        default: return fail(SEM_E_UNSUPPORTED, "tile height without a kernel instance");
-    }
+    corr_kernel<T, 4><<<grid, CORR_X * CORR_Z, 0, st>>>(c);
     LAUNCH_CHECK();
     return SEM_OK;
 }
 
 int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout& L) {
     if (!bt || bt->batch <= 0) return fail(SEM_E_INVALID, "batch must be >= 1");
-    if ((long)pl->ntiles * bt->batch >= (1L << 31) - 64) return fail(SEM_E_INVALID, "batch too large for this mesh");
+    if ((long)std::max(pl->ntiles, pl->ctiles) * bt->batch >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "batch too large for this mesh");
     if (!bt->c2 || !bt->wavelet || !bt->src_node || !bt->rec_set || (!bt->seis && bt->rec_pitch > 0))
         return fail(SEM_E_INVALID, "sem_batch: null pointer in c2/wavelet/src_node/rec_set/seis");
     if (bt->rec_pitch < 0) return fail(SEM_E_INVALID, "rec_pitch must be >= 0");
@@ -619,7 +639,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     const int t_lo = pl->d.adjoint_mode == SEM_ADJ_REFERENCE ? pl->d.win_begin : 0;
     if ((rc = fill<T>(ring_slot(t_hi - 1), T(0), state, st))) return rc;
     if ((rc = fill<T>(OM, T(0), state, st))) return rc;
-    if ((rc = fill<double>(G, 0.0, state, st))) return rc;
+    if ((rc = fill<double>(G, 0.0, (long)B * pl->gsb, st))) return rc;
 
     StepArgs<T, 4> f = base_args<T>(pl, bt, ws, L);  // recompute steps
     StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);  // adjoint steps
@@ -628,17 +648,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     a.src_node = nullptr;
     a.resid = RES;
     a.resid_bs = (long)nt * bt->rec_pitch;
-    CorrArgs<T, 4> c;
-    std::memset(&c, 0, sizeof(c));
-    c.ec = a.ec;
-    c.nex = a.nex;
-    c.nez = a.nez;
-    c.pitch = a.pitch;
-    c.ntx = a.ntx;
-    c.ntiles = a.ntiles;
-    c.total = a.total;
-    c.nper = a.nper;
-    c.sb = a.sb;
+    CorrArgs<T, 4> c = corr_args<T>(pl, a, B);
     c.ustep = state;
     c.ring = RING;
     c.ring_slots = ring_slots;
@@ -681,7 +691,8 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     }
     if (grad_c || grad_c2) {
         dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), B);
-        grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch,
+        grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->gsb, pl->sb, pl->gplane, pl->nexp, pl->d.nelem_x,
+                                                      pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch,
                                                       (const T*)(ws + L.C2P), bt->model_of, pl->J * inv_sigma2,
                                                       grad_c, grad_c2);
         LAUNCH_CHECK();
@@ -776,6 +787,11 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     pl->ntx = (d->nelem_x + TX - 1) / TX;
     pl->ntz = (d->nelem_z + pl->tz - 1) / pl->tz;
     pl->ntiles = pl->ntx * pl->ntz;
+    pl->ctx = (d->nelem_x + CORR_X - 1) / CORR_X;
+    pl->ctiles = pl->ctx * ((d->nelem_z + CORR_Z - 1) / CORR_Z);
+    pl->nexp = (d->nelem_x + 31) / 32 * 32;
+    pl->gplane = (long)d->nelem_z * pl->nexp;
+    pl->gsb = (pl->sb + 9 * pl->gplane + 63) / 64 * 64;
     pl->xi = gll_nodes(P);
     pl->wq = gll_weights(pl->xi);
     pl->Dm = diff_matrix(pl->xi);
@@ -1072,6 +1088,12 @@ int sem_plan_layout(const sem_plan* pl, int* pitch, long* elems) {
     return SEM_OK;
 }
 
+int sem_plan_gradient_elems(const sem_plan* pl, long* elems) {
+    if (!pl || !elems) return fail(SEM_E_INVALID, "sem_plan_gradient_elems: bad argument");
+    *elems = pl->gsb;
+    return SEM_OK;
+}
+
 }  // extern "C"
 
 static int prepare_receivers(sem_plan* pl) {
@@ -1086,17 +1108,7 @@ static int slots_corr(sem_plan* pl, const sem_slot* slots, int nslots, int chunk
     dummy.batch = nslots;
     Layout L{};
     const StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, L);
-    CorrArgs<T, 4> c;
-    std::memset(&c, 0, sizeof(c));
-    c.ec = a.ec;
-    c.nex = a.nex;
-    c.nez = a.nez;
-    c.pitch = a.pitch;
-    c.ntx = a.ntx;
-    c.ntiles = a.ntiles;
-    c.total = a.total;
-    c.nper = a.nper;
-    c.sb = a.sb;
+    CorrArgs<T, 4> c = corr_args<T>(pl, a, nslots);
     c.slots = slots;
     c.chunk = chunk;
     c.chunk_len = chunk_len;
@@ -1139,7 +1151,8 @@ extern "C" {
 int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
                    void* stream) {
     if (!pl || !slots || nslots < 1 || m < 0 || rec_pitch < 0) return fail(SEM_E_INVALID, "sem_slots_step: bad argument");
-    if ((long)pl->ntiles * nslots >= (1L << 31) - 64) return fail(SEM_E_INVALID, "too many slots for this mesh");
+    if ((long)std::max(pl->ntiles, pl->ctiles) * nslots >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "too many slots for this mesh");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, m, mask, rec_pitch, false, st)
                                        : slots_launch<float>(pl, slots, nslots, m, mask, rec_pitch, false, st);
@@ -1148,7 +1161,8 @@ int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsig
 int sem_slots_correlate(sem_plan* pl, const sem_slot* slots, int nslots, int chunk, int chunk_len, void* stream) {
     if (!pl || !slots || nslots < 1 || chunk < 0 || chunk_len < 1)
         return fail(SEM_E_INVALID, "sem_slots_correlate: bad argument");
-    if ((long)pl->ntiles * nslots >= (1L << 31) - 64) return fail(SEM_E_INVALID, "too many slots for this mesh");
+    if ((long)std::max(pl->ntiles, pl->ctiles) * nslots >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "too many slots for this mesh");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_corr<double>(pl, slots, nslots, chunk, chunk_len, st)
                                        : slots_corr<float>(pl, slots, nslots, chunk, chunk_len, st);
@@ -1214,11 +1228,13 @@ int sem_gradient_finalize(sem_plan* pl, const double* G, const void* c2p, double
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), 1);
     if (pl->d.precision == SEM_FP64)
-        grad_finalize_kernel<double><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch, (const double*)c2p,
-                                                           nullptr, pl->J * scale, grad_c, grad_c2);
+        grad_finalize_kernel<double><<<grid, 256, 0, st>>>(G, pl->gsb, pl->sb, pl->gplane, pl->nexp, pl->d.nelem_x,
+                                                           pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch,
+                                                           (const double*)c2p, nullptr, pl->J * scale, grad_c, grad_c2);
     else
-        grad_finalize_kernel<float><<<grid, 256, 0, st>>>(G, pl->sb, pl->NX, pl->NZ, pl->pitch, (const float*)c2p,
-                                                          nullptr, pl->J * scale, grad_c, grad_c2);
+        grad_finalize_kernel<float><<<grid, 256, 0, st>>>(G, pl->gsb, pl->sb, pl->gplane, pl->nexp, pl->d.nelem_x,
+                                                          pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch,
+                                                          (const float*)c2p, nullptr, pl->J * scale, grad_c, grad_c2);
     LAUNCH_CHECK();
     return SEM_OK;
 }

@@ -89,13 +89,14 @@ def corr_chunk(S: int) -> int:
     return -(-S // n)
 
 
-def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4):
+def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad_elems=None):
     """HBM bytes one solve holds while resident: c2, u0, u1, w, 2K checkpoint
-    states, S+1 segment states, the adjoint ring and omega, FP64 accumulator,
-    traces."""
+    states, S+1 segment states, the adjoint ring and omega, the FP64 gradient
+    accumulator (grad_elems doubles; default: one nodal array), traces."""
     E = pitch_elems
     K = max(1, math.ceil(nt / S))
-    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + (corr_chunk(S) + 1) + 1) + E * 8
+    GE = E if grad_elems is None else grad_elems
+    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + (corr_chunk(S) + 1) + 1) + GE * 8
     traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
     return nodal + traces
 
@@ -135,6 +136,9 @@ class Lane:
         pitch, elems = C.c_int(), C.c_long()
         L.check(self.plan.lib.sem_plan_layout(self.plan.h, C.byref(pitch), C.byref(elems)))
         self.E = elems.value
+        gelems = C.c_long()
+        L.check(self.plan.lib.sem_plan_gradient_elems(self.plan.h, C.byref(gelems)))
+        self.GE = gelems.value  # doubles in one solve's gradient accumulator
         # receiver tables: one set per admitted request; each receiver is listed in <= 4 tiles
         self.max_sets = server.max_sets
         self._reserve()
@@ -266,7 +270,7 @@ class SEMServer:
         if slot is None:
             return False
         plan = lane.plan
-        fp = request_footprint(plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S)
+        fp = request_footprint(plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S, grad_elems=lane.GE)
         if self.in_use + fp > self.budget:
             return False
         dev, E, S = self.device, lane.E, self.S
@@ -277,7 +281,7 @@ class SEMServer:
         with torch.cuda.stream(lane.stream):
             z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
             b = {"u0": z(E), "u1": z(E), "w": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E),
-                 "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(E, dt=torch.float64),
+                 "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
                  "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
                  "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
             if req.obs is not None:

@@ -603,10 +603,9 @@ __device__ __forceinline__ void element_corr_add(const ElemConst<T, P>& ec, cons
         }
 }
 
-// Correlation tiles: CORR_X elements along x (one 16-lane group of a wave)
-// by CORR_Z element rows, one wave per workgroup, no halo.
-constexpr int CORR_X = 16;
-constexpr int CORR_Z = 4;
+// Correlation workgroups: one wave of CORR_W consecutive elements in
+// row-major element order (rows wrap inside a wave), no halo.
+constexpr int CORR_W = 64;
 
 // Gradient accumulator of one solve: `sb` doubles in the padded nodal layout
 // (each element's 16 owned nodes, node (4 ez + j, 4 ex + i), i, j < 4),
@@ -619,7 +618,7 @@ template <typename T, int P>
 struct CorrArgs {
     ElemConst<T, P> ec;
     int nex, nez, pitch;
-    int ctx, ctiles, total, nper;  // correlation tiles per row, per solve; grid
+    int ctiles, total, nper;       // correlation workgroups per solve; grid
     long sb;                       // elements per nodal array of one solve
     long gsb, gplane;              // accumulator doubles per solve, per element plane
     int nexp;                      // element-plane row length
@@ -648,7 +647,7 @@ struct CorrArgs {
 // per SIMD), which hides the load latency as well as prefetching the next
 // step's blocks at 2 waves per SIMD does.
 template <typename T, int P>
-__global__ __launch_bounds__(CORR_X * CORR_Z) void corr_kernel(const CorrArgs<T, P> a) {
+__global__ __launch_bounds__(CORR_W) void corr_kernel(const CorrArgs<T, P> a) {
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
     const int wgid = blockIdx.x;
@@ -657,11 +656,12 @@ __global__ __launch_bounds__(CORR_X * CORR_Z) void corr_kernel(const CorrArgs<T,
     const int tile = lin % a.ctiles;
     const long b = lin / a.ctiles;
     const int t = threadIdx.x;
-    const int exl = t % CORR_X, ezl = t / CORR_X;
-    const int tx = tile % a.ctx, tz = tile / a.ctx;
-    const int ex = tx * CORR_X + exl;
-    const int ez = tz * CORR_Z + ezl;
-    const bool valid = ex < a.nex && ez < a.nez;
+    // lanes past the last element work on it (clamped) and store nothing
+    const int ne = a.nex * a.nez;
+    const int e = tile * CORR_W + t;
+    const bool valid = e < ne;
+    const int ex = (valid ? e : ne - 1) % a.nex;
+    const int ez = (valid ? e : ne - 1) / a.nex;
     const int pitch = a.pitch;
     // this solve's chunk: forward states u^{c0+1..}, adjoint ring, accumulator
     const T* ufb;
@@ -693,14 +693,13 @@ __global__ __launch_bounds__(CORR_X * CORR_Z) void corr_kernel(const CorrArgs<T,
         Gb = d.G;
         ring_slots = d.ring_slots;
     }
-    // every lane loads a block (clamped into the domain) so that the
-    // cross-lane fifth-node reads see their neighbours' data
-    const int exc = ex < a.nex ? ex : a.nex - 1;
-    const int ezc = ez < a.nez ? ez : a.nez - 1;
-    const int base_c = (P * ezc) * pitch + P * exc;
-    const bool from_right = exl < CORR_X - 1 && ex + 1 < a.nex;
-    GMEM double* __restrict__ Gp = gm(Gb) + base_c;
-    GMEM double* __restrict__ Ep = gm(Gb) + a.sb + (long)ezc * a.nexp + exc;
+    // every lane loads a block so that the cross-lane fifth-node reads see
+    // their neighbours' data; lane + 1 holds the right neighbour unless the
+    // row ends there
+    const int base = (P * ez) * pitch + P * ex;
+    const bool from_right = t < CORR_W - 1 && ex + 1 < a.nex;
+    GMEM double* __restrict__ Gp = gm(Gb) + base;
+    GMEM double* __restrict__ Ep = gm(Gb) + a.sb + (long)ez * a.nexp + ex;
 
     double acc[NP][NP];
     if (valid) {
@@ -726,8 +725,8 @@ __global__ __launch_bounds__(CORR_X * CORR_Z) void corr_kernel(const CorrArgs<T,
     T lc[NP][NP], uc[NP][NP];
     auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
         const int it = c0 + s;
-        const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep) + base_c;
-        const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep) + base_c;
+        const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep) + base;
+        const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep) + base;
         fetch_block(lp, pitch, !from_right, l);
         fetch_block(up, pitch, !from_right, u);
     };

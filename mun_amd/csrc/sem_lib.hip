@@ -222,9 +222,10 @@ struct sem_plan {
     int ntx = 0, ntz = 0, ntiles = 0;
     int ez = EZ_DEFAULT;  // tile height in element rows incl. halo (15, 7 or 3)
     int tz = EZ_DEFAULT - 1;
-    // gradient correlation: tiles of CORR_X x CORR_Z elements; accumulator of
-    // gsb doubles per solve = sb nodal + 9 element planes of gplane doubles
-    int ctx = 0, ctiles = 0, nexp = 0;
+    // gradient correlation: ctiles workgroups of CORR_W elements per solve;
+    // accumulator of gsb doubles per solve = sb nodal + 9 element planes of
+    // gplane doubles
+    int ctiles = 0, nexp = 0;
     long gplane = 0, gsb = 0;
     std::vector<double> xi, wq, Dm, x1d, z1d;
     std::vector<double> Dxv, Dzv;
@@ -452,7 +453,6 @@ CorrArgs<T, 4> corr_args(const sem_plan* pl, const StepArgs<T, 4>& a, int B) {
     c.nex = a.nex;
     c.nez = a.nez;
     c.pitch = a.pitch;
-    c.ctx = pl->ctx;
     c.ctiles = pl->ctiles;
     c.total = pl->ctiles * B;
     c.nper = (c.total + NXCD - 1) / NXCD;
@@ -467,7 +467,7 @@ template <typename T>
 int launch_corr(const sem_plan* pl, const CorrArgs<T, 4>& c, hipStream_t st) {
     (void)pl;
     const unsigned grid = (unsigned)c.nper * NXCD;
-    corr_kernel<T, 4><<<grid, CORR_X * CORR_Z, 0, st>>>(c);
+    corr_kernel<T, 4><<<grid, CORR_W, 0, st>>>(c);
     LAUNCH_CHECK();
     return SEM_OK;
 }
@@ -756,6 +756,10 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     *out = nullptr;
     if (d->order != 4) return fail(SEM_E_UNSUPPORTED, "only polynomial order 4 has kernels");
     if (d->nelem_x < 1 || d->nelem_z < 1) return fail(SEM_E_INVALID, "nelem_x/nelem_z must be >= 1");
+    // kernels address nodes of one solve's padded arrays with 32-bit offsets
+    if (d->nelem_x > (1 << 20) || d->nelem_z > (1 << 20) ||
+        ((long)d->nelem_z * 4 + 1) * (((long)d->nelem_x * 4 + 1 + 31) / 32 * 32) >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "mesh too large: one padded nodal array must hold fewer than 2^31 nodes");
     if (!(d->xmax > d->xmin) || !(d->zmax > d->zmin)) return fail(SEM_E_INVALID, "empty domain");
     if (!(d->dt > 0) || d->nt < 1) return fail(SEM_E_INVALID, "dt must be > 0 and nt >= 1");
     if (d->precision != SEM_FP32 && d->precision != SEM_FP64) return fail(SEM_E_INVALID, "bad precision");
@@ -787,8 +791,7 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     pl->ntx = (d->nelem_x + TX - 1) / TX;
     pl->ntz = (d->nelem_z + pl->tz - 1) / pl->tz;
     pl->ntiles = pl->ntx * pl->ntz;
-    pl->ctx = (d->nelem_x + CORR_X - 1) / CORR_X;
-    pl->ctiles = pl->ctx * ((d->nelem_z + CORR_Z - 1) / CORR_Z);
+    pl->ctiles = (int)(((long)d->nelem_x * d->nelem_z + CORR_W - 1) / CORR_W);
     pl->nexp = (d->nelem_x + 31) / 32 * 32;
     pl->gplane = (long)d->nelem_z * pl->nexp;
     pl->gsb = (pl->sb + 9 * pl->gplane + 63) / 64 * 64;

@@ -37,7 +37,9 @@ def _desc(**kw):
 @pytest.mark.parametrize("kw,code", [
     (dict(order=5), -2), (dict(nelem_x=0), -1), (dict(xmax=0.0), -1), (dict(dt=0.0), -1),
     (dict(nt=0), -1), (dict(precision=7), -1), (dict(adjoint_mode=3), -1), (dict(record_mode=2), -1),
-    (dict(sponge_width=0.0), -1)])
+    (dict(sponge_width=0.0), -1),
+    # 32-bit node offsets: 11585^2 elements pad to 46341 x 46368 > 2^31 nodes
+    (dict(nelem_x=11585, nelem_z=11585), -1), (dict(nelem_x=1 << 21), -1)])
 def test_plan_create_rejects_bad_descriptors(kw, code):
     lib = L.lib()
     h = C.c_void_p()

@@ -64,6 +64,32 @@ __device__ __forceinline__ const GMEM T* gm(const T* p) {
     return (const GMEM T*)p;
 }
 
+// y = M x for a 5 x 5 matrix with M[a][b] = -M[4-a][4-b] (the GLL
+// differentiation matrix, and D^T scaled by the symmetric weights, are of this
+// form): with e_b = x_b + x_{4-b}, o_b = x_b - x_{4-b},
+//   y_a, y_{4-a} = S_a +- T_a,  S_a = s_a0 o_0 + s_a1 o_1,
+//   T_a = t_a0 e_0 + t_a1 e_1 + t_a2 x_2   (a = 0, 1),
+//   y_2 = m0 o_0 + m1 o_1,
+// s_ab = (M[a][b] - M[a][4-b]) / 2, t_ab = (M[a][b] + M[a][4-b]) / 2,
+// t_a2 = M[a][2], m_b = M[2][b]: 20 operations instead of 25.
+template <typename T>
+struct Fold5 {
+    T s[2][2], t[2][3], m[2];
+};
+
+template <typename T>
+__device__ __forceinline__ void apply5(const Fold5<T>& f, T x0, T x1, T x2, T x3, T x4, T (&y)[5]) {
+    const T e0 = x0 + x4, o0 = x0 - x4, e1 = x1 + x3, o1 = x1 - x3;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const T S = f.s[a][0] * o0 + f.s[a][1] * o1;
+        const T D = f.t[a][0] * e0 + f.t[a][1] * e1 + f.t[a][2] * x2;
+        y[a] = S + D;
+        y[4 - a] = S - D;
+    }
+    y[2] = f.m[0] * o0 + f.m[1] * o1;
+}
+
 template <typename T, int P>
 struct ElemConst {
     T D[P + 1][P + 1];   // D[a][k] = l_k'(xi_a): derivative at node a
@@ -75,6 +101,8 @@ struct ElemConst {
     T iw[P + 1];         // 1 / w_i          : inverse assembled weight inside an element
     T iw_shared;         // 1 / (w_0 + w_P)  : inverse weight on a shared element edge
     T dt2;               // dt^2
+    // D and Dw^T folded for apply5 (both are centro-antisymmetric on GLL nodes)
+    Fold5<T> fD, fDwT;
 };
 
 template <typename T, int P>
@@ -306,44 +334,29 @@ __device__ __forceinline__ void load4nt(const GMEM T* __restrict__ p, T (&v)[4])
 template <typename T, int P>
 __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const T (&u)[P + 1][P + 1],
                                                const T (&c2)[P + 1][P + 1], T (&r)[P + 1][P + 1]) {
+    static_assert(P == 4, "apply5 is the 5-node operator");
     constexpr int NP = P + 1;
     // x direction, row by row
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-        T f[NP];
+        T d[NP], s[NP];
+        apply5(ec.fD, u[j][0], u[j][1], u[j][2], u[j][3], u[j][4], d);
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            T d = T(0);
+        for (int q = 0; q < NP; ++q) d[q] *= c2[j][q];
+        apply5(ec.fDwT, d[0], d[1], d[2], d[3], d[4], s);
 #pragma unroll
-            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[j][k];
-            f[q] = c2[j][q] * d;
-        }
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            T s = T(0);
-#pragma unroll
-            for (int q = 0; q < NP; ++q) s += ec.Dw[q][i] * f[q];
-            r[j][i] = ec.ax[j] * s;
-        }
+        for (int i = 0; i < NP; ++i) r[j][i] = ec.ax[j] * s[i];
     }
     // z direction, column by column
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-        T f[NP];
+        T d[NP], s[NP];
+        apply5(ec.fD, u[0][i], u[1][i], u[2][i], u[3][i], u[4][i], d);
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            T d = T(0);
+        for (int q = 0; q < NP; ++q) d[q] *= c2[q][i];
+        apply5(ec.fDwT, d[0], d[1], d[2], d[3], d[4], s);
 #pragma unroll
-            for (int k = 0; k < NP; ++k) d += ec.D[q][k] * u[k][i];
-            f[q] = c2[q][i] * d;
-        }
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            T s = T(0);
-#pragma unroll
-            for (int q = 0; q < NP; ++q) s += ec.Dw[q][j] * f[q];
-            r[j][i] += ec.az[i] * s;
-        }
+        for (int j = 0; j < NP; ++j) r[j][i] += ec.az[i] * s[j];
     }
 }
 
@@ -583,24 +596,29 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
 template <typename T, int P>
 __device__ __forceinline__ void element_corr_add(const ElemConst<T, P>& ec, const T (&l)[P + 1][P + 1],
                                                  const T (&uf)[P + 1][P + 1], double (&acc)[P + 1][P + 1]) {
+    static_assert(P == 4, "apply5 is the 5-node operator");
     constexpr int NP = P + 1;
+    T gxp[NP][NP];
 #pragma unroll
-    for (int j = 0; j < NP; ++j)
+    for (int j = 0; j < NP; ++j) {
+        T dl[NP], du[NP];
+        apply5(ec.fD, l[j][0], l[j][1], l[j][2], l[j][3], l[j][4], dl);
+        apply5(ec.fD, uf[j][0], uf[j][1], uf[j][2], uf[j][3], uf[j][4], du);
 #pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            T dxl = T(0), dxu = T(0), dzl = T(0), dzu = T(0);
+        for (int i = 0; i < NP; ++i) gxp[j][i] = ec.gx[j][i] * (du[i] * dl[i]);
+    }
 #pragma unroll
-            for (int k = 0; k < NP; ++k) dxl += ec.D[i][k] * l[j][k];
+    for (int i = 0; i < NP; ++i) {
+        T dl[NP], du[NP];
+        apply5(ec.fD, l[0][i], l[1][i], l[2][i], l[3][i], l[4][i], dl);
+        apply5(ec.fD, uf[0][i], uf[1][i], uf[2][i], uf[3][i], uf[4][i], du);
 #pragma unroll
-            for (int k = 0; k < NP; ++k) dxu += ec.D[i][k] * uf[j][k];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) dzl += ec.D[j][k] * l[k][i];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) dzu += ec.D[j][k] * uf[k][i];
-            T g = ec.gx[j][i] * (dxu * dxl);
-            g += ec.gz[j][i] * (dzu * dzl);
+        for (int j = 0; j < NP; ++j) {
+            T g = gxp[j][i];
+            g += ec.gz[j][i] * (du[j] * dl[j]);
             acc[j][i] += (double)g;
         }
+    }
 }
 
 // Correlation workgroups: one wave of CORR_W consecutive elements in

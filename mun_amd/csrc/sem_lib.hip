@@ -411,6 +411,20 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     a.sb = pl->sb;
     a.c2 = (const T*)(ws + L.C2P);
     a.model_of = bt->model_of;
+    // folded D and Dw^T for apply5 (M[a][b] = -M[4-a][4-b])
+    auto fold = [&](auto M, Fold5<T>& f) {
+        for (int r = 0; r < 2; ++r) {
+            for (int b = 0; b < 2; ++b) {
+                f.s[r][b] = (T)(0.5 * (M(r, b) - M(r, 4 - b)));
+                f.t[r][b] = (T)(0.5 * (M(r, b) + M(r, 4 - b)));
+            }
+            f.t[r][2] = (T)M(r, 2);
+        }
+        f.m[0] = (T)M(2, 0);
+        f.m[1] = (T)M(2, 1);
+    };
+    fold([&](int r, int c) { return pl->Dm[r * (P + 1) + c]; }, a.ec.fD);
+    fold([&](int r, int c) { return pl->Dm[c * (P + 1) + r] * pl->wq[c]; }, a.ec.fDwT);
     for (int i = 0; i <= P; ++i) a.ec.iw[i] = (T)(1.0 / pl->wq[i]);
     a.ec.iw_shared = (T)(1.0 / (pl->wq[0] + pl->wq[P]));
     a.ec.dt2 = (T)(pl->d.dt * pl->d.dt);

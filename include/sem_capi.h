@@ -123,8 +123,27 @@ int sem_receiver_set_size(const sem_plan* plan, int set_id, int* nrec);
 int sem_source_node(const sem_plan* plan, double x, double z, int* node);
 
 /* Workspace bytes for sem_forward (with_adjoint = 0) or sem_forward_adjoint
- * (with_adjoint = 1) at the given batch size and receiver pitch. */
+ * (with_adjoint = 1) at the given batch size and receiver pitch, under the
+ * plan's current schedule. */
 int sem_workspace_bytes(const sem_plan* plan, int batch, int rec_pitch, int with_adjoint, size_t* bytes);
+
+/* Schedule of sem_forward_adjoint (no reference counterpart: the reference
+ * solves one model at a time, core.py:1067-1371).  A batch of more than
+ * `group` solves runs as groups of consecutive solves (sizes differing by at
+ * most one), `lanes` groups at a time on streams the plan owns, each lane with
+ * a workspace region of its own; the call still enqueues on the caller's
+ * stream (the lanes start after it and the stream waits for them).
+ * checkpoint_every as in sem_desc (<= 0: full forward history, no recompute).
+ * group <= 0 runs the whole batch at once (the default).  Results are
+ * bitwise identical under every schedule; the workspace size changes. */
+int sem_plan_set_schedule(sem_plan* plan, int checkpoint_every, int group, int lanes);
+int sem_plan_get_schedule(const sem_plan* plan, int* checkpoint_every, int* group, int* lanes);
+/* Choose and set a schedule for `batch` solves whose workspace fits in
+ * budget_bytes: the full forward history for the whole batch if it fits, else
+ * up to 3 lanes of full-history groups as large as fit (when they keep the GPU
+ * busy), else checkpoints (the plan's checkpoint_every, or 100 steps) for as
+ * many solves at once as fit.  SEM_E_NOMEM if not even one solve fits. */
+int sem_plan_schedule_auto(sem_plan* plan, int batch, int rec_pitch, size_t budget_bytes);
 
 /* Forward solve of a batch; replaces run() / _time_stepping_loop_torch
  * (core.py:334-435, 598-659).  Writes b->seis. */

@@ -25,7 +25,8 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_curve_project", "sem_plan_layout", "sem_slots_step", "sem_slots_record_final", "sem_pad_nodal",
            "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize",
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
-           "sem_slots_correlate", "sem_plan_gradient_elems",
+           "sem_slots_correlate", "sem_plan_gradient_elems", "sem_plan_set_schedule", "sem_plan_get_schedule",
+           "sem_plan_schedule_auto",
            "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
@@ -96,6 +97,9 @@ def lib() -> C.CDLL:
     L.sem_receiver_set_size.argtypes = [vp, i, ip]
     L.sem_source_node.argtypes = [vp, d, d, ip]
     L.sem_workspace_bytes.argtypes = [vp, i, i, i, C.POINTER(sz)]
+    L.sem_plan_set_schedule.argtypes = [vp, i, i, i]
+    L.sem_plan_get_schedule.argtypes = [vp, ip, ip, ip]
+    L.sem_plan_schedule_auto.argtypes = [vp, i, i, sz]
     L.sem_forward.argtypes = [vp, C.POINTER(SemBatch), vp, sz, vp]
     L.sem_forward_adjoint.argtypes = [vp, C.POINTER(SemBatch), vp, d, vp, vp, vp, vp, sz, vp]
     L.sem_bspline_map.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]

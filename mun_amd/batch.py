@@ -35,7 +35,10 @@ class BatchSolver:
     @classmethod
     def from_workload(cls, wl, tile_rows="auto", **plan_kw):
         """Solver for a workload; tile_rows='auto' asks the autotuner for the
-        fastest tile height at this mesh and batch (models x shots)."""
+        fastest tile height at this mesh and batch (models x shots).  The plan
+        picks its schedule per call from free device memory
+        (Plan.schedule_auto) unless plan_kw sets schedule='whole'."""
+        plan_kw.setdefault("schedule", "auto")
         if tile_rows == "auto":
             from .autotune import best_tile_rows
             tile_rows = best_tile_rows(wl.mesh.nelem_x, wl.mesh.nelem_z, wl.nmodel * wl.nshot,

@@ -47,6 +47,9 @@ constexpr int TX = EX - 1;        // owned elements along x per tile (64 nodes)
 constexpr int EZ_DEFAULT = 15;
 __host__ __device__ constexpr int tile_threads(int ez) { return (EX * ez + 63) / 64 * 64; }
 constexpr int NXCD = 8;           // XCDs of an MI355X
+// Last-level (Infinity) cache of an MI355X: launches whose working set fits
+// run their state streams with plain accesses, larger ones non-temporally.
+constexpr size_t LLC_BYTES = size_t(256) << 20;
 
 // MODE_ADJS: adjoint step; its gradient correlation runs later per chunk of
 // steps in corr_kernel.
@@ -299,7 +302,10 @@ __device__ __forceinline__ void store4(GMEM T* __restrict__ p, const T (&v)[4]) 
 
 // Non-temporal variants for streams touched once per step (the increment and
 // the new state): they skip cache allocation, which keeps the halo-shared
-// inputs (u, c^2) resident and lets reads and writes stream.
+// inputs (u, c^2) resident and lets reads and writes stream.  They pay off
+// only when a launch's working set is larger than the last-level cache; when
+// it fits (LLC_BYTES), plain accesses let the next step find the new state and
+// the increment there (step_kernel's NTS parameter).
 template <typename T>
 __device__ __forceinline__ void store4nt(GMEM T* __restrict__ p, const T (&v)[4]) {
     if constexpr (std::is_same_v<T, float>) {
@@ -360,7 +366,9 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
     }
 }
 
-template <typename T, int P, int MODE, int EZ>
+// NTS: non-temporal loads and stores of the increment and the new state (for
+// launches whose working set exceeds the last-level cache).
+template <typename T, int P, int MODE, int EZ, bool NTS>
 __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T, P> a) {
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
@@ -445,7 +453,10 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
             if (owned) {
                 const GMEM T* __restrict__ wq = V.w + base;
 #pragma unroll
-                for (int j = 0; j < P; ++j) load4nt(wq + j * pitch, wv[j]);
+                for (int j = 0; j < P; ++j) {
+                    if constexpr (NTS) load4nt(wq + j * pitch, wv[j]);
+                    else load4(wq + j * pitch, wv[j]);
+                }
                 if (a.damp) {
                     load4(gm(a.Dx) + P * ex, dxv);
                     load4(gm(a.Dz) + P * ez, dzv);
@@ -559,8 +570,13 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                 wn[i] = update(j, i, wv[j][i]);
                 un[i] = r[j][i];
             }
-            store4nt(wp + j * pitch, wn);
-            store4nt(uo + j * pitch, un);
+            if constexpr (NTS) {
+                store4nt(wp + j * pitch, wn);
+                store4nt(uo + j * pitch, un);
+            } else {
+                store4(wp + j * pitch, wn);
+                store4(uo + j * pitch, un);
+            }
             if constexpr (MODE == MODE_FWD) {
                 if (V.ck_u) {
                     store4(V.ck_w + base + j * pitch, wn);

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -237,8 +238,20 @@ struct sem_plan {
     bool rec_dirty = true;
     size_t rec_up_entries = 0, rec_up_ptrs = 0;  // prefix already on the device
     std::map<std::pair<int, int>, BsplineTables*> bsp;
+    // schedule of sem_forward_adjoint (sem_plan_set_schedule): groups of at
+    // most sched_group solves (0: the whole batch), sched_lanes in flight
+    int sched_group = 0, sched_lanes = 1;
+    int ckpt_desc = 0;  // checkpoint_every as created (sem_plan_schedule_auto's fallback spacing)
+    // lane streams and events, created on first use on device lane_dev
+    int lane_dev = -1;
+    std::vector<hipStream_t> lane_st;
+    std::vector<hipEvent_t> lane_ev;
+    hipEvent_t fork_ev = nullptr;
     ~sem_plan() {
         for (auto& kv : bsp) delete kv.second;
+        for (hipStream_t s : lane_st) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : lane_ev) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
     }
 };
 
@@ -318,11 +331,37 @@ int upload_receivers(sem_plan* pl) {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Batch-level workspace: padded squared velocities of every model, the
+// per-solve receiver-set ids and receiver counts, a flag word.
+struct BatchLayout {
+    size_t C2P, NREC, RSET, FLAG, total;
+};
+
+BatchLayout batch_layout(const sem_plan* pl, int B) {
+    const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
+    BatchLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += align256(bytes);
+        return at;
+    };
+    L.C2P = take((size_t)B * pl->sb * Ts);
+    L.NREC = take((size_t)B * 4);
+    L.RSET = take((size_t)B * 4);
+    L.FLAG = take(4);
+    L.total = o;
+    return L;
+}
+
+// Workspace of one group of solves (a lane's region; the whole batch when it
+// is not split): states, checkpoints or the full forward history, the adjoint
+// ring, residuals and the gradient accumulator.
 struct Layout {
     bool full_history = false;
     int S = 0, nseg = 0;
     int C = 0;  // adjoint steps per gradient-correlation chunk
-    size_t C2P, U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, NREC, RSET, FLAG, total;
+    size_t U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, total;
 };
 
 // Correlation chunk: at most 32 adjoint steps, dividing a segment of S steps
@@ -334,14 +373,15 @@ int corr_chunk(int S) {
     return (S + nchunk - 1) / nchunk;
 }
 
-// Workspace regions; every nodal array is [batch][sb] in the padded layout.
-Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
+// Every nodal array is [solves][sb] in the padded layout.  ckpt: steps per
+// checkpoint, <= 0 or >= nt for the full forward history.
+Layout layout(const sem_plan* pl, int B, int pitch, bool adj, int ckpt) {
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
     const size_t state = (size_t)B * pl->sb * Ts;
     Layout L{};
     const int nt = pl->d.nt;
-    L.full_history = adj && (pl->d.checkpoint_every <= 0 || pl->d.checkpoint_every >= nt);
-    L.S = L.full_history ? nt : pl->d.checkpoint_every;
+    L.full_history = adj && (ckpt <= 0 || ckpt >= nt);
+    L.S = L.full_history ? nt : ckpt;
     L.nseg = adj ? (L.full_history ? 1 : (nt + L.S - 1) / L.S) : 0;
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -349,13 +389,9 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
         o += align256(bytes);
         return at;
     };
-    L.C2P = take(state);
     L.U0 = take(state);
     L.U1 = take(state);
     L.W = take(state);
-    L.NREC = take((size_t)B * 4);
-    L.RSET = take((size_t)B * 4);
-    L.FLAG = take(4);
     if (adj) {
         if (!L.full_history) {
             L.CKU = take(state * L.nseg);
@@ -374,6 +410,48 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj) {
     return L;
 }
 
+// How sem_forward_adjoint runs a batch of B solves: ngroups groups of
+// consecutive solves whose sizes differ by at most one, `lanes` of them in
+// flight at once (each lane on a stream of its own with a workspace region of
+// its own), every group with the history policy `ckpt`.
+struct Sched {
+    int group = 1, lanes = 1, ngroups = 1, ckpt = 0;
+    int first(int i, int B) const { return i * (B / ngroups) + std::min(i, B % ngroups); }
+    int size(int i, int B) const { return B / ngroups + (i < B % ngroups ? 1 : 0); }
+};
+
+Sched schedule(const sem_plan* pl, int B, bool adj) {
+    Sched s;
+    s.ckpt = pl->d.checkpoint_every;
+    s.group = B;
+    if (!adj || pl->sched_group <= 0 || pl->sched_group >= B) return s;
+    s.ngroups = (B + pl->sched_group - 1) / pl->sched_group;
+    s.lanes = std::max(1, std::min(pl->sched_lanes, s.ngroups));
+    // as many groups per lane on every lane (no lane idles at the end)
+    s.ngroups = std::min(B, (s.ngroups + s.lanes - 1) / s.lanes * s.lanes);
+    s.group = (B + s.ngroups - 1) / s.ngroups;  // the largest group
+    return s;
+}
+
+size_t workspace_total(const sem_plan* pl, int B, int pitch, bool adj) {
+    const Sched s = schedule(pl, B, adj);
+    return batch_layout(pl, B).total + (size_t)s.lanes * layout(pl, s.group, pitch, adj, s.ckpt).total;
+}
+
+// Non-temporal state streams when the solves in flight touch more than the
+// last-level cache per step (u^n, u^{n+1}, w, c^2 of each).
+// Schedules with groups in lanes need this many step workgroups in flight
+// (about 3/4 of what 256 CUs hold at 8 workgroups each); below it the batch
+// runs whole, with checkpoints.
+constexpr long MIN_INFLIGHT_TILES = 1536;
+// Lanes of groups in flight (measured best at 801^2: 3 lanes of 5 solves).
+constexpr int MAX_LANES = 3;
+
+bool use_nt(const sem_plan* pl, long inflight) {
+    const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
+    return (size_t)inflight * 4 * pl->sb * Ts > LLC_BYTES;
+}
+
 template <typename T>
 int fill(T* p, T v, long n, hipStream_t st) {
     if (n <= 0) return SEM_OK;
@@ -383,8 +461,11 @@ int fill(T* p, T v, long n, hipStream_t st) {
     return SEM_OK;
 }
 
+// Step arguments of the solves of `bt`: c2p holds the padded squared
+// velocities indexed by bt->model_of (or by solve), rset the device
+// receiver-set id of each solve.
 template <typename T>
-StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L) {
+StepArgs<T, 4> base_args(const sem_plan* pl, const sem_batch* bt, const void* c2p, const int* rset) {
     StepArgs<T, 4> a;
     std::memset(&a, 0, sizeof(a));
     const int P = 4;
@@ -409,7 +490,7 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     a.total = pl->ntiles * bt->batch;
     a.nper = (a.total + NXCD - 1) / NXCD;
     a.sb = pl->sb;
-    a.c2 = (const T*)(ws + L.C2P);
+    a.c2 = (const T*)c2p;
     a.model_of = bt->model_of;
     // folded D and Dw^T for apply5 (M[a][b] = -M[4-a][4-b])
     auto fold = [&](auto M, Fold5<T>& f) {
@@ -434,7 +515,7 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     a.src_node = bt->src_node;
     a.wavelet = (const T*)bt->wavelet;
     a.wav_bs = bt->wavelet_stride;
-    a.rec_set_of = (const int*)(ws + L.RSET);
+    a.rec_set_of = rset;
     a.rec_ptr = (const int*)pl->d_rec_ptr.p;
     a.recs = (const RecEntry<T, 4>*)pl->d_recs.p;
     a.seis = (T*)bt->seis;
@@ -446,17 +527,23 @@ StepArgs<T, 4> base_args(sem_plan* pl, const sem_batch* bt, char* ws, const Layo
     return a;
 }
 
-template <typename T, int MODE>
-int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
+template <typename T, int MODE, bool NTS>
+int launch_step_nt(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
     const unsigned grid = (unsigned)a.nper * NXCD;
     switch (pl->ez) {
-        case 15: step_kernel<T, 4, MODE, 15><<<grid, tile_threads(15), 0, st>>>(a); break;
-        case 7: step_kernel<T, 4, MODE, 7><<<grid, tile_threads(7), 0, st>>>(a); break;
-        case 3: step_kernel<T, 4, MODE, 3><<<grid, tile_threads(3), 0, st>>>(a); break;
+        case 15: step_kernel<T, 4, MODE, 15, NTS><<<grid, tile_threads(15), 0, st>>>(a); break;
+        case 7: step_kernel<T, 4, MODE, 7, NTS><<<grid, tile_threads(7), 0, st>>>(a); break;
+        case 3: step_kernel<T, 4, MODE, 3, NTS><<<grid, tile_threads(3), 0, st>>>(a); break;
         default: return fail(SEM_E_UNSUPPORTED, "tile height without a kernel instance");
     }
     LAUNCH_CHECK();
     return SEM_OK;
+}
+
+template <typename T, int MODE>
+int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
+    if constexpr (MODE == MODE_REC) return launch_step_nt<T, MODE, true>(pl, a, st);  // no state streams
+    return nts ? launch_step_nt<T, MODE, true>(pl, a, st) : launch_step_nt<T, MODE, false>(pl, a, st);
 }
 
 template <typename T>
@@ -486,7 +573,7 @@ int launch_corr(const sem_plan* pl, const CorrArgs<T, 4>& c, hipStream_t st) {
     return SEM_OK;
 }
 
-int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout& L) {
+int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, size_t need) {
     if (!bt || bt->batch <= 0) return fail(SEM_E_INVALID, "batch must be >= 1");
     if ((long)std::max(pl->ntiles, pl->ctiles) * bt->batch >= (1L << 31) - 64)
         return fail(SEM_E_INVALID, "batch too large for this mesh");
@@ -495,8 +582,8 @@ int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout
     if (bt->rec_pitch < 0) return fail(SEM_E_INVALID, "rec_pitch must be >= 0");
     if (bt->nmodel < 1 || bt->nmodel > bt->batch) return fail(SEM_E_INVALID, "nmodel must be in [1, batch]");
     if (!bt->model_of && bt->nmodel != bt->batch) return fail(SEM_E_INVALID, "model_of is NULL but nmodel != batch");
-    if (ws_bytes < L.total)
-        return fail(SEM_E_NOMEM, "workspace too small: need " + std::to_string(L.total) + " bytes");
+    if (ws_bytes < need)
+        return fail(SEM_E_NOMEM, "workspace too small: need " + std::to_string(need) + " bytes");
     for (int b = 0; b < bt->batch; ++b) {
         int s = bt->rec_set[b];
         if (s < 0 || s >= (int)pl->sets.size())
@@ -507,7 +594,7 @@ int check_batch(sem_plan* pl, const sem_batch* bt, size_t ws_bytes, const Layout
 }
 
 template <typename T>
-int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, hipStream_t st) {
+int stage_batch(sem_plan* pl, const sem_batch* bt, char* ws, const BatchLayout& L, hipStream_t st) {
     if (pl->rec_dirty) {
         int rc = upload_receivers<T>(pl);
         if (rc) return rc;
@@ -543,14 +630,49 @@ struct SnapSpec {
     void* out;
 };
 
-// Forward time loop.  Checkpoint mode: ping-pong U0/U1 with (u, w) mirrored
-// into checkpoint k after step k*S-1.  Full-history mode: u^n in SEG slot n.
+// The solves [g0, g0 + n) of a batch and the buffers they run in.
+struct Group {
+    int g0 = 0, n = 0;
+    sem_batch sub{};             // the group's solves (batch fields offset to solve g0)
+    const void* c2p = nullptr;   // padded c^2 as indexed by sub.model_of (or by solve)
+    const int* rset = nullptr;   // device receiver-set ids of the group's solves
+    const int* nrec = nullptr;   // device receiver counts of the group's solves
+    char* ws = nullptr;          // the group's workspace region (Layout offsets)
+    bool nts = true;             // non-temporal state streams
+};
+
 template <typename T>
-int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, bool adj, hipStream_t st,
+Group make_group(const sem_plan* pl, const sem_batch* bt, char* ws, const BatchLayout& BL, int g0, int n,
+                 char* region, bool nts) {
+    Group g;
+    g.g0 = g0;
+    g.n = n;
+    g.sub = *bt;
+    g.sub.batch = n;
+    g.sub.src_node = bt->src_node + g0;
+    g.sub.model_of = bt->model_of ? bt->model_of + g0 : nullptr;
+    g.sub.wavelet = (const T*)bt->wavelet + (long)g0 * bt->wavelet_stride;
+    g.sub.rec_set = bt->rec_set + g0;
+    g.sub.seis = bt->seis ? (void*)((T*)bt->seis + (long)g0 * pl->d.nt * bt->rec_pitch) : nullptr;
+    g.c2p = bt->model_of ? (const void*)(ws + BL.C2P) : (const void*)((const T*)(ws + BL.C2P) + (long)g0 * pl->sb);
+    g.rset = (const int*)(ws + BL.RSET) + g0;
+    g.nrec = (const int*)(ws + BL.NREC) + g0;
+    g.ws = region;
+    g.nts = nts;
+    return g;
+}
+
+// Forward time loop of a group.  Checkpoint mode: ping-pong U0/U1 with (u, w)
+// mirrored into checkpoint k after step k*S-1.  Full-history mode: u^n in SEG
+// slot n.
+template <typename T>
+int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStream_t st,
                  const SnapSpec* snap = nullptr) {
+    const sem_batch* bt = &g.sub;
+    char* ws = g.ws;
     const int B = bt->batch, nt = pl->d.nt;
     const long state = (long)B * pl->sb;
-    StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);
+    StepArgs<T, 4> a = base_args<T>(pl, bt, g.c2p, g.rset);
     T* U0 = (T*)(ws + L.U0);
     T* U1 = (T*)(ws + L.U1);
     T* W = (T*)(ws + L.W);
@@ -585,7 +707,7 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
             a.ck_u = (T*)(ws + L.CKU) + k * state;
             a.ck_w = (T*)(ws + L.CKW) + k * state;
         }
-        if ((rc = launch_step<T, MODE_FWD>(pl, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_FWD>(pl, a, st, g.nts))) return rc;
         last = a.u_out;
         if (snap && n % snap->every == 0) {
             const long per = (long)snap->nxr * snap->nzr;
@@ -600,7 +722,7 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
     if (post) {
         a.u_in = last;
         a.rec_t = nt - 1;
-        if ((rc = launch_step<T, MODE_REC>(pl, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_REC>(pl, a, st, g.nts))) return rc;
     }
     return SEM_OK;
 }
@@ -608,35 +730,34 @@ int forward_loop(sem_plan* pl, const sem_batch* bt, char* ws, const Layout& L, b
 template <typename T>
 int forward_impl(sem_plan* pl, const sem_batch* bt, void* wsv, size_t ws_bytes, hipStream_t st,
                  const SnapSpec* snap = nullptr) {
-    Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, false);
+    const int B = bt ? bt->batch : 1;
+    const BatchLayout BL = batch_layout(pl, B);
+    const Layout L = layout(pl, B, bt ? bt->rec_pitch : 0, false, 0);
     int rc;
-    if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
+    if ((rc = check_batch(pl, bt, ws_bytes, BL.total + L.total))) return rc;
     char* ws = (char*)wsv;
-    if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
+    if ((rc = stage_batch<T>(pl, bt, ws, BL, st))) return rc;
     if ((rc = fill<T>((T*)bt->seis, T(0), (long)bt->batch * pl->d.nt * bt->rec_pitch, st))) return rc;
-    return forward_loop<T>(pl, bt, ws, L, false, st, snap);
+    const Group g = make_group<T>(pl, bt, ws, BL, 0, B, ws + BL.total, use_nt(pl, B));
+    return forward_loop<T>(pl, g, L, false, st, snap);
 }
 
+// Forward, misfit, adjoint and gradient of one group, in order on one stream.
 template <typename T>
-int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, double inv_sigma2, double* misfit,
-                         double* grad_c, double* grad_c2, void* wsv, size_t ws_bytes, hipStream_t st) {
-    Layout L = layout(pl, bt ? bt->batch : 1, bt ? bt->rec_pitch : 0, true);
-    int rc;
-    if ((rc = check_batch(pl, bt, ws_bytes, L))) return rc;
-    if ((!obs && bt->rec_pitch > 0) || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
-    char* ws = (char*)wsv;
+int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, double inv_sigma2, double* misfit,
+              double* grad_c, double* grad_c2, hipStream_t st) {
+    const sem_batch* bt = &g.sub;
+    char* ws = g.ws;
     const int B = bt->batch, nt = pl->d.nt;
     const long state = (long)B * pl->sb;
-    if ((rc = stage_batch<T>(pl, bt, ws, L, st))) return rc;
-    if ((rc = fill<T>((T*)bt->seis, T(0), (long)B * nt * bt->rec_pitch, st))) return rc;
-    if ((rc = forward_loop<T>(pl, bt, ws, L, true, st))) return rc;
+    int rc;
+    if ((rc = forward_loop<T>(pl, g, L, true, st))) return rc;
 
     // residual and misfit
     T* RES = (T*)(ws + L.RES);
-    residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs, RES, misfit,
-                                          (const int*)(ws + L.NREC), 0, nt, bt->rec_pitch,
-                                          (long)nt * bt->rec_pitch, pl->d.win_begin, pl->d.win_end,
-                                          pl->d.win_decim);
+    residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs + (long)g.g0 * nt * bt->rec_pitch, RES,
+                                          misfit + g.g0, g.nrec, 0, nt, bt->rec_pitch, (long)nt * bt->rec_pitch,
+                                          pl->d.win_begin, pl->d.win_end, pl->d.win_decim);
     LAUNCH_CHECK();
 
     // adjoint sweep: per segment, recompute u into SEG, then chunks of up to C
@@ -655,8 +776,8 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     if ((rc = fill<T>(OM, T(0), state, st))) return rc;
     if ((rc = fill<double>(G, 0.0, (long)B * pl->gsb, st))) return rc;
 
-    StepArgs<T, 4> f = base_args<T>(pl, bt, ws, L);  // recompute steps
-    StepArgs<T, 4> a = base_args<T>(pl, bt, ws, L);  // adjoint steps
+    StepArgs<T, 4> f = base_args<T>(pl, bt, g.c2p, g.rset);  // recompute steps
+    StepArgs<T, 4> a = base_args<T>(pl, bt, g.c2p, g.rset);  // adjoint steps
     a.w = OM;
     a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
     a.src_node = nullptr;
@@ -683,7 +804,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
                 f.u_in = (m == 0) ? slot0 : SEG + (long)m * state;
                 f.u_out = SEG + (long)(m + 1) * state;
                 f.n = s0 + m;
-                if ((rc = launch_step<T, MODE_FWD>(pl, f, st))) return rc;
+                if ((rc = launch_step<T, MODE_FWD>(pl, f, st, g.nts))) return rc;
             }
         }
         const int useg0 = L.full_history ? 0 : s0;  // SEG slot j holds u^{useg0 + j}
@@ -694,7 +815,7 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
                 a.u_in = ring_slot(it);
                 a.u_out = ring_slot(it - 1);
                 a.res_t = it;
-                if ((rc = launch_step<T, MODE_ADJS>(pl, a, st))) return rc;
+                if ((rc = launch_step<T, MODE_ADJS>(pl, a, st, g.nts))) return rc;
             }
             c.uf = SEG + (long)(c0 + 1 - useg0) * state;  // u^{c0+1}
             c.c0 = c0;
@@ -704,13 +825,94 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
         }
     }
     if (grad_c || grad_c2) {
-        dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), B);
+        const long N = pl->N;
+        dim3 grid((unsigned)std::min<long>((N + 255) / 256, 2048), B);
         grad_finalize_kernel<T><<<grid, 256, 0, st>>>(G, pl->gsb, pl->sb, pl->gplane, pl->nexp, pl->d.nelem_x,
-                                                      pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch,
-                                                      (const T*)(ws + L.C2P), bt->model_of, pl->J * inv_sigma2,
-                                                      grad_c, grad_c2);
+                                                      pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch, (const T*)g.c2p,
+                                                      bt->model_of, pl->J * inv_sigma2,
+                                                      grad_c ? grad_c + (long)g.g0 * N : nullptr,
+                                                      grad_c2 ? grad_c2 + (long)g.g0 * N : nullptr);
         LAUNCH_CHECK();
     }
+    return SEM_OK;
+}
+
+// The plan's lane streams and events on device `dev` (created on first use).
+int ensure_lanes(sem_plan* pl, int lanes, int dev) {
+    if (pl->lane_dev != dev) {
+        if (!pl->lane_st.empty()) return fail(SEM_E_INVALID, "a plan's lanes serve the device they were created on");
+        pl->lane_dev = dev;
+    }
+    if (!pl->fork_ev) HIP_TRY(hipEventCreateWithFlags(&pl->fork_ev, hipEventDisableTiming));
+    while ((int)pl->lane_st.size() < lanes) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        pl->lane_st.push_back(s);
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        pl->lane_ev.push_back(e);
+    }
+    return SEM_OK;
+}
+
+template <typename T>
+int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, double inv_sigma2, double* misfit,
+                         double* grad_c, double* grad_c2, void* wsv, size_t ws_bytes, hipStream_t st) {
+    const int Bt = bt ? bt->batch : 1;
+    const Sched s = schedule(pl, Bt, true);
+    const BatchLayout BL = batch_layout(pl, Bt);
+    const Layout L = layout(pl, s.group, bt ? bt->rec_pitch : 0, true, s.ckpt);
+    int rc;
+    if ((rc = check_batch(pl, bt, ws_bytes, BL.total + (size_t)s.lanes * L.total))) return rc;
+    if ((!obs && bt->rec_pitch > 0) || !misfit) return fail(SEM_E_INVALID, "obs and misfit must be non-null");
+    char* ws = (char*)wsv;
+    if ((rc = stage_batch<T>(pl, bt, ws, BL, st))) return rc;
+    if ((rc = fill<T>((T*)bt->seis, T(0), (long)Bt * pl->d.nt * bt->rec_pitch, st))) return rc;
+    const bool nts = use_nt(pl, (long)s.lanes * s.group);
+    auto group = [&](int i, int lane) {
+        return make_group<T>(pl, bt, ws, BL, s.first(i, Bt), s.size(i, Bt), ws + BL.total + (size_t)lane * L.total,
+                             nts);
+    };
+    if (s.lanes == 1) {
+        for (int i = 0; i < s.ngroups; ++i)
+            if ((rc = run_group<T>(pl, group(i, 0), L, obs, inv_sigma2, misfit, grad_c, grad_c2, st))) return rc;
+        return SEM_OK;
+    }
+    // lanes: groups i = k, k + lanes, ... run on lane k's stream, issued by a
+    // host thread per lane; the lanes start after the staging on `st`, and
+    // `st` waits for every lane
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if ((rc = ensure_lanes(pl, s.lanes, dev))) return rc;
+    HIP_TRY(hipEventRecord(pl->fork_ev, st));
+    for (int k = 0; k < s.lanes; ++k) HIP_TRY(hipStreamWaitEvent(pl->lane_st[k], pl->fork_ev, 0));
+    std::vector<int> rcs(s.lanes, SEM_OK);
+    std::vector<std::string> msgs(s.lanes);
+    std::vector<std::thread> th;
+    for (int k = 0; k < s.lanes; ++k)
+        th.emplace_back([&, k] {
+            if (hipSetDevice(dev) != hipSuccess) {
+                rcs[k] = SEM_E_DEVICE;
+                msgs[k] = "hipSetDevice failed in a lane thread";
+                return;
+            }
+            for (int i = k; i < s.ngroups; i += s.lanes) {
+                const int r = run_group<T>(pl, group(i, k), L, obs, inv_sigma2, misfit, grad_c, grad_c2,
+                                           pl->lane_st[k]);
+                if (r) {
+                    rcs[k] = r;
+                    msgs[k] = g_err;
+                    return;
+                }
+            }
+        });
+    for (auto& t : th) t.join();
+    for (int k = 0; k < s.lanes; ++k) {
+        HIP_TRY(hipEventRecord(pl->lane_ev[k], pl->lane_st[k]));
+        HIP_TRY(hipStreamWaitEvent(st, pl->lane_ev[k], 0));
+    }
+    for (int k = 0; k < s.lanes; ++k)
+        if (rcs[k]) return fail(rcs[k], msgs[k]);
     return SEM_OK;
 }
 
@@ -791,6 +993,7 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     pl->d = *d;
     pl->d.win_end = wend;
     pl->d.win_decim = d->win_decim > 1 ? d->win_decim : 1;
+    pl->ckpt_desc = d->checkpoint_every;
     const int P = 4;
     pl->NX = d->nelem_x * P + 1;
     pl->NZ = d->nelem_z * P + 1;
@@ -933,8 +1136,51 @@ int sem_source_node(const sem_plan* pl, double x, double z, int* node) {
 
 int sem_workspace_bytes(const sem_plan* pl, int batch, int rec_pitch, int with_adjoint, size_t* bytes) {
     if (!pl || !bytes || batch < 1 || rec_pitch < 0) return fail(SEM_E_INVALID, "bad argument");
-    *bytes = layout(pl, batch, rec_pitch, with_adjoint != 0).total;
+    *bytes = workspace_total(pl, batch, rec_pitch, with_adjoint != 0);
     return SEM_OK;
+}
+
+int sem_plan_set_schedule(sem_plan* pl, int checkpoint_every, int group, int lanes) {
+    if (!pl || lanes < 0) return fail(SEM_E_INVALID, "sem_plan_set_schedule: bad argument");
+    pl->d.checkpoint_every = checkpoint_every;
+    pl->sched_group = group > 0 ? group : 0;
+    pl->sched_lanes = lanes > 0 ? lanes : 1;
+    return SEM_OK;
+}
+
+int sem_plan_get_schedule(const sem_plan* pl, int* checkpoint_every, int* group, int* lanes) {
+    if (!pl) return fail(SEM_E_INVALID, "null plan");
+    if (checkpoint_every) *checkpoint_every = pl->d.checkpoint_every;
+    if (group) *group = pl->sched_group;
+    if (lanes) *lanes = pl->sched_lanes;
+    return SEM_OK;
+}
+
+int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget_bytes) {
+    if (!pl || batch < 1 || rec_pitch < 0) return fail(SEM_E_INVALID, "sem_plan_schedule_auto: bad argument");
+    const size_t bl = batch_layout(pl, batch).total;
+    if (budget_bytes <= bl) return fail(SEM_E_NOMEM, "budget below the batch-level workspace");
+    const size_t avail = budget_bytes - bl;
+    const int nt = pl->d.nt;
+    auto bytes = [&](int G, int ck) { return layout(pl, G, rec_pitch, true, ck).total; };
+    // 1. the full forward history (no recompute pass): the whole batch in one
+    // group if it fits, else lanes of groups as large as fit, provided the
+    // solves in flight fill most of the GPU
+    const long F = std::min<long>(batch, (long)(avail / bytes(1, 0)));
+    if (F >= batch && bytes(batch, 0) <= avail) return sem_plan_set_schedule(pl, 0, 0, 1);
+    if (F >= 1 && F * pl->ntiles >= MIN_INFLIGHT_TILES) {
+        const int lanes = (int)std::min<long>(MAX_LANES, F);
+        int G = (int)(F / lanes);
+        while (G >= 1 && (size_t)lanes * bytes(G, 0) > avail) --G;
+        if (G >= 1) return sem_plan_set_schedule(pl, 0, G, lanes);
+    }
+    // 2. checkpoints every S steps, as many solves at once as fit, one lane
+    const int S = pl->ckpt_desc > 0 && pl->ckpt_desc < nt ? pl->ckpt_desc : std::min(100, nt);
+    if (bytes(batch, S) <= avail) return sem_plan_set_schedule(pl, S, 0, 1);
+    int G = (int)std::min<long>(batch, (long)(avail / bytes(1, S)));
+    while (G >= 1 && bytes(G, S) > avail) --G;
+    if (G < 1) return fail(SEM_E_NOMEM, "budget below one solve's workspace");
+    return sem_plan_set_schedule(pl, S, G, 1);
 }
 
 int sem_forward(sem_plan* pl, const sem_batch* b, void* ws, size_t ws_bytes, void* stream) {
@@ -1123,8 +1369,7 @@ static int slots_corr(sem_plan* pl, const sem_slot* slots, int nslots, int chunk
     sem_batch dummy;
     std::memset(&dummy, 0, sizeof(dummy));
     dummy.batch = nslots;
-    Layout L{};
-    const StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, L);
+    const StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, nullptr);
     CorrArgs<T, 4> c = corr_args<T>(pl, a, nslots);
     c.slots = slots;
     c.chunk = chunk;
@@ -1141,24 +1386,24 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
     std::memset(&dummy, 0, sizeof(dummy));
     dummy.batch = nslots;
     dummy.rec_pitch = rec_pitch;
-    Layout L{};
-    StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, L);
+    StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, nullptr);
     a.slots = slots;
     a.m = m;
     a.phase_mask = mask;
     a.post = pl->d.record_mode == SEM_RECORD_POST ? 1 : 0;
     a.rec_set_of = nullptr;
     a.c2 = nullptr;
-    if (final_record) return launch_step<T, MODE_REC>(pl, a, st);
+    if (final_record) return launch_step<T, MODE_REC>(pl, a, st, true);
+    const bool nts = use_nt(pl, nslots);
     const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_RECOMPUTE);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
     if (mask & fwd) {
         a.damp = 1;
-        if ((rc = launch_step<T, MODE_FWD>(pl, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_FWD>(pl, a, st, nts))) return rc;
     }
     if (mask & adj) {
         a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
-        if ((rc = launch_step<T, MODE_ADJS>(pl, a, st))) return rc;
+        if ((rc = launch_step<T, MODE_ADJS>(pl, a, st, nts))) return rc;
     }
     return SEM_OK;
 }

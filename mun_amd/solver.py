@@ -60,9 +60,12 @@ class Plan:
                  sponge_R: float = 1e-8, sponge_sides: Sequence[str] = ("xmin", "xmax", "zmin", "zmax"),
                  precision: str = "fp32", adjoint_mode: str = "reference", record: str = "post",
                  checkpoint_every: Optional[int] = 100, lagrange_eps: float = 1e-12, device=None,
-                 tile_rows: int = 0, misfit_window: Optional[Sequence[int]] = None):
+                 tile_rows: int = 0, misfit_window: Optional[Sequence[int]] = None, schedule: str = "whole"):
         if not torch.cuda.is_available():
             raise RuntimeError("mun_amd.Plan needs a ROCm GPU (torch.cuda.is_available() is False)")
+        if schedule not in ("whole", "auto"):
+            raise ValueError("schedule must be 'whole' or 'auto'")
+        self.schedule = schedule
         self.device = torch.device(device if device is not None else "cuda")
         self.lib = L.lib()
         code, self.dtype = PRECISIONS[precision]
@@ -216,6 +219,35 @@ class Plan:
                                              _stream_ptr(self.device)))
         return out
 
+    # ------------------------------------------------------------ schedule
+    def set_schedule(self, checkpoint_every: int, group: int = 0, lanes: int = 1):
+        """How forward_adjoint runs a batch: groups of at most `group` solves
+        (0: the whole batch), `lanes` groups at a time on streams of their own,
+        checkpoints every `checkpoint_every` steps (0: the full forward
+        history).  Results do not depend on it."""
+        L.check(self.lib.sem_plan_set_schedule(self.h, int(checkpoint_every), int(group), int(lanes)))
+
+    def get_schedule(self):
+        ck, g, ln = C.c_int(), C.c_int(), C.c_int()
+        L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))
+        return {"checkpoint_every": ck.value, "group": g.value, "lanes": ln.value}
+
+    def memory_budget(self, reserve_frac: float = 0.03, reserve_min: int = 4 << 30) -> int:
+        """Device bytes a workspace may take: free memory plus what this plan's
+        workspace and torch's cache already hold, less a reserve."""
+        free, total = torch.cuda.mem_get_info(self.device)
+        held = self._ws.numel() if self._ws is not None else 0
+        cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+        return max(0, free + held + cached - max(reserve_min, int(reserve_frac * total)))
+
+    def schedule_auto(self, batch: int, pitch: int, budget: Optional[int] = None):
+        """Pick the fastest schedule whose workspace fits `budget` bytes
+        (default: memory_budget()): the full forward history when it fits,
+        else lanes of full-history groups, else checkpoints."""
+        budget = self.memory_budget() if budget is None else int(budget)
+        L.check(self.lib.sem_plan_schedule_auto(self.h, int(batch), int(pitch), C.c_size_t(budget)))
+        return self.get_schedule()
+
     # ------------------------------------------------------------ solves
     def workspace(self, batch: int, pitch: int, adjoint: bool) -> torch.Tensor:
         n = C.c_size_t()
@@ -333,6 +365,8 @@ class Plan:
         if sig.size > 1 and not np.all(sig == sig[0]):
             raise ValueError("one noise_std per batch")
         inv_s2 = 1.0 / (float(sig[0]) ** 2 + 1e-24)
+        if self.schedule == "auto":
+            self.schedule_auto(B, pitch)
         ws = self.workspace(B, pitch, True)
         bt = self._batch(c2, wavelet, src, rec_sets, model_of, seis, pitch)
         L.check(self.lib.sem_forward_adjoint(self.h, C.byref(bt), _ptr(obs), inv_s2, _ptr(misfit), _ptr(gc),

@@ -1,0 +1,95 @@
+"""Schedules of sem_forward_adjoint: a batch run whole, with checkpoints or
+the full history, or split into groups on several lanes (streams of the
+plan's own) gives the same bits for every solve; the automatic planner picks
+a schedule that fits the memory budget it is given."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tests.gpu_helpers import gpu_setup, make_plan, rel_l2, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prob_sched():
+    prob, disc, coef, c = small_problem(nt=260)
+    seis, _ = O.forward(disc, c, record="post", history=None)
+    return prob, disc, coef, c, seis
+
+
+def _batch(prob, disc, coef, c, plan, nsolve=7, shared_models=True):
+    """nsolve solves over 4 models (model_of) or one model per solve, two
+    source positions."""
+    rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
+    rng = np.random.default_rng(5)
+    nm = 4 if shared_models else nsolve
+    models = [c] + [O.tensor_bspline_velocity(disc.mesh, coef + 70.0 * rng.standard_normal(coef.shape))[0]
+                    for _ in range(nm - 1)]
+    C2 = torch.as_tensor(np.stack(models) ** 2, dtype=plan.dtype, device=plan.device)
+    src2 = disc.mesh.find_closest_node(110.0, 55.0)
+    srcs = [src2 if k % 3 == 2 else src for k in range(nsolve)]  # solve 0: the fixture's model and source
+    model_of = [k % nm for k in range(nsolve)] if shared_models else None
+    return C2, w, srcs, model_of, rs
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("shared_models", [True, False])
+def test_every_schedule_gives_the_same_bits(prob_sched, precision, shared_models):
+    prob, disc, coef, c, seis = prob_sched
+    print(f"[schedules] {precision} shared_models={shared_models}", flush=True)
+    plan = make_plan(prob, precision, adjoint_mode="exact", checkpoint_every=50)
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan, shared_models=shared_models)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    # (checkpoint_every, group, lanes): whole batch with checkpoints, whole with
+    # the full history, groups on 1-3 lanes with either policy
+    schedules = [(50, 0, 1), (0, 0, 1), (0, 2, 3), (0, 3, 1), (37, 3, 2), (0, 1, 3), (50, 2, 2)]
+    outs = []
+    for sch in schedules:
+        plan.set_schedule(*sch)
+        assert plan.get_schedule() == dict(zip(("checkpoint_every", "group", "lanes"), sch))
+        o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        torch.cuda.synchronize()
+        outs.append({k: o[k].cpu() for k in ("seis", "misfit", "grad_c")})
+    for sch, o in zip(schedules[1:], outs[1:]):
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(o[k], outs[0][k]), (sch, k)
+    # and solve 0 (the fixture's model) still matches the oracle
+    ref = O.forward_adjoint(disc, c, seis * 0.9, 1e-9, mode="exact")
+    tol = 1e-9 if precision == "fp64" else 1e-4
+    assert rel_l2(outs[0]["grad_c"][0].numpy(), ref["dloglik_dc"]) < tol
+
+
+def test_schedule_auto_fits_the_budget(prob_sched):
+    prob, disc, coef, c, seis = prob_sched
+    print("[schedule auto]", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="reference", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
+    B, pitch = len(srcs), prob.rec_xz.shape[0]
+    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    ws_bytes = plan.workspace_bytes
+    # a large budget: the whole batch with its full history
+    assert plan.schedule_auto(B, pitch, 1 << 40) == {"checkpoint_every": 0, "group": 0, "lanes": 1}
+    full = ws_bytes(B, pitch, True)
+    ref = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+    # a budget below the full history of one solve: checkpoints (the plan's 40 steps)
+    plan.set_schedule(0, 1, 1)
+    one_full = ws_bytes(1, pitch, True)
+    sch = plan.schedule_auto(B, pitch, one_full - 1)
+    assert sch["checkpoint_every"] == 40
+    assert ws_bytes(B, pitch, True) <= one_full - 1
+    got = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+    torch.cuda.synchronize()
+    for k in ("seis", "misfit", "grad_c"):
+        assert torch.equal(got[k], ref[k]), k
+    # a budget between: groups with the full history, never above the budget
+    budget = full // 2
+    sch = plan.schedule_auto(B, pitch, budget)
+    assert ws_bytes(B, pitch, True) <= budget
+    assert 0 < sch["group"] < B or sch["checkpoint_every"] > 0
+    # below one checkpointed solve: a clean error
+    from mun_amd._lib import SemError
+    with pytest.raises(SemError):
+        plan.schedule_auto(B, pitch, 1024)

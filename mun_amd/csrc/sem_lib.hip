@@ -1164,9 +1164,12 @@ int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget
     const int nt = pl->d.nt;
     auto bytes = [&](int G, int ck) { return layout(pl, G, rec_pitch, true, ck).total; };
     // 1. the full forward history (no recompute pass): the whole batch in one
-    // group if it fits, else lanes of groups as large as fit, provided the
-    // solves in flight fill most of the GPU
-    const long F = std::min<long>(batch, (long)(avail / bytes(1, 0)));
+    // group if it fits in memory and its step working set in 3/4 of the
+    // last-level cache, else lanes of groups that do, provided the solves in
+    // flight fill most of the GPU
+    const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
+    const long f_llc = std::max<long>(1, (long)(LLC_BYTES * 3 / 4 / (4 * pl->sb * Ts)));
+    const long F = std::min<long>(std::min<long>(batch, f_llc), (long)(avail / bytes(1, 0)));
     if (F >= batch && bytes(batch, 0) <= avail) return sem_plan_set_schedule(pl, 0, 0, 1);
     if (F >= 1 && F * pl->ntiles >= MIN_INFLIGHT_TILES) {
         const int lanes = (int)std::min<long>(MAX_LANES, F);

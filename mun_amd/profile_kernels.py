@@ -2,6 +2,7 @@
 
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
     python -m mun_amd.profile_kernels --stream-ceiling
+    python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -21,6 +22,53 @@ from . import workloads as W
 from .batch import BatchSolver
 
 
+def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024):
+    """Bytes per forward-step launch from two rocprofv3 counter passes
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for the calibration copy (the runtime's copyBuffer kernel
+    moving copy_mb MiB), against the step kernel's 20 algorithmic bytes per
+    node (12 read, 8 written)."""
+    import csv
+    import statistics
+
+    def per_kernel(path, counter):
+        vals = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            vals.setdefault((name, int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
+        return vals
+
+    copy_bytes = copy_mb << 20
+    out = {"batch": batch, "nodes": nodes}
+    for counter, path, alg in (("FETCH_SIZE", fetch_csv, 12), ("WRITE_SIZE", write_csv, 8)):
+        vals = per_kernel(path, counter)
+        # calibration: the largest copyBuffer dispatches (the copy_mb MiB copies)
+        copies = [v for (n, g), vs in vals.items() if "copyBuffer" in n for v in vs]
+        cal_kib = statistics.median(sorted(copies)[-3:])
+        factor = copy_bytes / (cal_kib * 1024.0)
+        key = counter.split("_")[0].lower()
+        out[f"{key}_factor"] = round(factor, 4)
+        for (name, grid), vs in sorted(vals.items()):
+            if not name.startswith("sem::step_kernel") or ", 2, " in name:
+                continue
+            mode = "forward" if ", 0, " in name else "adjoint"
+            rec = out.setdefault(mode, {"kernel": name, "grid": grid, "dispatches": len(vs)})
+            rec[f"{key}_kib_raw_median"] = statistics.median(vs)
+            rec[f"{key}_bytes"] = statistics.median(vs) * 1024.0 * factor
+            rec[f"{key}_algorithmic_bytes"] = alg * nodes * batch
+    for mode in ("forward", "adjoint"):
+        rec = out.get(mode)
+        if rec and "fetch_bytes" in rec and "write_bytes" in rec:
+            rec["bytes_per_launch"] = int(rec["fetch_bytes"] + rec["write_bytes"])
+            rec["algorithmic_bytes_per_launch"] = 20 * nodes * batch
+            rec["ratio"] = round(rec["bytes_per_launch"] / rec["algorithmic_bytes_per_launch"], 4)
+    if "forward" in out:
+        out["bytes_per_launch"] = out["forward"].get("bytes_per_launch")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", type=int, default=256)
@@ -30,7 +78,19 @@ def main(argv=None):
     ap.add_argument("--tile-rows", type=int, default=15, help="step-kernel tile height (15, 7 or 3)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="run the HBM streaming-ceiling probe (mun_amd/stream_ceiling) instead")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarize two counter passes of this driver (no GPU work)")
+    ap.add_argument("--out", help="JSON file for --summarize-pmc")
     a = ap.parse_args(argv)
+    if a.summarize_pmc:
+        import json
+        res = summarize_pmc(*a.summarize_pmc, batch=a.batch, copy_mb=a.copy_mb)
+        text = json.dumps(res, indent=1)
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(text + "\n")
+        print(text)
+        return 0
     if a.stream_ceiling:
         import os
         import subprocess

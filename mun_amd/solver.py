@@ -219,6 +219,12 @@ class Plan:
                                              _stream_ptr(self.device)))
         return out
 
+    def layout(self):
+        """(pitch, elements per solve) of the padded nodal arrays."""
+        pitch, elems = C.c_int(), C.c_long()
+        L.check(self.lib.sem_plan_layout(self.h, C.byref(pitch), C.byref(elems)))
+        return pitch.value, elems.value
+
     # ------------------------------------------------------------ schedule
     def set_schedule(self, checkpoint_every: int, group: int = 0, lanes: int = 1):
         """How forward_adjoint runs a batch: groups of at most `group` solves
@@ -232,7 +238,7 @@ class Plan:
         L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))
         return {"checkpoint_every": ck.value, "group": g.value, "lanes": ln.value}
 
-    def memory_budget(self, reserve_frac: float = 0.03, reserve_min: int = 4 << 30) -> int:
+    def memory_budget(self, reserve_frac: float = 0.01, reserve_min: int = 3 << 30) -> int:
         """Device bytes a workspace may take: free memory plus what this plan's
         workspace and torch's cache already hold, less a reserve."""
         free, total = torch.cuda.mem_get_info(self.device)

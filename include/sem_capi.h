@@ -266,6 +266,9 @@ int sem_curve_project_ex(sem_plan* plan, const sem_curve_desc* cd, int nmodel, c
 #define SEM_SLOT_FORWARD 1    /* u0/u1 ping-pong (by step parity), w increment       */
 #define SEM_SLOT_RECOMPUTE 2  /* u0 = checkpoint u of the segment, w = its w, seg    */
 #define SEM_SLOT_ADJOINT 3    /* lambda in the ring, w = omega                        */
+#define SEM_SLOT_FORWARD_HIST 4 /* forward into the full history: u^{n0+m} in seg slot m
+                                   (seg = history + n0 * seg_stride), w increment; the
+                                   adjoint then reads u from the history (no recompute) */
 
 typedef struct {
     int phase;             /* SEM_SLOT_*                                              */
@@ -281,7 +284,8 @@ typedef struct {
     void* seis;            /* [nt][rec_pitch]                                         */
     void* ck_u;            /* forward: (u, w) at the end of the segment go here, or NULL */
     void* ck_w;
-    void* seg;             /* recompute: slots 1..len of u;  adjoint: u^{it+1} at slot it-n0+1 */
+    void* seg;             /* recompute: slots 1..len of u;  forward_hist: slots 0..len;
+                              adjoint: u^{it+1} at slot it-n0+1                        */
     long seg_stride;       /* elements between segment slots                          */
     const void* resid;     /* adjoint source [nt][rec_pitch]                          */
     double* G;             /* gradient accumulator, sem_plan_gradient_elems() doubles */
@@ -312,7 +316,8 @@ int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int
 int sem_slots_correlate(sem_plan* plan, const sem_slot* slots_device, int nslots, int chunk, int chunk_len,
                         void* stream);
 /* Receiver samples of u (final post-update sample) for slots in phase_mask:
- * forward slots sample u[(n0+len)&1] into seis row n0+len-1. */
+ * forward slots sample u[(n0+len)&1] (forward_hist slots seg slot len) into
+ * seis row n0+len-1. */
 int sem_slots_record_final(sem_plan* plan, const sem_slot* slots_device, int nslots, unsigned phase_mask,
                            int rec_pitch, void* stream);
 

@@ -57,7 +57,7 @@ class SemBatch(C.Structure):
                 ("rec_pitch", C.c_int), ("seis", C.c_void_p)]
 
 
-SLOT_IDLE, SLOT_FORWARD, SLOT_RECOMPUTE, SLOT_ADJOINT = 0, 1, 2, 3
+SLOT_IDLE, SLOT_FORWARD, SLOT_RECOMPUTE, SLOT_ADJOINT, SLOT_FORWARD_HIST = 0, 1, 2, 3, 4
 
 
 class SemSlot(C.Structure):

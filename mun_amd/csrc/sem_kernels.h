@@ -198,14 +198,15 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     v.rec_t = v.res_t = -1;
     if (MODE == MODE_REC) {  // final post-update sample of a forward slot
         const int n = d.n0 + d.len;
-        v.u_in = gm((const T*)((n & 1) ? d.u1 : d.u0));
+        v.u_in = d.phase == SEM_SLOT_FORWARD_HIST ? gm((const T*)d.seg + (long)d.len * d.seg_stride)
+                                                  : gm((const T*)((n & 1) ? d.u1 : d.u0));
         v.u_out = nullptr;
         v.w = nullptr;
         v.rec_t = n - 1;
-        return d.phase == SEM_SLOT_FORWARD && d.record;
+        return (d.phase == SEM_SLOT_FORWARD || d.phase == SEM_SLOT_FORWARD_HIST) && d.record;
     }
     if (a.m >= d.len) return false;
-    if (d.phase == SEM_SLOT_FORWARD || d.phase == SEM_SLOT_RECOMPUTE) {
+    if (d.phase == SEM_SLOT_FORWARD || d.phase == SEM_SLOT_RECOMPUTE || d.phase == SEM_SLOT_FORWARD_HIST) {
         const int n = d.n0 + a.m;
         v.wav = gm((const T*)d.wavelet + n);
         v.src = d.src_node;
@@ -218,6 +219,11 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
                 v.ck_u = gm((T*)d.ck_u);
                 v.ck_w = gm((T*)d.ck_w);
             }
+        } else if (d.phase == SEM_SLOT_FORWARD_HIST) {
+            v.u_in = gm((const T*)d.seg + (long)a.m * d.seg_stride);
+            v.u_out = gm((T*)d.seg + (long)(a.m + 1) * d.seg_stride);
+            v.w = gm((T*)d.w);
+            v.rec_t = d.record ? (a.post ? n - 1 : n) : -1;
         } else {
             v.u_in = gm((a.m == 0) ? (const T*)d.u0 : (const T*)d.seg + (long)a.m * d.seg_stride);
             v.u_out = gm((T*)d.seg + (long)(a.m + 1) * d.seg_stride);
@@ -255,19 +261,33 @@ __device__ __forceinline__ void load4(const GMEM T* __restrict__ p, T (&v)[4]) {
     }
 }
 
+// p + k with the byte offset formed in 32 bits (one solve's padded nodal
+// array stays below 4 GiB, checked when a plan is created): the compiler then
+// addresses with the workgroup-uniform base in SGPRs plus a 32-bit VGPR offset
+// instead of a 64-bit address per lane and row.
+template <typename T>
+__device__ __forceinline__ const GMEM T* at(const GMEM T* p, unsigned k) {
+    return reinterpret_cast<const GMEM T*>(reinterpret_cast<const GMEM char*>(p) + k * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ GMEM T* at(GMEM T* p, unsigned k) {
+    return reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(p) + k * (unsigned)sizeof(T));
+}
+
 // An element's 5 x 5 block in two halves.  fetch_block issues one 16-byte load
 // per row for the first four nodes and, only in lanes whose right neighbour
 // element is not held by lane + 1, a load of the fifth; finish_block takes the
 // fifth node of the other lanes from lane + 1, whose first node it is (a
 // cross-lane read instead of a memory instruction).  finish_block must run in
 // converged control flow.
+// (The block's first node is at element offset `off` from the uniform base p.)
 template <typename T>
-__device__ __forceinline__ void fetch_block(const GMEM T* __restrict__ p, int pitch, bool own_fifth,
-                                            T (&v)[5][5]) {
+__device__ __forceinline__ void fetch_block(const GMEM T* __restrict__ p, unsigned off, unsigned pitch,
+                                            bool own_fifth, T (&v)[5][5]) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
         T q[4];
-        load4(p + j * pitch, q);
+        load4(at(p, off + j * pitch), q);
         v[j][0] = q[0];
         v[j][1] = q[1];
         v[j][2] = q[2];
@@ -275,7 +295,7 @@ __device__ __forceinline__ void fetch_block(const GMEM T* __restrict__ p, int pi
     }
     if (own_fifth) {
 #pragma unroll
-        for (int j = 0; j < 5; ++j) v[j][4] = p[j * pitch + 4];
+        for (int j = 0; j < 5; ++j) v[j][4] = *at(p, off + j * pitch + 4);
     }
 }
 
@@ -434,8 +454,8 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
             // lane t + 1 holds the right neighbour (same tile row and wave, inside the domain)
             const bool from_right = exl < EX - 1 && (t & 63) != 63 && ex + 1 < a.nex;
             T c2[NP][NP];
-            fetch_block(V.u_in + base_c, pitch, valid && !from_right, u);
-            fetch_block(V.c2 + base_c, pitch, valid && !from_right, c2);
+            fetch_block(V.u_in, (unsigned)base_c, (unsigned)pitch, valid && !from_right, u);
+            fetch_block(V.c2, (unsigned)base_c, (unsigned)pitch, valid && !from_right, c2);
             finish_block(from_right, u);
             finish_block(from_right, c2);
             // the element math runs in every lane (branch-free: fewer live
@@ -451,11 +471,11 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                     }
             }
             if (owned) {
-                const GMEM T* __restrict__ wq = V.w + base;
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    if constexpr (NTS) load4nt(wq + j * pitch, wv[j]);
-                    else load4(wq + j * pitch, wv[j]);
+                    const GMEM T* wq = at((const GMEM T*)V.w, (unsigned)(base + j * pitch));
+                    if constexpr (NTS) load4nt(wq, wv[j]);
+                    else load4(wq, wv[j]);
                 }
                 if (a.damp) {
                     load4(gm(a.Dx) + P * ex, dxv);
@@ -570,17 +590,18 @@ __global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T
                 wn[i] = update(j, i, wv[j][i]);
                 un[i] = r[j][i];
             }
+            const unsigned o = (unsigned)(base + j * pitch);
             if constexpr (NTS) {
-                store4nt(wp + j * pitch, wn);
-                store4nt(uo + j * pitch, un);
+                store4nt(at(V.w, o), wn);
+                store4nt(at(V.u_out, o), un);
             } else {
-                store4(wp + j * pitch, wn);
-                store4(uo + j * pitch, un);
+                store4(at(V.w, o), wn);
+                store4(at(V.u_out, o), un);
             }
             if constexpr (MODE == MODE_FWD) {
                 if (V.ck_u) {
-                    store4(V.ck_w + base + j * pitch, wn);
-                    store4(V.ck_u + base + j * pitch, un);
+                    store4(at(V.ck_w, o), wn);
+                    store4(at(V.ck_u, o), un);
                 }
             }
         }
@@ -759,10 +780,10 @@ __global__ __launch_bounds__(CORR_W) void corr_kernel(const CorrArgs<T, P> a) {
     T lc[NP][NP], uc[NP][NP];
     auto fetch = [&](int s, T (&l)[NP][NP], T (&u)[NP][NP]) {
         const int it = c0 + s;
-        const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep) + base;
-        const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep) + base;
-        fetch_block(lp, pitch, !from_right, l);
-        fetch_block(up, pitch, !from_right, u);
+        const GMEM T* __restrict__ up = gm(ufb + (long)s * ustep);
+        const GMEM T* __restrict__ lp = gm(ringb + (long)((it + 1) % ring_slots) * rstep);
+        fetch_block(lp, (unsigned)base, (unsigned)pitch, !from_right, l);
+        fetch_block(up, (unsigned)base, (unsigned)pitch, !from_right, u);
     };
     for (int s = clen - 1; s >= 0; --s) {
         fetch(s, lc, uc);

@@ -972,10 +972,11 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     *out = nullptr;
     if (d->order != 4) return fail(SEM_E_UNSUPPORTED, "only polynomial order 4 has kernels");
     if (d->nelem_x < 1 || d->nelem_z < 1) return fail(SEM_E_INVALID, "nelem_x/nelem_z must be >= 1");
-    // kernels address nodes of one solve's padded arrays with 32-bit offsets
+    // kernels address one solve's padded arrays with 32-bit node and byte offsets
+    const long elem_bytes = d->precision == SEM_FP64 ? 8 : 4;
     if (d->nelem_x > (1 << 20) || d->nelem_z > (1 << 20) ||
-        ((long)d->nelem_z * 4 + 1) * (((long)d->nelem_x * 4 + 1 + 31) / 32 * 32) >= (1L << 31) - 64)
-        return fail(SEM_E_INVALID, "mesh too large: one padded nodal array must hold fewer than 2^31 nodes");
+        ((long)d->nelem_z * 4 + 1) * (((long)d->nelem_x * 4 + 1 + 31) / 32 * 32) * elem_bytes >= (1L << 32) - 256)
+        return fail(SEM_E_INVALID, "mesh too large: one padded nodal array must stay below 4 GiB");
     if (!(d->xmax > d->xmin) || !(d->zmax > d->zmin)) return fail(SEM_E_INVALID, "empty domain");
     if (!(d->dt > 0) || d->nt < 1) return fail(SEM_E_INVALID, "dt must be > 0 and nt >= 1");
     if (d->precision != SEM_FP32 && d->precision != SEM_FP64) return fail(SEM_E_INVALID, "bad precision");
@@ -1398,7 +1399,7 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
     a.c2 = nullptr;
     if (final_record) return launch_step<T, MODE_REC>(pl, a, st, true);
     const bool nts = use_nt(pl, nslots);
-    const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_RECOMPUTE);
+    const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_RECOMPUTE) | (1u << SEM_SLOT_FORWARD_HIST);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
     if (mask & fwd) {
         a.damp = 1;

@@ -4,11 +4,18 @@ Requests (mesh size class, step count, velocity B-spline nodes, source,
 receivers, observations) arrive over time.  Each size class is a *lane*: one
 plan, one HIP stream and a device-resident slot table (``sem_slot``, see
 include/sem_capi.h).  A solve occupies one slot from admission to completion
-and walks through rounds of S steps:
+and walks through rounds of S steps.  When its full forward history fits the
+HBM budget at admission it keeps it (no recompute):
+
+    FORWARD_HIST seg 0 .. K-1  ->  (final sample, residual, misfit)
+    -> ADJOINT seg K-1 -> ... -> ADJOINT seg 0
+    -> (gradient, projection onto the request's B-spline nodes) -> done
+
+otherwise it keeps checkpoints of (u, w) every S steps:
 
     FORWARD seg 0 .. K-1  ->  (final sample, residual, misfit)
     -> RECOMPUTE seg K-1 -> ADJOINT seg K-1 -> ... -> RECOMPUTE seg 0 -> ADJOINT seg 0
-    -> (gradient, projection onto the request's B-spline nodes) -> done
+    -> (gradient, projection) -> done
 
 Every round advances *all* occupied slots of a lane together: per step one
 forward-kernel launch covers the slots in FORWARD/RECOMPUTE and one
@@ -89,14 +96,17 @@ def corr_chunk(S: int) -> int:
     return -(-S // n)
 
 
-def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad_elems=None):
-    """HBM bytes one solve holds while resident: c2, u0, u1, w, 2K checkpoint
-    states, S+1 segment states, the adjoint ring and omega, the FP64 gradient
-    accumulator (grad_elems doubles; default: one nodal array), traces."""
+def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad_elems=None,
+                      full_history=False):
+    """HBM bytes one solve holds while resident: c2, w, either u0, u1, 2K
+    checkpoint states and S+1 segment states or the nt+1 states of the full
+    history, the adjoint ring and omega, the FP64 gradient accumulator
+    (grad_elems doubles; default: one nodal array), traces."""
     E = pitch_elems
     K = max(1, math.ceil(nt / S))
     GE = E if grad_elems is None else grad_elems
-    nodal = E * elem_bytes * (1 + 3 + 2 * K + (S + 1) + (corr_chunk(S) + 1) + 1) + GE * 8
+    states = (nt + 1) if full_history else (2 + 2 * K + (S + 1))
+    nodal = E * elem_bytes * (1 + 1 + states + (corr_chunk(S) + 1) + 1) + GE * 8
     traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
     return nodal + traces
 
@@ -114,6 +124,7 @@ class _Active:
     rec_set: int = 0
     nrec: int = 0
     src_node: int = 0
+    hist: bool = False            # full forward history (no recompute)
 
 
 class Lane:
@@ -181,8 +192,11 @@ class Lane:
         d.src_node, d.rec_set, d.record = a.src_node, a.rec_set, 1
         d.c2, d.wavelet = b["c2p"].data_ptr(), b["wav"].data_ptr()
         d.seis, d.resid, d.G = b["seis"].data_ptr(), b["res"].data_ptr(), b["G"].data_ptr()
-        d.seg, d.seg_stride = b["seg"].data_ptr(), self.E
-        if a.phase == L.SLOT_FORWARD:
+        d.seg_stride = self.E
+        d.seg = b["hist"][s0].data_ptr() if a.hist else b["seg"].data_ptr()
+        if a.phase == L.SLOT_FORWARD_HIST:
+            d.w = b["w"].data_ptr()
+        elif a.phase == L.SLOT_FORWARD:
             d.u0, d.u1, d.w = b["u0"].data_ptr(), b["u1"].data_ptr(), b["w"].data_ptr()
             if a.k + 1 < a.K:
                 d.ck_u, d.ck_w = b["cku"][a.k + 1].data_ptr(), b["ckw"][a.k + 1].data_ptr()
@@ -203,7 +217,7 @@ class Lane:
     def _launch_round(self):
         lib, h = self.plan.lib, self.plan.h
         sp = C.c_void_p(self.stream.cuda_stream)
-        mask = (1 << L.SLOT_FORWARD) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT)
+        mask = (1 << L.SLOT_FORWARD) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT) | (1 << L.SLOT_FORWARD_HIST)
         tab = C.c_void_p(self.table_dev.data_ptr())
         for m in range(self.S):
             L.check(lib.sem_slots_step(h, tab, self.C, m, mask, self.rec_pitch, sp))
@@ -236,7 +250,14 @@ class SEMServer:
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True):
+                 autotune: bool = True, history: str = "auto"):
+        """history: 'auto' admits a request with its full forward history
+        when that fits the budget (else with checkpoints every `segment`
+        steps); 'checkpoints' always uses checkpoints.  Results are the same
+        bits either way."""
+        if history not in ("auto", "checkpoints"):
+            raise ValueError("history must be 'auto' or 'checkpoints'")
+        self.history = history
         self.device = torch.device(device if device is not None else "cuda")
         self.budget = float(hbm_budget_bytes)
         self.S = int(segment)
@@ -270,7 +291,10 @@ class SEMServer:
         if slot is None:
             return False
         plan = lane.plan
-        fp = request_footprint(plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S, grad_elems=lane.GE)
+        fp_args = (plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S)
+        fp_full = request_footprint(*fp_args, grad_elems=lane.GE, full_history=True)
+        hist = self.history == "auto" and self.in_use + fp_full <= self.budget
+        fp = fp_full if hist else request_footprint(*fp_args, grad_elems=lane.GE)
         if self.in_use + fp > self.budget:
             return False
         dev, E, S = self.device, lane.E, self.S
@@ -280,10 +304,13 @@ class SEMServer:
             raise ValueError(f"request {req.rid}: {nrec} receivers exceed the lane pitch {lane.rec_pitch}")
         with torch.cuda.stream(lane.stream):
             z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
-            b = {"u0": z(E), "u1": z(E), "w": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E),
-                 "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
+            b = {"w": z(E), "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
                  "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
                  "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
+            if hist:
+                b["hist"] = z(req.nt + 1, E)  # u^0 .. u^nt; u^0 = 0
+            else:
+                b.update({"u0": z(E), "u1": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E)})
             if req.obs is not None:
                 b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float32, device=dev)
             w = source_wavelet(req.nt, req.dt, req.f0)
@@ -294,7 +321,8 @@ class SEMServer:
             L.check(plan.lib.sem_pad_nodal(plan.h, C.c_void_p(c2.data_ptr()), C.c_void_p(b["c2p"].data_ptr()), 1,
                                            C.c_void_p(lane.stream.cuda_stream)))
             b["c2_flat"] = c2
-        a = _Active(req=req, slot=slot, K=K, phase=L.SLOT_FORWARD, k=0, buf=b, footprint=fp, admitted=now)
+        a = _Active(req=req, slot=slot, K=K, phase=L.SLOT_FORWARD_HIST if hist else L.SLOT_FORWARD, k=0, buf=b,
+                    footprint=fp, admitted=now, hist=hist)
         lane.ensure_set_capacity()
         a.rec_set = plan.add_receivers(req.rec_xz)
         a.nrec = nrec
@@ -309,12 +337,14 @@ class SEMServer:
         plan = lane.plan
         lib, h = plan.lib, plan.h
         sp = C.c_void_p(lane.stream.cuda_stream)
-        finished_fwd = [a for a in lane.occupied() if a.phase == L.SLOT_FORWARD and a.k == a.K - 1]
+        fwd_phases = (L.SLOT_FORWARD, L.SLOT_FORWARD_HIST)
+        finished_fwd = [a for a in lane.occupied() if a.phase in fwd_phases and a.k == a.K - 1]
         with torch.cuda.stream(lane.stream):
             if finished_fwd:
                 # final post-update samples for the slots that just completed their forward pass
                 L.check(lib.sem_slots_record_final(h, C.c_void_p(lane.table_dev.data_ptr()), lane.C,
-                                                   1 << L.SLOT_FORWARD, lane.rec_pitch, sp))
+                                                   (1 << L.SLOT_FORWARD) | (1 << L.SLOT_FORWARD_HIST),
+                                                   lane.rec_pitch, sp))
                 for a in finished_fwd:
                     if a.k == a.K - 1:
                         b = a.buf
@@ -323,16 +353,16 @@ class SEMServer:
                                                  a.nrec, a.req.nt, lane.rec_pitch, sp))
             done = []
             for a in lane.occupied():
-                if a.phase == L.SLOT_FORWARD:
+                if a.phase in fwd_phases:
                     if a.k + 1 < a.K:
                         a.k += 1
                     else:
-                        a.phase, a.k = L.SLOT_RECOMPUTE, a.K - 1
+                        a.phase, a.k = (L.SLOT_ADJOINT if a.hist else L.SLOT_RECOMPUTE), a.K - 1
                 elif a.phase == L.SLOT_RECOMPUTE:
                     a.phase = L.SLOT_ADJOINT
                 elif a.phase == L.SLOT_ADJOINT:
                     if a.k > 0:
-                        a.phase, a.k = L.SLOT_RECOMPUTE, a.k - 1
+                        a.phase, a.k = (L.SLOT_ADJOINT if a.hist else L.SLOT_RECOMPUTE), a.k - 1
                     else:
                         done.append(a)
             for a in done:
@@ -409,7 +439,7 @@ def run_plain(requests: List[SolveRequest], device=None, adjoint_mode="reference
         if key not in plans:
             plans[key] = Plan(r.nelem_x, r.nelem_z, 0.0, r.xmax, 0.0, r.zmax, r.dt, r.nt, 4, r.sponge_width,
                               r.sponge_cmax, 1e-8, sides, precision="fp32", adjoint_mode=adjoint_mode,
-                              checkpoint_every=segment, device=dev)
+                              checkpoint_every=segment, device=dev, schedule="auto")
         p = plans[key]
         start = time.perf_counter() - t0
         rs = p.add_receivers(r.rec_xz)

@@ -11,8 +11,10 @@ libmun_sem.so.  Differences a caller can see are opt-in config keys:
                           core.py:1252-1263) | 'exact' (discrete adjoint)
   method.absorbing_sides  sides the sponge damps (default all four,
                           boundary.py:38-48)
-  method.checkpoint_every wavefield checkpoint interval (default 100;
-                          0 keeps the full history)
+  method.checkpoint_every wavefield checkpoint interval (0 keeps the full
+                          history); default: the full history when it fits
+                          in free device memory, else checkpoints every
+                          100 steps (results are the same bits either way)
   method.cfl_adjust       True (default): shrink dt like the reference
                           (utils.py:96-111); False: use time.dt as given
   time.nt                 explicit number of steps (default ceil(T/dt))
@@ -117,7 +119,9 @@ class SEMSimulation:
         self.precision = m.get("precision", "fp32")
         self.adjoint_mode = m.get("adjoint_mode", "reference")
         self.sides = tuple(m.get("absorbing_sides", SIDES_ALL))
-        self.checkpoint_every = int(m.get("checkpoint_every", 100))
+        ck = m.get("checkpoint_every")
+        self.checkpoint_every = 100 if ck is None else int(ck)
+        self.schedule = "auto" if ck is None else "whole"
         self.cfl_adjust = bool(m.get("cfl_adjust", True))
         self.adj_history_dtype = m.get("adj_history_dtype", "float64")
         # receivers: explicit positions or the circular array around the source
@@ -246,7 +250,7 @@ class SEMSimulation:
             p = Plan(self.nelem_x, self.nelem_z, self.xmin, self.xmax, self.zmin, self.zmax, self.dt, self.nt,
                      self.npol, self.pml_thickness, self.c_max, 1e-8, self.sides, precision=self.precision,
                      adjoint_mode=self.adjoint_mode, record=record, checkpoint_every=self.checkpoint_every,
-                     device=self.device, misfit_window=window)
+                     device=self.device, misfit_window=window, schedule=self.schedule)
             p.add_receivers(np.asarray(self.receiver_coords_sem, dtype=np.float64))
             self._plans[key] = p
         return self._plans[key]

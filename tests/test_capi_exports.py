@@ -38,8 +38,10 @@ def _desc(**kw):
     (dict(order=5), -2), (dict(nelem_x=0), -1), (dict(xmax=0.0), -1), (dict(dt=0.0), -1),
     (dict(nt=0), -1), (dict(precision=7), -1), (dict(adjoint_mode=3), -1), (dict(record_mode=2), -1),
     (dict(sponge_width=0.0), -1),
-    # 32-bit node offsets: 11585^2 elements pad to 46341 x 46368 > 2^31 nodes
-    (dict(nelem_x=11585, nelem_z=11585), -1), (dict(nelem_x=1 << 21), -1)])
+    # 32-bit byte offsets within one solve's padded array: 8192^2 elements pad to
+    # 32769 x 32800 nodes, 4.3 GB in FP32; 5800^2 to 23201 x 23232 nodes, 4.3 GB in FP64
+    (dict(nelem_x=11585, nelem_z=11585), -1), (dict(nelem_x=1 << 21), -1),
+    (dict(nelem_x=8192, nelem_z=8192), -1), (dict(nelem_x=5800, nelem_z=5800, precision=1), -1)])
 def test_plan_create_rejects_bad_descriptors(kw, code):
     lib = L.lib()
     h = C.c_void_p()

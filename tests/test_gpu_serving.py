@@ -86,3 +86,32 @@ def test_served_request_matches_oracle():
     assert rel_l2(res.seis, ref["seis"]) < 1e-5
     assert rel_l2(res.grad, gref) < 1e-4
     assert abs(res.loglik - ref["loglik"]) < 1e-4 * abs(ref["loglik"])
+
+
+def test_history_and_checkpoint_slots_give_the_same_bits():
+    """A request served with its full forward history (no recompute) gives the
+    same bits as with checkpoints, alone and when the budget admits some
+    requests one way and some the other."""
+    from mun_amd.serving import SEMServer, request_footprint
+    reqs = make_requests(6, seed=7)
+    print("[serve-history] checkpoints", flush=True)
+    kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420)
+    ck = SEMServer(hbm_budget_bytes=8e9, history="checkpoints", **kw).run(reqs)
+    print("[serve-history] full history", flush=True)
+    srv = SEMServer(hbm_budget_bytes=8e9, history="auto", **kw)
+    full = srv.run(reqs)
+    assert all(a is None for l in srv.lanes.values() for a in l.slots)
+    # a budget that holds one full-history request of the largest mesh plus
+    # checkpointed ones: admission mixes both kinds
+    big = max(reqs, key=lambda r: (r.nelem_x, r.nt))
+    lb = srv.lanes[big.lane_key]
+    fp_full = request_footprint(lb.plan.NX, lb.plan.NZ, lb.E, big.nt, lb.rec_pitch, 50, grad_elems=lb.GE,
+                                full_history=True)
+    print(f"[serve-history] mixed budget {1.5 * fp_full / 1e6:.1f} MB", flush=True)
+    mixed = SEMServer(hbm_budget_bytes=1.5 * fp_full, history="auto", **kw).run(reqs)
+    for r in reqs:
+        for other in (full, mixed):
+            a, b = ck[r.rid], other[r.rid]
+            assert a.loglik == b.loglik
+            np.testing.assert_array_equal(a.grad, b.grad)
+            np.testing.assert_array_equal(a.seis, b.seis)

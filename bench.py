@@ -233,13 +233,21 @@ def main():
     nts = inflight * 4 * plan.layout()[1] * 4 > (256 << 20)  # the library's choice (LLC_BYTES)
     log(f"[bench] step kernel on {inflight} solves: {t_launch * 1e6:.1f} us/launch, {achieved:.0f} GB/s; "
         f"whole solve {path_gbs:.0f} GB/s")
-    traffic = None
+    # bytes per launch across L2 from the committed counter runs: exact when
+    # they were taken at this launch size, else their measured ratio to the
+    # algorithmic bytes applied to this launch (noted in traffic_basis)
+    traffic, traffic_basis = None, None
     tp = os.path.join(ROOT, "profiles", "step_fwd_batch200_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
             tj = json.load(f)
+        fw = tj.get("forward", {})
         if tj.get("batch") == inflight:
-            traffic = tj.get("bytes_per_launch")
+            traffic, traffic_basis = tj.get("bytes_per_launch"), f"counters at {inflight} solves per launch"
+        elif fw.get("ratio"):
+            traffic = int(fw["ratio"] * bytes_launch)
+            traffic_basis = (f"counters at {tj.get('batch')} solves per launch: {fw['ratio']} x the algorithmic "
+                             f"bytes, applied to {inflight}")
 
     out = {"metric": "fwd+adjoint SEM solves/s (1 GPU, batched vs plain); p50/p99 solve latency",
            "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -251,6 +259,7 @@ def main():
                       "parallelism": "none (1 GPU)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "traffic_basis": traffic_basis,
                         "kernel": f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>",
                         "solves_per_launch": inflight, "launch_us": round(t_launch * 1e6, 1),
                         "algorithmic_bytes_per_launch": bytes_launch, "bytes_per_node": FWD_BYTES_PER_NODE,

@@ -93,3 +93,26 @@ def test_schedule_auto_fits_the_budget(prob_sched):
     from mun_amd._lib import SemError
     with pytest.raises(SemError):
         plan.schedule_auto(B, pitch, 1024)
+
+
+def test_lanes_at_the_headline_mesh():
+    """The benchmark's mesh (200 x 200 elements, 801^2 nodes): lanes of
+    full-history groups (as many tiles and solves in flight as the bench
+    runs) give the same bits as the whole batch with checkpoints."""
+    from mun_amd import workloads as W
+    from mun_amd.batch import BatchSolver
+    print("[schedules 801^2]", flush=True)
+    wl = W.batch200(nmodel=8, nt=60)
+    s = BatchSolver.from_workload(wl, precision="fp32", adjoint_mode="exact", tile_rows=15, schedule="whole")
+    obs = s.forward(torch.as_tensor(wl.true_coef[None], device="cuda"))[0]
+    coef = torch.as_tensor(wl.coef, device="cuda")
+    outs = []
+    for sch in ((20, 0, 1), (0, 3, 3), (0, 8, 1)):
+        s.plan.set_schedule(*sch)
+        ll, g, seis = s.loglik_grad(coef, obs, wl.noise_std)
+        torch.cuda.synchronize()
+        outs.append((ll.cpu(), g.cpu(), seis.cpu()))
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert torch.equal(a, b)
+    assert bool(torch.isfinite(outs[0][1]).all()) and float(outs[0][1].abs().max()) > 0

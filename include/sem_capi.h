@@ -147,6 +147,16 @@ int sem_plan_get_schedule(const sem_plan* plan, int* checkpoint_every, int* grou
  * at once as fit.  SEM_E_NOMEM if not even one solve fits. */
 int sem_plan_schedule_auto(sem_plan* plan, int batch, int rec_pitch, size_t budget_bytes);
 
+/* Memory hints of the step kernels' once-per-step streams (the increment and
+ * the new state): SEM_STREAM_AUTO (default) uses plain loads and stores when
+ * the solves in flight touch at most the 256 MB last-level cache per step,
+ * non-temporal ones otherwise; the other two force one kind (for measurement).
+ * Results do not depend on it.  No reference counterpart. */
+#define SEM_STREAM_AUTO 0
+#define SEM_STREAM_PLAIN 1
+#define SEM_STREAM_NONTEMPORAL 2
+int sem_plan_set_streaming(sem_plan* plan, int mode);
+
 /* Forward solve of a batch; replaces run() / _time_stepping_loop_torch
  * (core.py:334-435, 598-659).  Writes b->seis. */
 int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_bytes, void* stream);

@@ -242,6 +242,7 @@ struct sem_plan {
     // most sched_group solves (0: the whole batch), sched_lanes in flight
     int sched_group = 0, sched_lanes = 1;
     int ckpt_desc = 0;  // checkpoint_every as created (sem_plan_schedule_auto's fallback spacing)
+    int streaming = SEM_STREAM_AUTO;  // state-stream memory hints (sem_plan_set_streaming)
     // lane streams and events, created on first use on device lane_dev
     int lane_dev = -1;
     std::vector<hipStream_t> lane_st;
@@ -448,6 +449,7 @@ constexpr long MIN_INFLIGHT_TILES = 1536;
 constexpr int MAX_LANES = 3;
 
 bool use_nt(const sem_plan* pl, long inflight) {
+    if (pl->streaming != SEM_STREAM_AUTO) return pl->streaming == SEM_STREAM_NONTEMPORAL;
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
     return (size_t)inflight * 4 * pl->sb * Ts > LLC_BYTES;
 }
@@ -1146,6 +1148,13 @@ int sem_plan_set_schedule(sem_plan* pl, int checkpoint_every, int group, int lan
     pl->d.checkpoint_every = checkpoint_every;
     pl->sched_group = group > 0 ? group : 0;
     pl->sched_lanes = lanes > 0 ? lanes : 1;
+    return SEM_OK;
+}
+
+int sem_plan_set_streaming(sem_plan* pl, int mode) {
+    if (!pl || mode < SEM_STREAM_AUTO || mode > SEM_STREAM_NONTEMPORAL)
+        return fail(SEM_E_INVALID, "sem_plan_set_streaming: bad argument");
+    pl->streaming = mode;
     return SEM_OK;
 }
 

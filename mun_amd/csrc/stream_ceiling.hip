@@ -11,6 +11,12 @@
 //
 // Buffers of 1 GiB each (4 allocated), n4 = 2^26 16-byte elements per buffer,
 // one thread per element, grid = n4 / 256 exactly; every access is in bounds.
+//
+// Then a working-set sweep of the two mix kernels on prefixes of the same
+// buffers (2^20 .. 2^26 16-byte elements each, 16 MiB .. 1 GiB, four buffers
+// touched per launch), with u and u' swapped between launches as the step
+// kernels do: the bandwidth plain and non-temporal streams reach when the
+// working set fits the 256 MB last-level cache and when it does not.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -107,6 +113,31 @@ int main() {
     if (run("copy", 2 * B1, 0) || run("copy_nt", 2 * B1, 1) || run("read", B1, 2) ||
         run("mix3r2w_nt", 5 * B1, 3) || run("mix3r2w", 5 * B1, 4))
         return 1;
+    // working-set sweep: m4 16-byte elements per buffer, m4 a multiple of 256
+    std::printf("%-12s %10s %10s %10s %8s\n", "sweep", "MiB/buf", "set MiB", "us", "GB/s");
+    for (long m4 = 1L << 20; m4 <= n4; m4 <<= 1) {
+        const unsigned g = (unsigned)(m4 / bs);
+        const int reps = m4 <= (1L << 22) ? 200 : 20;
+        for (int nt = 0; nt < 2; ++nt) {
+            for (int r = -3; r < reps; ++r) {
+                if (r == 0) CK(hipEventRecord(e0));
+                vf4* U = (r & 1) ? D : (vf4*)a;
+                vf4* U2 = (r & 1) ? (vf4*)a : D;
+                if (nt) mix_nt_k<<<g, bs>>>(U, B, C, U2, m4);
+                else mix_k<<<g, bs>>>(U, B, C, U2, m4);
+                CK(hipGetLastError());
+            }
+            CK(hipEventRecord(e1));
+            CK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            CK(hipEventElapsedTime(&ms, e0, e1));
+            const double us = ms * 1e3 / reps;
+            const double mib = (double)m4 * 16.0 / (1 << 20);
+            std::printf("%-12s %10.0f %10.0f %10.2f %8.0f\n", nt ? "mix3r2w_nt" : "mix3r2w", mib, 4.0 * mib, us,
+                        5.0 * m4 * 16.0 / (us * 1e-6) / 1e9);
+            std::fflush(stdout);
+        }
+    }
     CK(hipDeviceSynchronize());
     std::printf("done\n");
     return 0;

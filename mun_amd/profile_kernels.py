@@ -3,6 +3,7 @@
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
     python -m mun_amd.profile_kernels --stream-ceiling
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
+    python -m mun_amd.profile_kernels --scan-streaming [--nt 200]
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -69,6 +70,37 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024):
     return out
 
 
+def scan_streaming(nt=200, batches=(6, 12, 18, 21, 32, 64, 128, 256), tile_rows=15):
+    """Forward step time per launch against solves per launch, with plain and
+    with non-temporal state streams (HIP events over nt launches on the
+    library's stream), on the 801 x 801-node workload mesh."""
+    dev = torch.device("cuda:0")
+    wl = W.batch200(nmodel=max(batches), nt=nt)
+    s = BatchSolver.from_workload(wl, tile_rows=tile_rows, precision="fp32", device=dev, schedule="whole")
+    _, c2 = s.velocity(torch.as_tensor(wl.coef, device=dev))
+    st = torch.cuda.current_stream(dev)
+    _, elems = s.plan.layout()
+    print(f"{'solves':>6} {'set MB':>8} {'hints':>12} {'us/launch':>10} {'GB/s (20 B/node)':>17}", flush=True)
+    rows = []
+    for B in batches:
+        src, model_of = s._expand(B)
+        for mode in ("plain", "nontemporal"):
+            s.plan.set_streaming(mode)
+            args = (c2[:B], s.wavelet, src, [s.rec_set] * B)
+            s.plan.forward(*args, model_of=model_of)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            s.plan.forward(*args, model_of=model_of)
+            e1.record(st)
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / nt
+            gbs = 20 * s.plan.N * B / us / 1e3
+            rows.append((B, mode, us, gbs))
+            print(f"{B:6d} {4 * 4 * elems * B / 1e6:8.0f} {mode:>12} {us:10.2f} {gbs:17.0f}", flush=True)
+    s.plan.set_streaming("auto")
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", type=int, default=256)
@@ -81,7 +113,12 @@ def main(argv=None):
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two counter passes of this driver (no GPU work)")
     ap.add_argument("--out", help="JSON file for --summarize-pmc")
+    ap.add_argument("--scan-streaming", action="store_true",
+                    help="time the forward step against solves per launch, plain vs non-temporal streams")
     a = ap.parse_args(argv)
+    if a.scan_streaming:
+        scan_streaming(nt=a.nt if a.nt != 40 else 200, tile_rows=a.tile_rows)
+        return 0
     if a.summarize_pmc:
         import json
         res = summarize_pmc(*a.summarize_pmc, batch=a.batch, copy_mb=a.copy_mb)

@@ -233,6 +233,13 @@ class Plan:
         history).  Results do not depend on it."""
         L.check(self.lib.sem_plan_set_schedule(self.h, int(checkpoint_every), int(group), int(lanes)))
 
+    def set_streaming(self, mode: str = "auto"):
+        """State-stream memory hints of the step kernels: 'auto' (plain when
+        the solves in flight fit the last-level cache, else non-temporal),
+        'plain' or 'nontemporal' (for measurement; results do not change)."""
+        code = {"auto": L.SEM_STREAM_AUTO, "plain": L.SEM_STREAM_PLAIN, "nontemporal": L.SEM_STREAM_NONTEMPORAL}[mode]
+        L.check(self.lib.sem_plan_set_streaming(self.h, code))
+
     def get_schedule(self):
         ck, g, ln = C.c_int(), C.c_int(), C.c_int()
         L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))

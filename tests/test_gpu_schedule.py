@@ -116,3 +116,22 @@ def test_lanes_at_the_headline_mesh():
         for a, b in zip(o, outs[0]):
             assert torch.equal(a, b)
     assert bool(torch.isfinite(outs[0][1]).all()) and float(outs[0][1].abs().max()) > 0
+
+
+def test_streaming_hints_give_the_same_bits(prob_sched):
+    """Plain and non-temporal state streams change only cache behaviour."""
+    prob, disc, coef, c, seis = prob_sched
+    print("[streaming hints]", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="exact", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    outs = []
+    for mode in ("plain", "nontemporal", "auto"):
+        plan.set_streaming(mode)
+        o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        torch.cuda.synchronize()
+        outs.append(o)
+    for o in outs[1:]:
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(o[k], outs[0][k]), k

@@ -139,12 +139,13 @@ int sem_workspace_bytes(const sem_plan* plan, int batch, int rec_pitch, int with
 int sem_plan_set_schedule(sem_plan* plan, int checkpoint_every, int group, int lanes);
 int sem_plan_get_schedule(const sem_plan* plan, int* checkpoint_every, int* group, int* lanes);
 /* Choose and set a schedule for `batch` solves whose workspace fits in
- * budget_bytes: the full forward history for the whole batch if it fits and
- * the solves' per-step working set (u^n, u^{n+1}, w, c^2) fits in 3/4 of the
- * GPU's 256 MB last-level cache, else up to 3 lanes of full-history groups
- * as large as both limits allow (when they keep the GPU busy), else
- * checkpoints (the plan's checkpoint_every, or 100 steps) for as many solves
- * at once as fit.  SEM_E_NOMEM if not even one solve fits. */
+ * budget_bytes: the full forward history for the whole batch, split into one
+ * group per lane (up to 3 lanes), if it fits and the solves' per-step working
+ * set (u^n, u^{n+1}, w, c^2) fits in 3/4 of the GPU's 256 MB last-level
+ * cache; else 3 lanes of full-history groups as large as both limits allow
+ * (when they keep the GPU busy); else checkpoints (the plan's
+ * checkpoint_every, or 100 steps) for as many solves at once as fit.
+ * SEM_E_NOMEM if not even one solve fits. */
 int sem_plan_schedule_auto(sem_plan* plan, int batch, int rec_pitch, size_t budget_bytes);
 
 /* Memory hints of the step kernels' once-per-step streams (the increment and

@@ -445,7 +445,10 @@ size_t workspace_total(const sem_plan* pl, int B, int pitch, bool adj) {
 // (about 3/4 of what 256 CUs hold at 8 workgroups each); below it the batch
 // runs whole, with checkpoints.
 constexpr long MIN_INFLIGHT_TILES = 1536;
-// Lanes of groups in flight (measured best at 801^2: 3 lanes of 5 solves).
+// Lanes of groups in flight: 3 lanes and the caller's stream use the 4
+// hardware queues HIP gives a process by default; a fourth lane shares a
+// queue and was slower (801^2: 3 x 6 solves best; 161^2: 3 x 86 solves 1772
+// solves/s, 4 x 64 1261).
 constexpr int MAX_LANES = 3;
 
 bool use_nt(const sem_plan* pl, long inflight) {
@@ -1180,7 +1183,14 @@ int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
     const long f_llc = std::max<long>(1, (long)(LLC_BYTES * 3 / 4 / (4 * pl->sb * Ts)));
     const long F = std::min<long>(std::min<long>(batch, f_llc), (long)(avail / bytes(1, 0)));
-    if (F >= batch && bytes(batch, 0) <= avail) return sem_plan_set_schedule(pl, 0, 0, 1);
+    if (F >= batch && bytes(batch, 0) <= avail) {
+        // one group per lane: the lanes overlap each other's launch ramps and
+        // tails (small launches are dispatch-bound)
+        const int lanes = std::min(MAX_LANES, batch);
+        const int G = (batch + lanes - 1) / lanes;
+        if (lanes > 1 && (size_t)lanes * bytes(G, 0) <= avail) return sem_plan_set_schedule(pl, 0, G, lanes);
+        return sem_plan_set_schedule(pl, 0, 0, 1);
+    }
     if (F >= 1 && F * pl->ntiles >= MIN_INFLIGHT_TILES) {
         const int lanes = (int)std::min<long>(MAX_LANES, F);
         int G = (int)(F / lanes);

@@ -70,8 +70,8 @@ def test_schedule_auto_fits_the_budget(prob_sched):
     B, pitch = len(srcs), prob.rec_xz.shape[0]
     obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
     ws_bytes = plan.workspace_bytes
-    # a large budget: the whole batch with its full history
-    assert plan.schedule_auto(B, pitch, 1 << 40) == {"checkpoint_every": 0, "group": 0, "lanes": 1}
+    # a large budget: the whole batch with its full history, one group per lane
+    assert plan.schedule_auto(B, pitch, 1 << 40) == {"checkpoint_every": 0, "group": -(-B // 3), "lanes": 3}
     full = ws_bytes(B, pitch, True)
     ref = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
     # a budget below the full history of one solve: checkpoints (the plan's 40 steps)

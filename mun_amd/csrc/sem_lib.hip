@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <new>
 #include <string>
 #include <thread>
 #include <utility>
@@ -34,6 +36,19 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                              \
             return fail(SEM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Function-try-block tail of every C entry point: no C++ exception (host
+// allocation, thread start) leaves the C ABI.
+#define ABI_CATCH                                                                  \
+    catch (const std::bad_alloc&) {                                                \
+        return fail(SEM_E_NOMEM, "host allocation failed");                        \
+    }                                                                              \
+    catch (const std::exception& e_) {                                             \
+        return fail(SEM_E_DEVICE, std::string("host error: ") + e_.what());        \
+    }                                                                              \
+    catch (...) {                                                                  \
+        return fail(SEM_E_DEVICE, "host error");                                   \
+    }
 
 #define LAUNCH_CHECK()                                                                      \
     do {                                                                                    \
@@ -893,24 +908,34 @@ int forward_adjoint_impl(sem_plan* pl, const sem_batch* bt, const void* obs, dou
     for (int k = 0; k < s.lanes; ++k) HIP_TRY(hipStreamWaitEvent(pl->lane_st[k], pl->fork_ev, 0));
     std::vector<int> rcs(s.lanes, SEM_OK);
     std::vector<std::string> msgs(s.lanes);
-    std::vector<std::thread> th;
-    for (int k = 0; k < s.lanes; ++k)
-        th.emplace_back([&, k] {
-            if (hipSetDevice(dev) != hipSuccess) {
-                rcs[k] = SEM_E_DEVICE;
-                msgs[k] = "hipSetDevice failed in a lane thread";
+    auto lane_work = [&](int k) {
+        if (hipSetDevice(dev) != hipSuccess) {
+            rcs[k] = SEM_E_DEVICE;
+            msgs[k] = "hipSetDevice failed in a lane thread";
+            return;
+        }
+        for (int i = k; i < s.ngroups; i += s.lanes) {
+            const int r = run_group<T>(pl, group(i, k), L, obs, inv_sigma2, misfit, grad_c, grad_c2,
+                                       pl->lane_st[k]);
+            if (r) {
+                rcs[k] = r;
+                msgs[k] = g_err;
                 return;
             }
-            for (int i = k; i < s.ngroups; i += s.lanes) {
-                const int r = run_group<T>(pl, group(i, k), L, obs, inv_sigma2, misfit, grad_c, grad_c2,
-                                           pl->lane_st[k]);
-                if (r) {
-                    rcs[k] = r;
-                    msgs[k] = g_err;
-                    return;
-                }
-            }
-        });
+        }
+    };
+    // lane 0 runs on the calling thread; a lane whose thread cannot be
+    // started runs there too, after it (no exception leaves the C ABI)
+    std::vector<std::thread> th;
+    std::vector<int> inline_lanes{0};
+    for (int k = 1; k < s.lanes; ++k) {
+        try {
+            th.emplace_back(lane_work, k);
+        } catch (...) {
+            inline_lanes.push_back(k);
+        }
+    }
+    for (int k : inline_lanes) lane_work(k);
     for (auto& t : th) t.join();
     for (int k = 0; k < s.lanes; ++k) {
         HIP_TRY(hipEventRecord(pl->lane_ev[k], pl->lane_st[k]));
@@ -972,7 +997,7 @@ extern "C" {
 
 const char* sem_last_error(void) { return g_err.c_str(); }
 
-int sem_plan_create(const sem_desc* d, sem_plan** out) {
+int sem_plan_create(const sem_desc* d, sem_plan** out) try {
     if (!d || !out) return fail(SEM_E_INVALID, "null argument");
     *out = nullptr;
     if (d->order != 4) return fail(SEM_E_UNSUPPORTED, "only polynomial order 4 has kernels");
@@ -1061,29 +1086,29 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) {
     }
     *out = pl;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_destroy(sem_plan* plan) {
+int sem_plan_destroy(sem_plan* plan) try {
     delete plan;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_dims(const sem_plan* pl, int* NX, int* NZ, long* N) {
+int sem_plan_dims(const sem_plan* pl, int* NX, int* NZ, long* N) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     if (NX) *NX = pl->NX;
     if (NZ) *NZ = pl->NZ;
     if (N) *N = pl->N;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_axes(const sem_plan* pl, double* x1d, double* z1d) {
+int sem_plan_axes(const sem_plan* pl, double* x1d, double* z1d) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     if (x1d) std::memcpy(x1d, pl->x1d.data(), sizeof(double) * pl->NX);
     if (z1d) std::memcpy(z1d, pl->z1d.data(), sizeof(double) * pl->NZ);
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_receiver_set_add(sem_plan* pl, int nrec, const double* xz, int* set_id) {
+int sem_receiver_set_add(sem_plan* pl, int nrec, const double* xz, int* set_id) try {
     if (!pl || nrec < 0 || (nrec > 0 && !xz) || !set_id) return fail(SEM_E_INVALID, "bad receiver set");
     std::vector<RecHost> s;
     const auto& d = pl->d;
@@ -1113,15 +1138,15 @@ int sem_receiver_set_add(sem_plan* pl, int nrec, const double* xz, int* set_id) 
     pl->rec_dirty = true;
     *set_id = (int)pl->sets.size() - 1;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_receiver_set_size(const sem_plan* pl, int set_id, int* nrec) {
+int sem_receiver_set_size(const sem_plan* pl, int set_id, int* nrec) try {
     if (!pl || !nrec || set_id < 0 || set_id >= (int)pl->sets.size()) return fail(SEM_E_INVALID, "bad set id");
     *nrec = pl->set_size[set_id];
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_source_node(const sem_plan* pl, double x, double z, int* node) {
+int sem_source_node(const sem_plan* pl, double x, double z, int* node) try {
     if (!pl || !node) return fail(SEM_E_INVALID, "null argument");
     // nearest node; on a tensor grid the squared distance separates per axis
     auto nearest = [](const std::vector<double>& v, double q) {
@@ -1138,38 +1163,38 @@ int sem_source_node(const sem_plan* pl, double x, double z, int* node) {
     };
     *node = nearest(pl->z1d, z) * pl->NX + nearest(pl->x1d, x);
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_workspace_bytes(const sem_plan* pl, int batch, int rec_pitch, int with_adjoint, size_t* bytes) {
+int sem_workspace_bytes(const sem_plan* pl, int batch, int rec_pitch, int with_adjoint, size_t* bytes) try {
     if (!pl || !bytes || batch < 1 || rec_pitch < 0) return fail(SEM_E_INVALID, "bad argument");
     *bytes = workspace_total(pl, batch, rec_pitch, with_adjoint != 0);
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_set_schedule(sem_plan* pl, int checkpoint_every, int group, int lanes) {
+int sem_plan_set_schedule(sem_plan* pl, int checkpoint_every, int group, int lanes) try {
     if (!pl || lanes < 0) return fail(SEM_E_INVALID, "sem_plan_set_schedule: bad argument");
     pl->d.checkpoint_every = checkpoint_every;
     pl->sched_group = group > 0 ? group : 0;
     pl->sched_lanes = lanes > 0 ? lanes : 1;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_set_streaming(sem_plan* pl, int mode) {
+int sem_plan_set_streaming(sem_plan* pl, int mode) try {
     if (!pl || mode < SEM_STREAM_AUTO || mode > SEM_STREAM_NONTEMPORAL)
         return fail(SEM_E_INVALID, "sem_plan_set_streaming: bad argument");
     pl->streaming = mode;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_get_schedule(const sem_plan* pl, int* checkpoint_every, int* group, int* lanes) {
+int sem_plan_get_schedule(const sem_plan* pl, int* checkpoint_every, int* group, int* lanes) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     if (checkpoint_every) *checkpoint_every = pl->d.checkpoint_every;
     if (group) *group = pl->sched_group;
     if (lanes) *lanes = pl->sched_lanes;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget_bytes) {
+int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget_bytes) try {
     if (!pl || batch < 1 || rec_pitch < 0) return fail(SEM_E_INVALID, "sem_plan_schedule_auto: bad argument");
     const size_t bl = batch_layout(pl, batch).total;
     if (budget_bytes <= bl) return fail(SEM_E_NOMEM, "budget below the batch-level workspace");
@@ -1204,26 +1229,26 @@ int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget
     while (G >= 1 && bytes(G, S) > avail) --G;
     if (G < 1) return fail(SEM_E_NOMEM, "budget below one solve's workspace");
     return sem_plan_set_schedule(pl, S, G, 1);
-}
+} ABI_CATCH
 
-int sem_forward(sem_plan* pl, const sem_batch* b, void* ws, size_t ws_bytes, void* stream) {
+int sem_forward(sem_plan* pl, const sem_batch* b, void* ws, size_t ws_bytes, void* stream) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? forward_impl<double>(pl, b, ws, ws_bytes, st)
                                        : forward_impl<float>(pl, b, ws, ws_bytes, st);
-}
+} ABI_CATCH
 
 int sem_forward_adjoint(sem_plan* pl, const sem_batch* b, const void* obs, double inv_sigma2, double* misfit,
-                        double* grad_c, double* grad_c2, void* ws, size_t ws_bytes, void* stream) {
+                        double* grad_c, double* grad_c2, void* ws, size_t ws_bytes, void* stream) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64
                ? forward_adjoint_impl<double>(pl, b, obs, inv_sigma2, misfit, grad_c, grad_c2, ws, ws_bytes, st)
                : forward_adjoint_impl<float>(pl, b, obs, inv_sigma2, misfit, grad_c, grad_c2, ws, ws_bytes, st);
-}
+} ABI_CATCH
 
 int sem_bspline_map(sem_plan* pl, int nmodel, int nbz, int nbx, const double* coef, const double* base, double* c,
-                    void* c2, void* stream) {
+                    void* c2, void* stream) try {
     if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !coef || !c2)
         return fail(SEM_E_INVALID, "sem_bspline_map: bad argument (need >= 4 nodes per axis)");
     int rc = SEM_OK;
@@ -1241,10 +1266,10 @@ int sem_bspline_map(sem_plan* pl, int nmodel, int nbz, int nbx, const double* co
                                                         (float*)c2);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
 int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double* g_c, double* g_coef,
-                        void* stream) {
+                        void* stream) try {
     if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !g_c || !g_coef)
         return fail(SEM_E_INVALID, "sem_bspline_project: bad argument");
     if (nbx > 32) return fail(SEM_E_UNSUPPORTED, "sem_bspline_project: at most 32 nodes along x");
@@ -1264,7 +1289,7 @@ int sem_bspline_project(sem_plan* pl, int nmodel, int nbz, int nbx, const double
     LAUNCH_CHECK();
     HIP_TRY(hipFreeAsync(tmp, st));
     return SEM_OK;
-}
+} ABI_CATCH
 
 static int curve_params(const sem_plan* pl, const sem_curve_desc* cd, int nmodel, CurveParams& p) {
     if (!pl || !cd || nmodel < 1 || nmodel > 65535) return fail(SEM_E_INVALID, "sem_curve: bad argument");
@@ -1293,7 +1318,7 @@ static int curve_params(const sem_plan* pl, const sem_curve_desc* cd, int nmodel
 }
 
 int sem_curve_velocity_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, const double* ctrl, double* c,
-                          void* c2, double* aux, void* stream) {
+                          void* c2, double* aux, void* stream) try {
     CurveParams p;
     int rc = curve_params(pl, cd, nmodel, p);
     if (rc) return rc;
@@ -1308,10 +1333,10 @@ int sem_curve_velocity_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, co
                                                            pl->NX, pl->NZ, c, (float*)c2, aux);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
 int sem_curve_project_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, const double* dL_dv,
-                         const double* aux, double* grad_ctrl, void* stream) {
+                         const double* aux, double* grad_ctrl, void* stream) try {
     CurveParams p;
     int rc = curve_params(pl, cd, nmodel, p);
     if (rc) return rc;
@@ -1327,22 +1352,22 @@ int sem_curve_project_ex(sem_plan* pl, const sem_curve_desc* cd, int nmodel, con
     LAUNCH_CHECK();
     HIP_TRY(hipFreeAsync(partial, st));
     return SEM_OK;
-}
+} ABI_CATCH
 
 int sem_curve_velocity(sem_plan* pl, int nmodel, const double* ctrl, double v_in, double v_out, double tau,
-                       int samples, int newton_steps, double* c, void* c2, double* aux, void* stream) {
+                       int samples, int newton_steps, double* c, void* c2, double* aux, void* stream) try {
     sem_curve_desc cd{SEM_CURVE_CLAMPED6, 6, samples, newton_steps, v_in, v_out, tau, nullptr, SEM_BLEND_SMOOTH};
     return sem_curve_velocity_ex(pl, &cd, nmodel, ctrl, c, c2, aux, stream);
-}
+} ABI_CATCH
 
 int sem_curve_project(sem_plan* pl, int nmodel, const double* dL_dv, const double* aux, double v_in, double v_out,
-                      double tau, double* grad_ctrl, void* stream) {
+                      double tau, double* grad_ctrl, void* stream) try {
     sem_curve_desc cd{SEM_CURVE_CLAMPED6, 6, 2, 0, v_in, v_out, tau, nullptr, SEM_BLEND_SMOOTH};
     return sem_curve_project_ex(pl, &cd, nmodel, dL_dv, aux, grad_ctrl, stream);
-}
+} ABI_CATCH
 
 int sem_forward_snapshots(sem_plan* pl, const sem_batch* b, int every, int ix0, int ix1, int iz0, int iz1,
-                          void* snapshots, void* ws, size_t ws_bytes, void* stream) {
+                          void* snapshots, void* ws, size_t ws_bytes, void* stream) try {
     if (!pl || !snapshots) return fail(SEM_E_INVALID, "sem_forward_snapshots: null argument");
     if (every < 1) return fail(SEM_E_INVALID, "sem_forward_snapshots: every must be >= 1");
     if (ix0 < 0 || ix1 > pl->NX || ix0 >= ix1 || iz0 < 0 || iz1 > pl->NZ || iz0 >= iz1)
@@ -1351,10 +1376,10 @@ int sem_forward_snapshots(sem_plan* pl, const sem_batch* b, int every, int ix0, 
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? forward_impl<double>(pl, b, ws, ws_bytes, st, &sp)
                                        : forward_impl<float>(pl, b, ws, ws_bytes, st, &sp);
-}
+} ABI_CATCH
 
 int sem_grid_to_nodes(sem_plan* pl, int nxg, int nzg, const double* xg, const double* zg, const double* vp,
-                      int method, double* out, void* stream) {
+                      int method, double* out, void* stream) try {
     if (!pl || !xg || !zg || !vp || !out) return fail(SEM_E_INVALID, "sem_grid_to_nodes: null argument");
     if (nxg < 2 || nzg < 2) return fail(SEM_E_INVALID, "sem_grid_to_nodes: the grid needs >= 2 points per axis");
     if (method != SEM_INTERP_LINEAR && method != SEM_INTERP_NEAREST)
@@ -1365,20 +1390,20 @@ int sem_grid_to_nodes(sem_plan* pl, int nxg, int nzg, const double* xg, const do
                                                 (double*)pl->d_z1d.p, pl->NX, pl->NZ, out);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_layout(const sem_plan* pl, int* pitch, long* elems) {
+int sem_plan_layout(const sem_plan* pl, int* pitch, long* elems) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     if (pitch) *pitch = pl->pitch;
     if (elems) *elems = pl->sb;
     return SEM_OK;
-}
+} ABI_CATCH
 
-int sem_plan_gradient_elems(const sem_plan* pl, long* elems) {
+int sem_plan_gradient_elems(const sem_plan* pl, long* elems) try {
     if (!pl || !elems) return fail(SEM_E_INVALID, "sem_plan_gradient_elems: bad argument");
     *elems = pl->gsb;
     return SEM_OK;
-}
+} ABI_CATCH
 
 }  // extern "C"
 
@@ -1434,16 +1459,16 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
 extern "C" {
 
 int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
-                   void* stream) {
+                   void* stream) try {
     if (!pl || !slots || nslots < 1 || m < 0 || rec_pitch < 0) return fail(SEM_E_INVALID, "sem_slots_step: bad argument");
     if ((long)std::max(pl->ntiles, pl->ctiles) * nslots >= (1L << 31) - 64)
         return fail(SEM_E_INVALID, "too many slots for this mesh");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, m, mask, rec_pitch, false, st)
                                        : slots_launch<float>(pl, slots, nslots, m, mask, rec_pitch, false, st);
-}
+} ABI_CATCH
 
-int sem_slots_correlate(sem_plan* pl, const sem_slot* slots, int nslots, int chunk, int chunk_len, void* stream) {
+int sem_slots_correlate(sem_plan* pl, const sem_slot* slots, int nslots, int chunk, int chunk_len, void* stream) try {
     if (!pl || !slots || nslots < 1 || chunk < 0 || chunk_len < 1)
         return fail(SEM_E_INVALID, "sem_slots_correlate: bad argument");
     if ((long)std::max(pl->ntiles, pl->ctiles) * nslots >= (1L << 31) - 64)
@@ -1451,17 +1476,17 @@ int sem_slots_correlate(sem_plan* pl, const sem_slot* slots, int nslots, int chu
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_corr<double>(pl, slots, nslots, chunk, chunk_len, st)
                                        : slots_corr<float>(pl, slots, nslots, chunk, chunk_len, st);
-}
+} ABI_CATCH
 
 int sem_slots_record_final(sem_plan* pl, const sem_slot* slots, int nslots, unsigned mask, int rec_pitch,
-                           void* stream) {
+                           void* stream) try {
     if (!pl || !slots || nslots < 1) return fail(SEM_E_INVALID, "sem_slots_record_final: bad argument");
     hipStream_t st = (hipStream_t)stream;
     return pl->d.precision == SEM_FP64 ? slots_launch<double>(pl, slots, nslots, 0, mask, rec_pitch, true, st)
                                        : slots_launch<float>(pl, slots, nslots, 0, mask, rec_pitch, true, st);
-}
+} ABI_CATCH
 
-int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) {
+int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) try {
     if (!pl || max_entries < 0 || max_sets < 0) return fail(SEM_E_INVALID, "sem_receivers_reserve: bad argument");
     const size_t eb = pl->d.precision == SEM_FP64 ? sizeof(RecEntry<double, 4>) : sizeof(RecEntry<float, 4>);
     bool m1 = false, m2 = false;
@@ -1473,14 +1498,14 @@ int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) {
         pl->rec_dirty = true;
     }
     return prepare_receivers(pl);
-}
+} ABI_CATCH
 
-int sem_receivers_sync(sem_plan* pl) {
+int sem_receivers_sync(sem_plan* pl) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     return prepare_receivers(pl);
-}
+} ABI_CATCH
 
-int sem_pad_nodal(sem_plan* pl, const void* src, void* dst, int nmodel, void* stream) {
+int sem_pad_nodal(sem_plan* pl, const void* src, void* dst, int nmodel, void* stream) try {
     if (!pl || !src || !dst || nmodel < 1 || nmodel > 65535) return fail(SEM_E_INVALID, "sem_pad_nodal: bad argument");
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), nmodel);
@@ -1490,10 +1515,10 @@ int sem_pad_nodal(sem_plan* pl, const void* src, void* dst, int nmodel, void* st
         pad_rows_kernel<float><<<grid, 256, 0, st>>>((const float*)src, (float*)dst, pl->NX, pl->NZ, pl->pitch, pl->sb);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
 int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
-                 int rec_pitch, void* stream) {
+                 int rec_pitch, void* stream) try {
     if (!pl || !seis || !obs || !res || !misfit || nrec < 0 || nrec > rec_pitch || nt < 1)
         return fail(SEM_E_INVALID, "sem_residual: bad argument");
     hipStream_t st = (hipStream_t)stream;
@@ -1505,10 +1530,10 @@ int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, dou
                                                   nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
 int sem_gradient_finalize(sem_plan* pl, const double* G, const void* c2p, double scale, double* grad_c,
-                          double* grad_c2, void* stream) {
+                          double* grad_c2, void* stream) try {
     if (!pl || !G || !c2p) return fail(SEM_E_INVALID, "sem_gradient_finalize: bad argument");
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), 1);
@@ -1522,6 +1547,6 @@ int sem_gradient_finalize(sem_plan* pl, const double* G, const void* c2p, double
                                                           (const float*)c2p, nullptr, pl->J * scale, grad_c, grad_c2);
     LAUNCH_CHECK();
     return SEM_OK;
-}
+} ABI_CATCH
 
 }  // extern "C"

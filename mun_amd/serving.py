@@ -160,7 +160,7 @@ class Lane:
         self.slots: List[Optional[_Active]] = [None] * capacity
         self.queue = deque()
         self.event = None
-        self.graph = None
+        self.graphs = {}  # phase mask -> captured round
         self.rounds = 0
 
     def _reserve(self):
@@ -171,7 +171,7 @@ class Lane:
         if len(self.plan.rec_sizes) + 1 > self.max_sets:
             self.max_sets *= 2
             self._reserve()
-            self.graph = None  # captured arguments point at the old table
+            self.graphs = {}  # captured arguments point at the old table
 
     # -------------------------------------------------------------- slots
     def free_slot(self):
@@ -214,31 +214,42 @@ class Lane:
         self.table_pinned.numpy()[:] = np.frombuffer(self.table_host, dtype=np.uint8)
         self.table_dev.copy_(self.table_pinned, non_blocking=True)
 
-    def _launch_round(self):
+    def phase_mask(self) -> int:
+        """Phases present in the lane (bit 1 << SLOT_*): a round launches the
+        forward kernel only when a slot runs forward or recompute steps, the
+        adjoint kernel and the correlation only when a slot runs adjoint steps."""
+        mask = 0
+        for a in self.occupied():
+            mask |= 1 << a.phase
+        return mask
+
+    def _launch_round(self, mask: int):
         lib, h = self.plan.lib, self.plan.h
         sp = C.c_void_p(self.stream.cuda_stream)
-        mask = (1 << L.SLOT_FORWARD) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT) | (1 << L.SLOT_FORWARD_HIST)
         tab = C.c_void_p(self.table_dev.data_ptr())
+        adjoint = bool(mask & (1 << L.SLOT_ADJOINT))
         for m in range(self.S):
             L.check(lib.sem_slots_step(h, tab, self.C, m, mask, self.rec_pitch, sp))
-            if (m + 1) % self.chunk == 0 or m == self.S - 1:
+            if adjoint and ((m + 1) % self.chunk == 0 or m == self.S - 1):
                 # gradient correlation of the adjoint slots' chunk that just ended
                 L.check(lib.sem_slots_correlate(h, tab, self.C, m // self.chunk, self.chunk, sp))
 
     def launch_round(self, use_graph: bool):
+        mask = self.phase_mask()
         with torch.cuda.stream(self.stream):
             self.write_table()
             if use_graph:
-                if self.graph is None:
+                g = self.graphs.get(mask)
+                if g is None:
                     L.check(self.plan.lib.sem_receivers_sync(self.plan.h))
                     torch.cuda.synchronize(self.server.device)
                     g = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(g, stream=self.stream):
-                        self._launch_round()
-                    self.graph = g
-                self.graph.replay()
+                        self._launch_round(mask)
+                    self.graphs[mask] = g
+                g.replay()
             else:
-                self._launch_round()
+                self._launch_round(mask)
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
         self.rounds += 1

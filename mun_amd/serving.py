@@ -398,7 +398,10 @@ class SEMServer:
 
     # -------------------------------------------------------------- driver
     def run(self, requests: List[SolveRequest], progress=None, realtime: bool = True) -> Dict[int, SolveResult]:
-        """Serve `requests` (arrival times in seconds from start) to completion."""
+        """Serve `requests` (arrival times in seconds from start) to completion.
+        Lanes, their captured rounds and tile choices persist across calls (a
+        warm server); each call returns its own results."""
+        self.results = {}
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         self.t0 = time.perf_counter()
         total = len(pending)
@@ -435,11 +438,13 @@ class SEMServer:
 
 
 def run_plain(requests: List[SolveRequest], device=None, adjoint_mode="reference",
-              sides=("xmin", "xmax", "zmax"), segment=100, progress=None) -> Dict[int, SolveResult]:
+              sides=("xmin", "xmax", "zmax"), segment=100, progress=None,
+              plans: Optional[dict] = None) -> Dict[int, SolveResult]:
     """The plain path: one request at a time through the batch path at batch 1
-    (one launch sequence per solve, no overlap), in arrival order."""
+    (one launch sequence per solve, no overlap), in arrival order.  `plans`
+    (a dict) keeps the plans of earlier calls for reuse."""
     dev = torch.device(device if device is not None else "cuda")
-    plans = {}
+    plans = {} if plans is None else plans
     out = {}
     t0 = time.perf_counter()
     for r in sorted(requests, key=lambda q: q.arrival):

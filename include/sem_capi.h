@@ -171,7 +171,8 @@ int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_b
  *            loglik = -misfit * inv_sigma2 (core.py:1230; inv_sigma2 =
  *            1/(noise_std^2 + 1e-24) as core.py:1086)
  *   grad_c2  device double [batch][N] or NULL     d loglik / d (c^2)
- * The gradient-time integral is accumulated in double. */
+ * The gradient-time integral is accumulated in double.  With both gradient
+ * outputs NULL only the forward pass and the misfit run. */
 int sem_forward_adjoint(sem_plan* plan, const sem_batch* b, const void* obs, double inv_sigma2,
                         double* misfit, double* grad_c, double* grad_c2,
                         void* workspace, size_t ws_bytes, void* stream);

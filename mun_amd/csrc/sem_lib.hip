@@ -770,8 +770,10 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
     char* ws = g.ws;
     const int B = bt->batch, nt = pl->d.nt;
     const long state = (long)B * pl->sb;
+    // without a gradient output only the misfit is wanted: no history, no adjoint
+    const bool adjoint = grad_c || grad_c2;
     int rc;
-    if ((rc = forward_loop<T>(pl, g, L, true, st))) return rc;
+    if ((rc = forward_loop<T>(pl, g, L, adjoint, st))) return rc;
 
     // residual and misfit
     T* RES = (T*)(ws + L.RES);
@@ -779,6 +781,7 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
                                           misfit + g.g0, g.nrec, 0, nt, bt->rec_pitch, (long)nt * bt->rec_pitch,
                                           pl->d.win_begin, pl->d.win_end, pl->d.win_decim);
     LAUNCH_CHECK();
+    if (!adjoint) return SEM_OK;
 
     // adjoint sweep: per segment, recompute u into SEG, then chunks of up to C
     // adjoint steps (states in a ring indexed by time) each followed by the

@@ -135,3 +135,22 @@ def test_streaming_hints_give_the_same_bits(prob_sched):
     for o in outs[1:]:
         for k in ("seis", "misfit", "grad_c"):
             assert torch.equal(o[k], outs[0][k]), k
+
+
+def test_misfit_only_call_skips_the_adjoint(prob_sched):
+    """Without gradient outputs the call returns the same misfit and
+    seismograms as with them."""
+    prob, disc, coef, c, seis = prob_sched
+    print("[misfit only]", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="reference", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    for sch in ((40, 0, 1), (0, 3, 3)):
+        plan.set_schedule(*sch)
+        full = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        only = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of, want_grad_c=False)
+        torch.cuda.synchronize()
+        assert only["grad_c"] is None
+        assert torch.equal(only["misfit"], full["misfit"])
+        assert torch.equal(only["seis"], full["seis"])

@@ -28,6 +28,11 @@ the device.  A round can be replayed from a captured HIP graph (its launch
 arguments only point at the lane's device slot table, which is rewritten
 between rounds).
 
+Rounds are pipelined: the host enqueues a lane's next round (with the state
+transitions, final samples, residuals and finishing work of the previous one,
+all deterministic) while the previous round runs, keeping up to
+ROUNDS_AHEAD rounds per lane on the device, so no round waits for the host.
+
 Each slot's arithmetic is the batch path's, so a result does not depend on
 which other requests share its launches.
 """
@@ -45,6 +50,8 @@ import torch
 
 from . import _lib as L
 from .solver import Plan, source_wavelet
+
+ROUNDS_AHEAD = 2  # rounds a lane keeps enqueued ahead of the host
 
 
 @dataclass
@@ -155,11 +162,15 @@ class Lane:
         self._reserve()
         self.stream = torch.cuda.Stream(device=server.device)
         self.table_host = (L.SemSlot * capacity)()
-        self.table_pinned = torch.empty(C.sizeof(self.table_host), dtype=torch.uint8).pin_memory()
+        # two pinned staging copies of the table: the copy for round r + 2 is
+        # written only after round r (and its table upload) has completed
+        self.table_pinned = [torch.empty(C.sizeof(self.table_host), dtype=torch.uint8).pin_memory()
+                             for _ in range(ROUNDS_AHEAD)]
         self.table_dev = torch.zeros(C.sizeof(self.table_host), dtype=torch.uint8, device=server.device)
         self.slots: List[Optional[_Active]] = [None] * capacity
         self.queue = deque()
-        self.event = None
+        self.inflight = deque()   # completion events of the rounds enqueued and not yet seen done
+        self.finishing = []       # (solve, event, host copies) of solves whose results are on their way
         self.graphs = {}  # phase mask -> captured round
         self.rounds = 0
 
@@ -211,8 +222,12 @@ class Lane:
         for i in range(self.C):
             a = self.slots[i]
             self.table_host[i] = self._desc(a) if a is not None else L.SemSlot()
-        self.table_pinned.numpy()[:] = np.frombuffer(self.table_host, dtype=np.uint8)
-        self.table_dev.copy_(self.table_pinned, non_blocking=True)
+        pinned = self.table_pinned[self.rounds % ROUNDS_AHEAD]
+        pinned.numpy()[:] = np.frombuffer(self.table_host, dtype=np.uint8)
+        self.table_dev.copy_(pinned, non_blocking=True)
+
+    def busy(self) -> bool:
+        return bool(self.occupied() or self.queue or self.inflight or self.finishing)
 
     def phase_mask(self) -> int:
         """Phases present in the lane (bit 1 << SLOT_*): a round launches the
@@ -250,8 +265,9 @@ class Lane:
                 g.replay()
             else:
                 self._launch_round(mask)
-            self.event = torch.cuda.Event()
-            self.event.record(self.stream)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+            self.inflight.append(ev)
         self.rounds += 1
 
 
@@ -344,7 +360,12 @@ class SEMServer:
         return True
 
     # -------------------------------------------------------------- transitions
-    def _after_round(self, lane: Lane):
+    def _after_enqueue(self, lane: Lane):
+        """Work that follows the round just enqueued, in stream order: final
+        samples and residuals of the slots whose forward pass it completes,
+        the slots' next phases, and the finishing work of the solves it
+        completes (their slots free up at once; their memory when the results
+        have reached the host)."""
         plan = lane.plan
         lib, h = plan.lib, plan.h
         sp = C.c_void_p(lane.stream.cuda_stream)
@@ -380,6 +401,9 @@ class SEMServer:
                 self._finish(lane, a)
 
     def _finish(self, lane: Lane, a: _Active):
+        """Enqueue (on the lane's stream, inside its stream context) the
+        gradient of a completed solve, its projection onto the request's
+        B-spline nodes and the copies of the results to pinned host memory."""
         plan, b, req = lane.plan, a.buf, a.req
         inv_s2 = 1.0 / (req.noise_std ** 2 + 1e-24)
         gc = torch.empty(plan.N, dtype=torch.float64, device=self.device)
@@ -388,13 +412,34 @@ class SEMServer:
                                                C.c_void_p(lane.stream.cuda_stream)))
         nbz, nbx = np.asarray(req.coef).shape
         g = plan.bspline_project(gc[None], nbz, nbx)[0]
-        lane.stream.synchronize()
-        mis = float(b["misfit"].item())
-        now = self.clock()
-        self.results[req.rid] = SolveResult(req.rid, -mis * inv_s2, mis, g.cpu().numpy(),
-                                            b["seis"][:, :a.nrec].cpu().numpy(), req.arrival, a.admitted, now)
+        host = {"g": torch.empty(g.shape, dtype=g.dtype, pin_memory=True),
+                "seis": torch.empty((req.nt, a.nrec), dtype=b["seis"].dtype, pin_memory=True),
+                "misfit": torch.empty(1, dtype=torch.float64, pin_memory=True)}
+        host["g"].copy_(g, non_blocking=True)
+        host["seis"].copy_(b["seis"][:, :a.nrec], non_blocking=True)
+        host["misfit"].copy_(b["misfit"], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(lane.stream)
+        lane.finishing.append((a, ev, host, (gc, g)))
         lane.slots[a.slot] = None
-        self.in_use -= a.footprint
+
+    def _collect(self, lane: Lane) -> bool:
+        """Results of finished solves whose copies have landed; their memory
+        returns to the budget."""
+        got = False
+        for item in list(lane.finishing):
+            a, ev, host, _ = item
+            if not ev.query():
+                continue
+            req = a.req
+            inv_s2 = 1.0 / (req.noise_std ** 2 + 1e-24)
+            mis = float(host["misfit"][0])
+            self.results[req.rid] = SolveResult(req.rid, -mis * inv_s2, mis, host["g"].numpy().copy(),
+                                                host["seis"].numpy().copy(), req.arrival, a.admitted, self.clock())
+            lane.finishing.remove(item)
+            self.in_use -= a.footprint
+            got = True
+        return got
 
     # -------------------------------------------------------------- driver
     def run(self, requests: List[SolveRequest], progress=None, realtime: bool = True) -> Dict[int, SolveResult]:
@@ -406,7 +451,7 @@ class SEMServer:
         self.t0 = time.perf_counter()
         total = len(pending)
         last_report = 0.0
-        while pending or any(l.occupied() or l.queue for l in self.lanes.values()):
+        while pending or any(l.busy() for l in self.lanes.values()):
             now = self.clock()
             while pending and (pending[0].arrival <= now or not realtime):
                 r = pending.popleft()
@@ -415,16 +460,19 @@ class SEMServer:
                 self.lane(r).queue.append(r)
             progressed = False
             for lane in list(self.lanes.values()):
-                if lane.event is not None:
-                    if not lane.event.query():
-                        continue
-                    lane.event = None
-                    self._after_round(lane)
+                while lane.inflight and lane.inflight[0].query():
+                    lane.inflight.popleft()
                     progressed = True
-                while lane.queue and self._admit(lane, lane.queue[0], self.clock()):
-                    lane.queue.popleft()
-                if lane.occupied():
+                progressed |= self._collect(lane)
+                # keep up to ROUNDS_AHEAD rounds enqueued: admit, enqueue the
+                # next round, then the work that follows it
+                while len(lane.inflight) < ROUNDS_AHEAD:
+                    while lane.queue and self._admit(lane, lane.queue[0], self.clock()):
+                        lane.queue.popleft()
+                    if not lane.occupied():
+                        break
                     lane.launch_round(self.use_graph)
+                    self._after_enqueue(lane)
                     progressed = True
             if progress and self.clock() - last_report > 5.0:
                 last_report = self.clock()

@@ -154,3 +154,28 @@ def test_misfit_only_call_skips_the_adjoint(prob_sched):
         assert only["grad_c"] is None
         assert torch.equal(only["misfit"], full["misfit"])
         assert torch.equal(only["seis"], full["seis"])
+
+
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_schedules_with_a_misfit_window(prob_sched, mode):
+    """A misfit window (the adjoint runs over the window only in the
+    reference mode) gives the same bits under every schedule."""
+    from mun_amd.solver import Plan
+    prob, disc, coef, c, seis = prob_sched
+    print(f"[schedules window] {mode}", flush=True)
+    plan = Plan(prob.nelem_x, prob.nelem_z, prob.xmin, prob.xmax, prob.zmin, prob.zmax, prob.dt, prob.nt, 4,
+                prob.sponge_width, prob.sponge_cmax, prob.sponge_R, prob.sponge_sides, precision="fp32",
+                adjoint_mode=mode, checkpoint_every=30, misfit_window=(40, 200, 3))
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.95, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    outs = []
+    for sch in ((30, 0, 1), (0, 2, 3), (45, 3, 2)):
+        plan.set_schedule(*sch)
+        o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        torch.cuda.synchronize()
+        outs.append(o)
+    assert float(outs[0]["grad_c"].abs().max()) > 0
+    for o in outs[1:]:
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(o[k], outs[0][k]), k

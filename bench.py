@@ -37,8 +37,9 @@ HBM_PEAK_GBS = 8000.0
 FWD_BYTES_PER_NODE = 20  # u^n, w^n, c^2 read; u^{n+1}, w^{n+1} written (FP32, increment form)
 # one forward+adjoint solve step with the full forward history: forward 20 B,
 # adjoint 20 B, correlation 8 B (u^{k+1}, lambda_k read), FP64 accumulator
-# 16 B read+written once per chunk of 32 steps
-SOLVE_BYTES_PER_NODE_STEP = 20 + 20 + 8 + 16 / 32
+# (25 doubles per element of 16 nodes = 12.5 B per node) read and written
+# once per chunk of 32 steps
+SOLVE_BYTES_PER_NODE_STEP = 20 + 20 + 8 + 25 / 32
 STD_BYTES_PER_NODE = 16  # the standard two-level form's minimum (u^n, u^{n-1}, c^2 read; u^{n+1} written),
                          # which misses the FP32 tolerance at this size
 

@@ -23,12 +23,15 @@ from . import workloads as W
 from .batch import BatchSolver
 
 
-def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024):
-    """Bytes per forward-step launch from two rocprofv3 counter passes
+def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles_per_solve=195, threads=256):
+    """Bytes per step-kernel launch from two rocprofv3 counter passes
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
     counter reads for the calibration copy (the runtime's copyBuffer kernel
     moving copy_mb MiB), against the step kernel's 20 algorithmic bytes per
-    node (12 read, 8 written)."""
+    node (12 read, 8 written).  Solves per launch follow from each dispatch's
+    grid (tiles_per_solve workgroups of `threads` per solve; 195 tiles of the
+    15-row kernel on the 801 x 801-node mesh).  `forward` / `adjoint` report
+    the launches of `batch` solves (else the largest), `kernels` all."""
     import csv
     import statistics
 
@@ -43,6 +46,7 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024):
 
     copy_bytes = copy_mb << 20
     out = {"batch": batch, "nodes": nodes}
+    recs = {}
     for counter, path, alg in (("FETCH_SIZE", fetch_csv, 12), ("WRITE_SIZE", write_csv, 8)):
         vals = per_kernel(path, counter)
         # calibration: the largest copyBuffer dispatches (the copy_mb MiB copies)
@@ -55,16 +59,23 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024):
             if not name.startswith("sem::step_kernel") or ", 2, " in name:
                 continue
             mode = "forward" if ", 0, " in name else "adjoint"
-            rec = out.setdefault(mode, {"kernel": name, "grid": grid, "dispatches": len(vs)})
+            solves = max(1, round(grid / threads / tiles_per_solve))
+            rec = recs.setdefault((mode, solves), {"mode": mode, "kernel": name, "grid": grid, "solves": solves,
+                                                   "dispatches": len(vs)})
             rec[f"{key}_kib_raw_median"] = statistics.median(vs)
             rec[f"{key}_bytes"] = statistics.median(vs) * 1024.0 * factor
-            rec[f"{key}_algorithmic_bytes"] = alg * nodes * batch
-    for mode in ("forward", "adjoint"):
-        rec = out.get(mode)
-        if rec and "fetch_bytes" in rec and "write_bytes" in rec:
+            rec[f"{key}_algorithmic_bytes"] = alg * nodes * solves
+    for rec in recs.values():
+        if "fetch_bytes" in rec and "write_bytes" in rec:
             rec["bytes_per_launch"] = int(rec["fetch_bytes"] + rec["write_bytes"])
-            rec["algorithmic_bytes_per_launch"] = 20 * nodes * batch
+            rec["algorithmic_bytes_per_launch"] = 20 * nodes * rec["solves"]
             rec["ratio"] = round(rec["bytes_per_launch"] / rec["algorithmic_bytes_per_launch"], 4)
+    out["kernels"] = sorted(recs.values(), key=lambda r: (r["mode"], r["solves"]))
+    for mode in ("forward", "adjoint"):
+        mine = [r for r in out["kernels"] if r["mode"] == mode and "ratio" in r]
+        if mine:
+            pick = [r for r in mine if r["solves"] == batch] or [max(mine, key=lambda r: r["solves"])]
+            out[mode] = pick[0]
     if "forward" in out:
         out["bytes_per_launch"] = out["forward"].get("bytes_per_launch")
     return out

@@ -819,26 +819,32 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
 // r = seis - obs (first nrec columns of each row), misfit[b] = 0.5 * sum r^2 in double.
 // Rows outside the misfit window [w0, w1) or off its decimation grid give r = 0.
 // nrec_of: per-solve receiver counts on the device, or null to use nrec.
+// One block of RESID_THREADS per solve: each wave walks rows (wave, wave +
+// nwaves, ...), its lanes the columns; per-thread FP64 sums are combined in a
+// fixed tree, so the misfit is deterministic.
+constexpr int RESID_THREADS = 1024;
 template <typename T>
-__global__ __launch_bounds__(256) void residual_kernel(const T* seis, const T* obs, T* res, double* misfit,
-                                                       const int* nrec_of, int nrec, int nt, int pitch, long bs,
-                                                       int w0, int w1, int decim) {
+__global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, const T* obs, T* res,
+                                                                 double* misfit, const int* nrec_of, int nrec,
+                                                                 int nt, int pitch, long bs, int w0, int w1,
+                                                                 int decim) {
     const long b = blockIdx.x;
     if (nrec_of) nrec = nrec_of[b];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = RESID_THREADS / 64;
     double acc = 0.0;
-    const long total = (long)nt * pitch;
-    for (long k = threadIdx.x; k < total; k += blockDim.x) {
-        const int c = (int)(k % pitch);
-        const int row = (int)(k / pitch);
+    for (int row = wave; row < nt; row += nwaves) {
         const bool in = row >= w0 && row < w1 && (row - w0) % decim == 0;
-        const T rv = (c < nrec && in) ? seis[b * bs + k] - obs[b * bs + k] : T(0);
-        res[b * bs + k] = rv;
-        acc += (double)rv * (double)rv;
+        const long o = b * bs + (long)row * pitch;
+        for (int c = lane; c < pitch; c += 64) {
+            const T rv = (c < nrec && in) ? seis[o + c] - obs[o + c] : T(0);
+            res[o + c] = rv;
+            acc += (double)rv * (double)rv;
+        }
     }
-    __shared__ double red[256];
+    __shared__ double red[RESID_THREADS];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = RESID_THREADS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }

@@ -777,7 +777,7 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
 
     // residual and misfit
     T* RES = (T*)(ws + L.RES);
-    residual_kernel<T><<<B, 256, 0, st>>>((const T*)bt->seis, (const T*)obs + (long)g.g0 * nt * bt->rec_pitch, RES,
+    residual_kernel<T><<<B, RESID_THREADS, 0, st>>>((const T*)bt->seis, (const T*)obs + (long)g.g0 * nt * bt->rec_pitch, RES,
                                           misfit + g.g0, g.nrec, 0, nt, bt->rec_pitch, (long)nt * bt->rec_pitch,
                                           pl->d.win_begin, pl->d.win_end, pl->d.win_decim);
     LAUNCH_CHECK();
@@ -1526,10 +1526,10 @@ int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, dou
         return fail(SEM_E_INVALID, "sem_residual: bad argument");
     hipStream_t st = (hipStream_t)stream;
     if (pl->d.precision == SEM_FP64)
-        residual_kernel<double><<<1, 256, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
+        residual_kernel<double><<<1, RESID_THREADS, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
                                                    nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     else
-        residual_kernel<float><<<1, 256, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
+        residual_kernel<float><<<1, RESID_THREADS, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
                                                   nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     LAUNCH_CHECK();
     return SEM_OK;

@@ -454,11 +454,9 @@ size_t workspace_total(const sem_plan* pl, int B, int pitch, bool adj) {
     return batch_layout(pl, B).total + (size_t)s.lanes * layout(pl, s.group, pitch, adj, s.ckpt).total;
 }
 
-// Non-temporal state streams when the solves in flight touch more than the
-// last-level cache per step (u^n, u^{n+1}, w, c^2 of each).
-// Schedules with groups in lanes need this many step workgroups in flight
-// (about 3/4 of what 256 CUs hold at 8 workgroups each); below it the batch
-// runs whole, with checkpoints.
+// Schedules with full-history groups in lanes need this many step workgroups
+// in flight (1.5 times the 1024 that 256 CUs hold at once with the step
+// kernels' 4 waves per SIMD); below it the batch runs whole, with checkpoints.
 constexpr long MIN_INFLIGHT_TILES = 1536;
 // Lanes of groups in flight: 3 lanes and the caller's stream use the 4
 // hardware queues HIP gives a process by default; a fourth lane shares a
@@ -466,6 +464,8 @@ constexpr long MIN_INFLIGHT_TILES = 1536;
 // solves/s, 4 x 64 1261).
 constexpr int MAX_LANES = 3;
 
+// Non-temporal state streams when the solves in flight touch more than the
+// last-level cache per step (u^n, u^{n+1}, w, c^2 of each).
 bool use_nt(const sem_plan* pl, long inflight) {
     if (pl->streaming != SEM_STREAM_AUTO) return pl->streaming == SEM_STREAM_NONTEMPORAL;
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
@@ -1204,10 +1204,10 @@ int sem_plan_schedule_auto(sem_plan* pl, int batch, int rec_pitch, size_t budget
     const size_t avail = budget_bytes - bl;
     const int nt = pl->d.nt;
     auto bytes = [&](int G, int ck) { return layout(pl, G, rec_pitch, true, ck).total; };
-    // 1. the full forward history (no recompute pass): the whole batch in one
-    // group if it fits in memory and its step working set in 3/4 of the
-    // last-level cache, else lanes of groups that do, provided the solves in
-    // flight fill most of the GPU
+    // 1. the full forward history (no recompute pass): the whole batch, one
+    // group per lane, if it fits in memory and its step working set in 3/4 of
+    // the last-level cache; else lanes of groups as large as both limits
+    // allow, provided the solves in flight keep the GPU busy
     const size_t Ts = pl->d.precision == SEM_FP64 ? 8 : 4;
     const long f_llc = std::max<long>(1, (long)(LLC_BYTES * 3 / 4 / (4 * pl->sb * Ts)));
     const long F = std::min<long>(std::min<long>(batch, f_llc), (long)(avail / bytes(1, 0)));

@@ -100,3 +100,15 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(name, "size")] == C.sizeof(cls), name
         for fname, _ in cls._fields_:
             assert got[(name, fname)] == getattr(cls, fname).offset, (name, fname)
+
+
+def test_schedule_and_streaming_entry_points_validate_arguments():
+    """The schedule and streaming entry points reject a null plan or bad
+    values before any device work."""
+    lib = L.lib()
+    assert lib.sem_plan_set_schedule(None, 0, 0, 1) == -1
+    assert lib.sem_plan_get_schedule(None, None, None, None) == -1
+    assert lib.sem_plan_schedule_auto(None, 1, 1, 1 << 30) == -1
+    assert lib.sem_plan_set_streaming(None, 0) == -1
+    assert lib.sem_workspace_bytes(None, 1, 1, 1, None) == -1
+    assert "bad argument" in lib.sem_last_error().decode()

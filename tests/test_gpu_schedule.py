@@ -179,3 +179,30 @@ def test_schedules_with_a_misfit_window(prob_sched, mode):
     for o in outs[1:]:
         for k in ("seis", "misfit", "grad_c"):
             assert torch.equal(o[k], outs[0][k]), k
+
+
+def test_schedules_with_per_solve_wavelets(prob_sched):
+    """Per-solve source wavelets (rows of a [B, nt] tensor) follow their
+    solves into the groups of every schedule."""
+    prob, disc, coef, c, seis = prob_sched
+    print("[schedules per-solve wavelets]", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="reference", checkpoint_every=50)
+    C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
+    B = len(srcs)
+    W2 = torch.stack([w * (1.0 + 0.1 * k) for k in range(B)]).contiguous()
+    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    outs = []
+    for sch in ((50, 0, 1), (0, 2, 3), (0, 3, 1)):
+        plan.set_schedule(*sch)
+        o = plan.forward_adjoint(C2, W2, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        torch.cuda.synchronize()
+        outs.append(o)
+    s0 = outs[0]["seis"]
+    for o in outs[1:]:
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(o[k], outs[0][k]), k
+    # solve k's wavelet is (1 + 0.1 k) w, and the seismograms are linear in it
+    one = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+    torch.cuda.synchronize()
+    ratio = (s0[3].double().norm() / one["seis"][3].double().norm()).item()
+    assert abs(ratio - 1.3) < 1e-5

@@ -387,9 +387,15 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
 }
 
 // NTS: non-temporal loads and stores of the increment and the new state (for
-// launches whose working set exceeds the last-level cache).
+// launches whose working set exceeds the last-level cache).  The FP32 adjoint
+// step is compiled for 5 waves per SIMD (94 VGPRs, no spills), which hides
+// more memory latency; the forward step and the FP64 instances keep the
+// compiler's choice (103 VGPRs, 4 waves; FP64 about 200), since at 5 waves
+// they spill.
 template <typename T, int P, int MODE, int EZ, bool NTS>
-__global__ __launch_bounds__(tile_threads(EZ)) void step_kernel(const StepArgs<T, P> a) {
+__global__ __launch_bounds__(tile_threads(EZ))
+__attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
+void step_kernel(const StepArgs<T, P> a) {
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;

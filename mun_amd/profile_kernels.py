@@ -4,6 +4,7 @@
     python -m mun_amd.profile_kernels --stream-ceiling
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
     python -m mun_amd.profile_kernels --scan-streaming [--nt 200]
+    python -m mun_amd.profile_kernels --summarize-trace KERNEL_TRACE_CSV [--top 30]
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -81,6 +82,35 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles
     return out
 
 
+def summarize_trace(trace_csv, top=30):
+    """Markdown table of a rocprofv3 kernel trace (`--kernel-trace
+    --output-format csv`): per (kernel, grid) the call count, average and
+    total duration and share of the summed kernel time, with the process
+    span.  Durations of kernels that overlap (lanes on separate streams) are
+    summed as recorded."""
+    import collections
+    import csv
+
+    grp = collections.defaultdict(lambda: [0, 0])
+    t0 = t1 = None
+    for r in csv.DictReader(open(trace_csv)):
+        s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        t0 = s if t0 is None else min(t0, s)
+        t1 = e if t1 is None else max(t1, e)
+        key = (r["Kernel_Name"].split("(")[0].replace("void ", ""), int(r["Grid_Size_X"]), r["VGPR_Count"],
+               r["LDS_Block_Size"])
+        grp[key][0] += 1
+        grp[key][1] += e - s
+    total = sum(v[1] for v in grp.values()) or 1
+    lines = [f"span {(t1 - t0) / 1e9:.3f} s, kernel time {total / 1e9:.3f} s" if t0 is not None else "empty trace",
+             "| kernel | grid | VGPR | LDS B | calls | avg us | total ms | % of kernel time |",
+             "|---|---|---|---|---|---|---|---|"]
+    for (name, grid, vgpr, lds), (n, ns) in sorted(grp.items(), key=lambda kv: -kv[1][1])[:top]:
+        lines.append(f"| `{name}` | {grid} | {vgpr} | {lds} | {n} | {ns / n / 1e3:.1f} | {ns / 1e6:.1f} | "
+                     f"{100 * ns / total:.2f} |")
+    return "\n".join(lines)
+
+
 def scan_streaming(nt=200, batches=(6, 12, 18, 21, 32, 64, 128, 256), tile_rows=15):
     """Forward step time per launch against solves per launch, with plain and
     with non-temporal state streams (HIP events over nt launches on the
@@ -124,9 +154,15 @@ def main(argv=None):
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two counter passes of this driver (no GPU work)")
     ap.add_argument("--out", help="JSON file for --summarize-pmc")
+    ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
+                    help="per-kernel table of a rocprofv3 kernel trace (no GPU work)")
+    ap.add_argument("--top", type=int, default=30, help="rows of --summarize-trace")
     ap.add_argument("--scan-streaming", action="store_true",
                     help="time the forward step against solves per launch, plain vs non-temporal streams")
     a = ap.parse_args(argv)
+    if a.summarize_trace:
+        print(summarize_trace(a.summarize_trace, a.top))
+        return 0
     if a.scan_streaming:
         scan_streaming(nt=a.nt if a.nt != 40 else 200, tile_rows=a.tile_rows)
         return 0

@@ -26,8 +26,8 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_receivers_reserve", "sem_receivers_sync", "sem_residual", "sem_gradient_finalize",
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
            "sem_slots_correlate", "sem_plan_gradient_elems", "sem_plan_set_schedule", "sem_plan_get_schedule",
-           "sem_plan_schedule_auto", "sem_plan_set_streaming",
-           "sem_last_error"]
+           "sem_plan_schedule_auto", "sem_plan_set_streaming", "sem_plan_set_step_fusion",
+           "sem_plan_get_step_fusion", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
 SEM_STREAM_AUTO, SEM_STREAM_PLAIN, SEM_STREAM_NONTEMPORAL = 0, 1, 2
@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
     L.sem_plan_get_schedule.argtypes = [vp, ip, ip, ip]
     L.sem_plan_schedule_auto.argtypes = [vp, i, i, sz]
     L.sem_plan_set_streaming.argtypes = [vp, i]
+    L.sem_plan_set_step_fusion.argtypes = [vp, i]
+    L.sem_plan_get_step_fusion.argtypes = [vp, ip]
     L.sem_forward.argtypes = [vp, C.POINTER(SemBatch), vp, sz, vp]
     L.sem_forward_adjoint.argtypes = [vp, C.POINTER(SemBatch), vp, d, vp, vp, vp, vp, sz, vp]
     L.sem_bspline_map.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]

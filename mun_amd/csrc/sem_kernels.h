@@ -67,6 +67,18 @@ __device__ __forceinline__ const GMEM T* gm(const T* p) {
     return (const GMEM T*)p;
 }
 
+// a * b + c rounded once.  The element action, the update and the receiver
+// sums are written with explicit fused multiply-adds and with automatic
+// contraction off (`#pragma clang fp contract(off)` at the top of each), so
+// that every kernel that shares them rounds identically, whatever the
+// surrounding control flow lets the compiler fuse: one- and two-step launches
+// give the same bits.
+template <typename T>
+__device__ __forceinline__ T fmadd(T a, T b, T c) {
+    if constexpr (std::is_same_v<T, float>) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
 // y = M x for a 5 x 5 matrix with M[a][b] = -M[4-a][4-b] (the GLL
 // differentiation matrix, and D^T scaled by the symmetric weights, are of this
 // form): with e_b = x_b + x_{4-b}, o_b = x_b - x_{4-b},
@@ -82,15 +94,34 @@ struct Fold5 {
 
 template <typename T>
 __device__ __forceinline__ void apply5(const Fold5<T>& f, T x0, T x1, T x2, T x3, T x4, T (&y)[5]) {
+#pragma clang fp contract(off)
     const T e0 = x0 + x4, o0 = x0 - x4, e1 = x1 + x3, o1 = x1 - x3;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-        const T S = f.s[a][0] * o0 + f.s[a][1] * o1;
-        const T D = f.t[a][0] * e0 + f.t[a][1] * e1 + f.t[a][2] * x2;
+        const T S = fmadd(f.s[a][1], o1, f.s[a][0] * o0);
+        const T D = fmadd(f.t[a][2], x2, fmadd(f.t[a][1], e1, f.t[a][0] * e0));
         y[a] = S + D;
         y[4 - a] = S - D;
     }
-    y[2] = f.m[0] * o0 + f.m[1] * o1;
+    y[2] = fmadd(f.m[1], o1, f.m[0] * o0);
+}
+
+// apply5 of the products x_b c_b, with the even / odd sums formed by fused
+// multiply-adds (x_0 c_0 + x_4 c_4 rounded once per sum).
+template <typename T>
+__device__ __forceinline__ void apply5_scaled(const Fold5<T>& f, const T (&x)[5], const T (&c)[5], T (&y)[5]) {
+#pragma clang fp contract(off)
+    const T p4 = x[4] * c[4], p3 = x[3] * c[3], x2 = x[2] * c[2];
+    const T e0 = fmadd(x[0], c[0], p4), o0 = fmadd(x[0], c[0], -p4);
+    const T e1 = fmadd(x[1], c[1], p3), o1 = fmadd(x[1], c[1], -p3);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const T S = fmadd(f.s[a][1], o1, f.s[a][0] * o0);
+        const T D = fmadd(f.t[a][2], x2, fmadd(f.t[a][1], e1, f.t[a][0] * e0));
+        y[a] = S + D;
+        y[4 - a] = S - D;
+    }
+    y[2] = fmadd(f.m[1], o1, f.m[0] * o0);
 }
 
 template <typename T, int P>
@@ -148,6 +179,14 @@ struct StepArgs {
     int m;                  // step index within the slots' current segments
     unsigned phase_mask;    // slots advanced by this launch (bit 1 << SEM_SLOT_*)
     int post;               // record mode: 1 = sample after the update
+    // two-step launches (step2_kernel): u^{n+1} (null: not stored), the
+    // increment's second buffer (read a.w, write w_out), and the receiver
+    // table and tile counts of the two-step tiling
+    T* u_mid;
+    T* w_out;
+    const int* rec_ptr2;
+    const RecEntry<T, P>* recs2;
+    int ntx2, ntiles2, total2, nper2;
 };
 
 // Per-workgroup pointers and indices of one solve at one step.
@@ -360,6 +399,7 @@ __device__ __forceinline__ void load4nt(const GMEM T* __restrict__ p, T (&v)[4])
 template <typename T, int P>
 __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const T (&u)[P + 1][P + 1],
                                                const T (&c2)[P + 1][P + 1], T (&r)[P + 1][P + 1]) {
+#pragma clang fp contract(off)
     static_assert(P == 4, "apply5 is the 5-node operator");
     constexpr int NP = P + 1;
     // x direction, row by row
@@ -367,22 +407,20 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
     for (int j = 0; j < NP; ++j) {
         T d[NP], s[NP];
         apply5(ec.fD, u[j][0], u[j][1], u[j][2], u[j][3], u[j][4], d);
-#pragma unroll
-        for (int q = 0; q < NP; ++q) d[q] *= c2[j][q];
-        apply5(ec.fDwT, d[0], d[1], d[2], d[3], d[4], s);
+        apply5_scaled(ec.fDwT, d, c2[j], s);
 #pragma unroll
         for (int i = 0; i < NP; ++i) r[j][i] = ec.ax[j] * s[i];
     }
     // z direction, column by column
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-        T d[NP], s[NP];
+        T d[NP], s[NP], cc[NP];
         apply5(ec.fD, u[0][i], u[1][i], u[2][i], u[3][i], u[4][i], d);
 #pragma unroll
-        for (int q = 0; q < NP; ++q) d[q] *= c2[q][i];
-        apply5(ec.fDwT, d[0], d[1], d[2], d[3], d[4], s);
+        for (int q = 0; q < NP; ++q) cc[q] = c2[q][i];
+        apply5_scaled(ec.fDwT, d, cc, s);
 #pragma unroll
-        for (int j = 0; j < NP; ++j) r[j][i] += ec.az[i] * s[j];
+        for (int j = 0; j < NP; ++j) r[j][i] = fmadd(ec.az[i], s[j], r[j][i]);
     }
 }
 
@@ -396,6 +434,7 @@ template <typename T, int P, int MODE, int EZ, bool NTS>
 __global__ __launch_bounds__(tile_threads(EZ))
 __attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
 void step_kernel(const StepArgs<T, P> a) {
+#pragma clang fp contract(off)
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
@@ -437,8 +476,8 @@ void step_kernel(const StepArgs<T, P> a) {
             for (int j = 0; j < NP; ++j) {
                 T row = T(0);
 #pragma unroll
-                for (int i = 0; i < NP; ++i) row += e.lx[i] * uin[j * pitch + i];
-                acc += e.lz[j] * row;
+                for (int i = 0; i < NP; ++i) row = fmadd(e.lx[i], uin[j * pitch + i], row);
+                acc = fmadd(e.lz[j], row, acc);
             }
             V.seis[(long)V.rec_t * a.rec_pitch + e.r] = acc;
         }
@@ -504,8 +543,8 @@ void step_kernel(const StepArgs<T, P> a) {
                         for (int j = 0; j < NP; ++j) {
                             T row = T(0);
 #pragma unroll
-                            for (int i = 0; i < NP; ++i) row += e.lx[i] * u[j][i];
-                            acc += e.lz[j] * row;
+                            for (int i = 0; i < NP; ++i) row = fmadd(e.lx[i], u[j][i], row);
+                            acc = fmadd(e.lz[j], row, acc);
                         }
                         V.seis[(long)V.rec_t * a.rec_pitch + e.r] = acc;
                     }
@@ -526,7 +565,7 @@ void step_kernel(const StepArgs<T, P> a) {
 #pragma unroll
                         for (int j = 0; j < NP; ++j)
 #pragma unroll
-                            for (int i = 0; i < NP; ++i) r[j][i] -= (e.lz[j] * e.lx[i]) * res;
+                            for (int i = 0; i < NP; ++i) r[j][i] = fmadd(-(e.lz[j] * e.lx[i]), res, r[j][i]);
                     }
                 }
             }
@@ -579,13 +618,13 @@ void step_kernel(const StepArgs<T, P> a) {
             return dxi * dzj;
         };
         auto update = [&](int j, int i, T wold) -> T {  // returns w^{n+1}; leaves u^{n+1} in r[j][i]
-            T av = -((a.ec.dt2 * inv_wz(j)) * inv_wx(i)) * r[j][i];
+            T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
             if constexpr (MODE == MODE_FWD) {
-                if (j * NP + i == s_loc) av += sterm;
+                if (j * NP + i == s_loc) ws += sterm;
             }
             const T Dn = damp_at(j, i);
-            const T wn = Dn * (wold + av);
-            r[j][i] = Dn * u[j][i] + wn;
+            const T wn = Dn * ws;
+            r[j][i] = fmadd(Dn, u[j][i], wn);
             return wn;
         };
 #pragma unroll
@@ -628,6 +667,322 @@ void step_kernel(const StepArgs<T, P> a) {
                             V.ck_u[base + o] = r[j][i];
                         }
                     }
+                }
+        }
+    }
+}
+
+// Two steps per launch (n -> n+1 -> n+2) on tiles of EX2 x EZ2 element
+// threads: the owned block O of TX2 x TZ2 elements, two halo columns on the
+// left and one on the right, two halo rows below and one above.  Step one
+// runs the element action on the whole block and updates the nodes of every
+// element but the first column and row (O and the ring around it: the nodes
+// step two's elements touch); their u^{n+1} stay in LDS (U1), and the
+// increment of O's nodes too (W1).  Step two runs the element action on O plus
+// its left and lower ring, reading u^{n+1} from LDS, and updates O's nodes.
+// Element math, assembly order and update are those of step_kernel, so a
+// two-step launch gives the same bits as two single steps.  The halo reads
+// w^n of neighbouring tiles' nodes, so the increment is read from a.w and
+// written to a.w_out (two buffers).  u^{n+1} of O's nodes is written to
+// a.u_mid when it is not null (the forward history, the adjoint ring), with
+// non-temporal stores: no launch reads it back (the gradient correlation does,
+// later), so it is kept out of the last-level cache that holds the state the
+// next launch reads.  Batch mode only (no slot table); FP32 and FP64.
+constexpr int EX2 = 19;                 // element threads along x (16 owned: 256-byte rows)
+constexpr int EZ2 = 26;                 // element threads along z (23 owned)
+constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
+constexpr int T2_THREADS = (EX2 * EZ2 + 63) / 64 * 64;
+constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;  // level-(n+1) nodes of the ring + O
+constexpr int U1_H = 4 * (EZ2 - 1) + 1;
+constexpr int W1_W = (4 * TX2 + 1 + 3) / 4 * 4;        // increment of O's nodes
+constexpr int W1_H = 4 * TZ2 + 1;
+
+template <typename T, int P, int MODE, bool NTS>
+__global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> a) {
+#pragma clang fp contract(off)
+    static_assert(P == 4, "vector loads assume 5 nodes per element edge");
+    static_assert(MODE == MODE_FWD || MODE == MODE_ADJS, "forward or adjoint steps");
+    constexpr int NP = P + 1;
+    constexpr bool ADJ = (MODE == MODE_ADJS);
+    __shared__ T xr_col[EZ2][EX2][NP];
+    __shared__ T xr_row[EZ2][EX2][P];
+    __shared__ __attribute__((aligned(16))) T U1[U1_H][U1_W];
+    __shared__ __attribute__((aligned(16))) T W1[W1_H][W1_W];
+
+    const int wgid = blockIdx.x;
+    const int lin = (wgid % NXCD) * a.nper2 + wgid / NXCD;
+    if (lin >= a.total2) return;  // whole workgroup leaves together
+    const int tile = lin % a.ntiles2;
+    const long b = lin / a.ntiles2;
+
+    const int t = threadIdx.x;
+    const int exl = t % EX2, ezl = t / EX2;
+    const int tx = tile % a.ntx2, tz = tile / a.ntx2;
+    const int ex = tx * TX2 - 2 + exl;
+    const int ez = tz * TZ2 - 2 + ezl;
+    const bool inb = t < EX2 * EZ2;
+    const bool valid = inb && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
+    const bool n1 = valid && exl >= 1 && ezl >= 1;                                      // updates level n+1
+    const bool r2 = valid && exl >= 1 && exl <= EX2 - 2 && ezl >= 1 && ezl <= EZ2 - 2;  // step-two element
+    const bool own = valid && exl >= 2 && exl <= EX2 - 2 && ezl >= 2 && ezl <= EZ2 - 2; // updates level n+2
+    const int pitch = a.pitch;
+    const int base = (P * ez) * pitch + P * ex;
+    const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
+
+    // solve b's arrays (uniform per workgroup)
+    const long soff = b * a.sb;
+    const long mdl = a.model_of ? a.model_of[b] : b;
+    const GMEM T* __restrict__ u_in = gm(a.u_in + soff);
+    GMEM T* __restrict__ u_out = gm(a.u_out + soff);
+    GMEM T* __restrict__ u_mid = a.u_mid ? gm(a.u_mid + soff) : nullptr;
+    const GMEM T* __restrict__ w_in = gm((const T*)a.w + soff);
+    GMEM T* __restrict__ w_out = gm(a.w_out + soff);
+    const GMEM T* __restrict__ c2p = gm(a.c2 + mdl * a.sb);
+    GMEM T* __restrict__ seis = a.seis ? gm(a.seis + b * a.seis_bs) : nullptr;
+    const GMEM T* __restrict__ resid = a.resid ? gm(a.resid + b * a.resid_bs) : nullptr;
+    const GMEM T* __restrict__ wav = (MODE == MODE_FWD && a.src_node) ? gm(a.wavelet + b * a.wav_bs + a.n) : nullptr;
+    const int src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
+    const int set = a.rec_set_of ? a.rec_set_of[b] : 0;
+    const int rbeg = a.rec_ptr2[set * (a.ntiles2 + 1) + tile];
+    const int rend = a.rec_ptr2[set * (a.ntiles2 + 1) + tile + 1];
+
+    // every lane reads an in-range block (clamped) so that the cross-lane
+    // fifth-node reads see their neighbours' data
+    const int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
+    const int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+    const int base_c = (P * ezc) * pitch + P * exc;
+    const bool from_right = exl < EX2 - 1 && (t & 63) != 63 && ex + 1 < a.nex;
+
+    auto inv_wx = [&](int i) -> T { return (i == 0 && ex > 0) ? a.ec.iw_shared : a.ec.iw[i]; };
+    auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
+    T dxv[P], dzv[P];
+    auto damp_at = [&](int j, int i) -> T {
+        if (!a.damp) return T(1);
+        const T dxi = (i < P) ? dxv[i] : gm(a.Dx)[P * ex + i];
+        const T dzj = (j < P) ? dzv[j] : gm(a.Dz)[P * ez + j];
+        return dxi * dzj;
+    };
+    // source node of this element (owned nodes incl. domain edges), else -1
+    auto source_slot = [&]() -> int {
+        if (src < 0) return -1;
+        const int sz = src / a.NX, sx = src - sz * a.NX;
+        const int lj = sz - P * ez, li = sx - P * ex;
+        const int jm = last_z ? P : P - 1, im = last_x ? P : P - 1;
+        return (lj >= 0 && lj <= jm && li >= 0 && li <= im) ? lj * NP + li : -1;
+    };
+    // assemble the element's owned nodes: own contribution + left / below / below-left
+    auto assemble = [&](T (&r)[NP][NP]) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                if (i == 0) r[j][i] += xr_col[ezl][exl - 1][j];
+                if (j == 0) r[j][i] += (i < P) ? xr_row[ezl - 1][exl][i] : xr_col[ezl - 1][exl][P];
+                if (i == 0 && j == 0) r[j][i] += xr_col[ezl - 1][exl - 1][P];
+            }
+    };
+    auto publish = [&](const T (&r)[NP][NP]) {
+        if (inb) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) xr_col[ezl][exl][j] = r[j][P];
+#pragma unroll
+            for (int i = 0; i < P; ++i) xr_row[ezl][exl][i] = r[P][i];
+        }
+    };
+    // seismogram sample R u of this element's block at time index rt
+    auto sample = [&](const T (&u)[NP][NP], int rt) {
+        if (rt < 0 || !seis) return;
+        for (int k = rbeg; k < rend; ++k) {
+            const RecEntry<T, P>& e = a.recs2[k];
+            if (e.owned && e.ex == ex && e.ez == ez) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    T row = T(0);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) row = fmadd(e.lx[i], u[j][i], row);
+                    acc = fmadd(e.lz[j], row, acc);
+                }
+                seis[(long)rt * a.rec_pitch + e.r] = acc;
+            }
+        }
+    };
+    // adjoint source R^T res at time index rt into the element's contributions
+    auto inject = [&](T (&r)[NP][NP], int rt) {
+        for (int k = rbeg; k < rend; ++k) {
+            const RecEntry<T, P>& e = a.recs2[k];
+            if (e.ex == ex && e.ez == ez) {
+                const T res = resid[(long)rt * a.rec_pitch + e.r] * a.invJ;
+#pragma unroll
+                for (int j = 0; j < NP; ++j)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) r[j][i] = fmadd(-(e.lz[j] * e.lx[i]), res, r[j][i]);
+            }
+        }
+    };
+
+    // ---- step one: level n -> n+1 on the whole block ----
+    {
+        T u[NP][NP], r[NP][NP];
+        T wv[P][P];
+        {
+            T c2[NP][NP];
+            fetch_block(u_in, (unsigned)base_c, (unsigned)pitch, valid && !from_right, u);
+            fetch_block(c2p, (unsigned)base_c, (unsigned)pitch, valid && !from_right, c2);
+            finish_block(from_right, u);
+            finish_block(from_right, c2);
+            element_action<T, P>(a.ec, u, c2, r);
+        }
+        if (!valid) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) r[j][i] = T(0);
+        }
+        if constexpr (MODE == MODE_FWD) {
+            if (own) sample(u, a.rec_t);
+        }
+        if (n1) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const GMEM T* wq = at(w_in, (unsigned)(base + j * pitch));
+                if constexpr (NTS) load4nt(wq, wv[j]);
+                else load4(wq, wv[j]);
+            }
+            if (a.damp) {
+                load4(gm(a.Dx) + P * ex, dxv);
+                load4(gm(a.Dz) + P * ez, dzv);
+            }
+        }
+        if constexpr (ADJ) {
+            if (valid) inject(r, a.res_t);
+        }
+        publish(r);
+        __syncthreads();
+        if (n1) {
+            assemble(r);
+            const int s_loc = source_slot();
+            const T sterm = (s_loc >= 0) ? a.ec.dt2 * wav[0] : T(0);
+            auto update = [&](int j, int i, T wold, T uold) -> T {  // w^{n+1}; u^{n+1} left in r[j][i]
+                T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
+                if constexpr (MODE == MODE_FWD) {
+                    if (j * NP + i == s_loc) ws += sterm;
+                }
+                const T Dn = damp_at(j, i);
+                const T wn = Dn * ws;
+                r[j][i] = fmadd(Dn, uold, wn);
+                return wn;
+            };
+            const int X0 = P * (exl - 1), Z0 = P * (ezl - 1);     // in U1
+            const int X0w = P * (exl - 2), Z0w = P * (ezl - 2);   // in W1 (own only)
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                T wn[4], un[4];
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    wn[i] = update(j, i, wv[j][i], u[j][i]);
+                    un[i] = r[j][i];
+                }
+#pragma unroll
+                for (int i = 0; i < P; ++i) U1[Z0 + j][X0 + i] = un[i];
+                if (own) {
+#pragma unroll
+                    for (int i = 0; i < P; ++i) W1[Z0w + j][X0w + i] = wn[i];
+                    if (u_mid) store4nt(at(u_mid, (unsigned)(base + j * pitch)), un);
+                }
+            }
+            if (last_x || last_z) {
+#pragma unroll
+                for (int j = 0; j < NP; ++j)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        if (!((i == P && last_x && (j < P || last_z)) || (j == P && last_z && i < P))) continue;
+                        const int o = j * pitch + i;
+                        const T wn = update(j, i, w_in[base + o], u[j][i]);
+                        U1[Z0 + j][X0 + i] = r[j][i];
+                        if (own) {
+                            W1[Z0w + j][X0w + i] = wn;
+                            if (u_mid) __builtin_nontemporal_store(r[j][i], &u_mid[base + o]);
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- step two: level n+1 -> n+2 on O and its left / lower ring ----
+    {
+        T u[NP][NP], r[NP][NP];
+        {
+            // u^{n+1} block from LDS (lanes outside the step-two block read a
+            // clamped in-range block and contribute zero)
+            const int xl = exl < 1 ? 0 : (exl > EX2 - 2 ? EX2 - 2 : exl - 1);
+            const int zl = ezl < 1 ? 0 : (ezl > EZ2 - 2 ? EZ2 - 2 : ezl - 1);
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) u[j][i] = U1[P * zl + j][P * xl + i];
+            T c2[NP][NP];
+            fetch_block(c2p, (unsigned)base_c, (unsigned)pitch, valid && !from_right, c2);
+            finish_block(from_right, c2);
+            element_action<T, P>(a.ec, u, c2, r);
+        }
+        if (!r2) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) r[j][i] = T(0);
+        }
+        if constexpr (MODE == MODE_FWD) {
+            if (own) sample(u, a.rec_t + 1);
+        }
+        if constexpr (ADJ) {
+            if (r2) inject(r, a.res_t - 1);
+        }
+        publish(r);
+        __syncthreads();
+        if (!own) return;
+        assemble(r);
+        const int s_loc = source_slot();
+        const T sterm = (s_loc >= 0) ? a.ec.dt2 * wav[1] : T(0);
+        const int X0w = P * (exl - 2), Z0w = P * (ezl - 2);
+        auto update = [&](int j, int i, T wold, T uold) -> T {
+            T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
+            if constexpr (MODE == MODE_FWD) {
+                if (j * NP + i == s_loc) ws += sterm;
+            }
+            const T Dn = damp_at(j, i);
+            const T wn = Dn * ws;
+            r[j][i] = fmadd(Dn, uold, wn);
+            return wn;
+        };
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            T wn[4], un[4];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                wn[i] = update(j, i, W1[Z0w + j][X0w + i], u[j][i]);
+                un[i] = r[j][i];
+            }
+            const unsigned o = (unsigned)(base + j * pitch);
+            if constexpr (NTS) {
+                store4nt(at(w_out, o), wn);
+                store4nt(at(u_out, o), un);
+            } else {
+                store4(at(w_out, o), wn);
+                store4(at(u_out, o), un);
+            }
+        }
+        if (last_x || last_z) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    if (!((i == P && last_x && (j < P || last_z)) || (j == P && last_z && i < P))) continue;
+                    const int o = j * pitch + i;
+                    const T wn = update(j, i, W1[Z0w + j][X0w + i], u[j][i]);
+                    w_out[base + o] = wn;
+                    u_out[base + o] = r[j][i];
                 }
         }
     }

@@ -252,6 +252,11 @@ struct sem_plan {
     DevBuf d_recs, d_rec_ptr;
     bool rec_dirty = true;
     size_t rec_up_entries = 0, rec_up_ptrs = 0;  // prefix already on the device
+    // two-step tiling (step2_kernel) and its receiver table
+    int ntx2 = 0, ntz2 = 0, ntiles2 = 0;
+    DevBuf d_recs2, d_rec_ptr2;
+    size_t rec2_up_entries = 0, rec2_up_ptrs = 0;
+    int fusion = 2;  // time steps per step-kernel launch where the batch path allows two (sem_plan_set_step_fusion)
     std::map<std::pair<int, int>, BsplineTables*> bsp;
     // schedule of sem_forward_adjoint (sem_plan_set_schedule): groups of at
     // most sched_group solves (0: the whole batch), sched_lanes in flight
@@ -288,32 +293,32 @@ int upload_tables(sem_plan* pl) {
     return SEM_OK;
 }
 
+// Receiver entries grouped by tile for one tiling: tiles of tx_own x tz_own
+// owned elements whose element block reaches lo elements to the left / below
+// and hi to the right / above; `owned` marks the tile that owns the element.
 template <typename T>
-int upload_receivers(sem_plan* pl) {
-    std::vector<RecEntry<T, 4>> flat;
-    std::vector<int> ptr;
+void build_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int ntz, int lo, int hi,
+                     std::vector<RecEntry<T, 4>>& flat, std::vector<int>& ptr) {
+    const int ntiles = ntx * ntz;
     for (auto& s : pl->sets) {
-        // entries of a set are stored grouped by tile: rebuild the tile CSR
-        std::vector<std::vector<const RecHost*>> by_tile(pl->ntiles);
+        std::vector<std::vector<const RecHost*>> by_tile(ntiles);
         for (auto& e : s) {
-            // tiles whose extended element block [tx*TX-1, tx*TX+TX-1] holds e
-            const int TZ = pl->tz;
-            for (int tz = std::max(0, e.ez / TZ - 1); tz <= std::min(pl->ntz - 1, e.ez / TZ + 1); ++tz)
-                for (int tx = std::max(0, e.ex / TX - 1); tx <= std::min(pl->ntx - 1, e.ex / TX + 1); ++tx) {
-                    if (e.ex < tx * TX - 1 || e.ex > tx * TX + TX - 1) continue;
-                    if (e.ez < tz * TZ - 1 || e.ez > tz * TZ + TZ - 1) continue;
-                    by_tile[tz * pl->ntx + tx].push_back(&e);
+            for (int tz = std::max(0, e.ez / tz_own - 2); tz <= std::min(ntz - 1, e.ez / tz_own + 2); ++tz)
+                for (int tx = std::max(0, e.ex / tx_own - 2); tx <= std::min(ntx - 1, e.ex / tx_own + 2); ++tx) {
+                    if (e.ex < tx * tx_own - lo || e.ex > tx * tx_own + tx_own - 1 + hi) continue;
+                    if (e.ez < tz * tz_own - lo || e.ez > tz * tz_own + tz_own - 1 + hi) continue;
+                    by_tile[tz * ntx + tx].push_back(&e);
                 }
         }
-        for (int t = 0; t < pl->ntiles; ++t) {
+        for (int t = 0; t < ntiles; ++t) {
             ptr.push_back((int)flat.size());
-            const int tx = t % pl->ntx, tz = t / pl->ntx;
+            const int tx = t % ntx, tz = t / ntx;
             for (const RecHost* e : by_tile[t]) {
                 RecEntry<T, 4> r;
                 r.ex = e->ex;
                 r.ez = e->ez;
                 r.r = e->r;
-                r.owned = (e->ex / TX == tx && e->ez / pl->tz == tz) ? 1 : 0;
+                r.owned = (e->ex / tx_own == tx && e->ez / tz_own == tz) ? 1 : 0;
                 for (int q = 0; q < 5; ++q) {
                     r.lx[q] = (T)e->lx[q];
                     r.lz[q] = (T)e->lz[q];
@@ -323,24 +328,44 @@ int upload_receivers(sem_plan* pl) {
         }
         ptr.push_back((int)flat.size());
     }
-    // Sets are only ever appended, so the device prefix stays valid: upload the
-    // new suffix in place (kernels in flight keep reading unchanged bytes).  Only
-    // growth past the reserved capacity reallocates.
+}
+
+// Sets are only ever appended, so a table's device prefix stays valid: upload
+// the new suffix in place (kernels in flight keep reading unchanged bytes).
+// Only growth past the reserved capacity reallocates.
+template <typename T>
+int upload_rec_table(DevBuf& d_recs, DevBuf& d_ptr, size_t& up_entries, size_t& up_ptrs,
+                     const std::vector<RecEntry<T, 4>>& flat, const std::vector<int>& ptr) {
     const size_t eb = sizeof(RecEntry<T, 4>);
     int rc;
     bool m1 = false, m2 = false;
-    if ((rc = pl->d_recs.reserve(std::max<size_t>(flat.size() * eb, 16), m1))) return rc;
-    if ((rc = pl->d_rec_ptr.reserve(ptr.size() * sizeof(int), m2))) return rc;
-    if (m1 || m2) pl->rec_up_entries = pl->rec_up_ptrs = 0;
-    if (m1 && !m2) pl->rec_up_ptrs = 0;
-    if ((rc = pl->d_recs.write_at(pl->rec_up_entries * eb, flat.data() + pl->rec_up_entries,
-                                  (flat.size() - pl->rec_up_entries) * eb)))
+    if ((rc = d_recs.reserve(std::max<size_t>(flat.size() * eb, 16), m1))) return rc;
+    if ((rc = d_ptr.reserve(ptr.size() * sizeof(int), m2))) return rc;
+    if (m1 || m2) up_entries = up_ptrs = 0;
+    if ((rc = d_recs.write_at(up_entries * eb, flat.data() + up_entries, (flat.size() - up_entries) * eb)))
         return rc;
-    if ((rc = pl->d_rec_ptr.write_at(pl->rec_up_ptrs * sizeof(int), ptr.data() + pl->rec_up_ptrs,
-                                     (ptr.size() - pl->rec_up_ptrs) * sizeof(int))))
+    if ((rc = d_ptr.write_at(up_ptrs * sizeof(int), ptr.data() + up_ptrs, (ptr.size() - up_ptrs) * sizeof(int))))
         return rc;
-    pl->rec_up_entries = flat.size();
-    pl->rec_up_ptrs = ptr.size();
+    up_entries = flat.size();
+    up_ptrs = ptr.size();
+    return SEM_OK;
+}
+
+template <typename T>
+int upload_receivers(sem_plan* pl) {
+    // single-step tiles: one halo column / row on the left and below
+    std::vector<RecEntry<T, 4>> flat;
+    std::vector<int> ptr;
+    build_rec_table<T>(pl, TX, pl->tz, pl->ntx, pl->ntz, 1, 0, flat, ptr);
+    int rc;
+    if ((rc = upload_rec_table<T>(pl->d_recs, pl->d_rec_ptr, pl->rec_up_entries, pl->rec_up_ptrs, flat, ptr)))
+        return rc;
+    // two-step tiles: two halo columns / rows on the left and below, one on the right and above
+    std::vector<RecEntry<T, 4>> flat2;
+    std::vector<int> ptr2;
+    build_rec_table<T>(pl, TX2, TZ2, pl->ntx2, pl->ntz2, 2, 1, flat2, ptr2);
+    if ((rc = upload_rec_table<T>(pl->d_recs2, pl->d_rec_ptr2, pl->rec2_up_entries, pl->rec2_up_ptrs, flat2, ptr2)))
+        return rc;
     pl->rec_dirty = false;
     return SEM_OK;
 }
@@ -538,6 +563,12 @@ StepArgs<T, 4> base_args(const sem_plan* pl, const sem_batch* bt, const void* c2
     a.rec_set_of = rset;
     a.rec_ptr = (const int*)pl->d_rec_ptr.p;
     a.recs = (const RecEntry<T, 4>*)pl->d_recs.p;
+    a.rec_ptr2 = (const int*)pl->d_rec_ptr2.p;
+    a.recs2 = (const RecEntry<T, 4>*)pl->d_recs2.p;
+    a.ntx2 = pl->ntx2;
+    a.ntiles2 = pl->ntiles2;
+    a.total2 = pl->ntiles2 * bt->batch;
+    a.nper2 = (a.total2 + NXCD - 1) / NXCD;
     a.seis = (T*)bt->seis;
     a.seis_bs = (long)pl->d.nt * bt->rec_pitch;
     a.rec_pitch = bt->rec_pitch;
@@ -564,6 +595,19 @@ template <typename T, int MODE>
 int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
     if constexpr (MODE == MODE_REC) return launch_step_nt<T, MODE, true>(pl, a, st);  // no state streams
     return nts ? launch_step_nt<T, MODE, true>(pl, a, st) : launch_step_nt<T, MODE, false>(pl, a, st);
+}
+
+// Two steps (n, n+1) per launch: u_in -> u_mid -> u_out, increment a.w -> a.w_out.
+template <typename T, int MODE>
+int launch_step2(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
+    (void)pl;
+    if (!a.rec_ptr2 || !a.w_out || a.w_out == a.w || a.u_out == a.u_in || a.u_mid == a.u_in || a.slots)
+        return fail(SEM_E_INVALID, "two-step launch with aliased or missing buffers");
+    const unsigned grid = (unsigned)a.nper2 * NXCD;
+    if (nts) step2_kernel<T, 4, MODE, true><<<grid, T2_THREADS, 0, st>>>(a);
+    else step2_kernel<T, 4, MODE, false><<<grid, T2_THREADS, 0, st>>>(a);
+    LAUNCH_CHECK();
+    return SEM_OK;
 }
 
 template <typename T>
@@ -711,7 +755,24 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     const bool post = pl->d.record_mode == SEM_RECORD_POST;
     a.w = W;
     const T* last = nullptr;
+    // two steps per launch under the full history (the increment alternates
+    // between W and U0, which the full history leaves unused)
+    const bool two = pl->fusion == 2 && adj && L.full_history && !snap;
     for (int n = 0; n < nt; ++n) {
+        if (two && n + 1 < nt) {
+            a.u_in = SEG + (long)n * state;
+            a.u_mid = SEG + (long)(n + 1) * state;
+            a.u_out = SEG + (long)(n + 2) * state;
+            a.w_out = (a.w == W) ? U0 : W;
+            a.n = n;
+            a.rec_t = post ? n - 1 : n;
+            a.ck_u = a.ck_w = nullptr;
+            if ((rc = launch_step2<T, MODE_FWD>(pl, a, st, g.nts))) return rc;
+            a.w = a.w_out;
+            last = a.u_out;
+            ++n;
+            continue;
+        }
         if (adj && L.full_history) {
             a.u_in = SEG + (long)n * state;
             a.u_out = SEG + (long)(n + 1) * state;
@@ -788,6 +849,7 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
     // gradient correlation of the chunk
     T* RING = (T*)(ws + L.RING);
     T* OM = (T*)(ws + L.OM);
+    T* OM2 = (T*)(ws + L.U1);  // second increment buffer of two-step launches
     double* G = (double*)(ws + L.G);
     const int ring_slots = L.C + 1;
     auto ring_slot = [&](int t) { return RING + (long)((t + 1) % ring_slots) * state; };  // lambda_t
@@ -836,8 +898,19 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
             const int c0 = ce - clen;
             for (int it = ce - 1; it >= c0; --it) {
                 a.u_in = ring_slot(it);
-                a.u_out = ring_slot(it - 1);
                 a.res_t = it;
+                if (pl->fusion == 2 && it - 1 >= c0 && ring_slots >= 3) {
+                    // steps it and it - 1 in one launch (the increment alternates
+                    // between OM and U1, idle while the adjoint runs)
+                    a.u_mid = ring_slot(it - 1);
+                    a.u_out = ring_slot(it - 2);
+                    a.w_out = (a.w == OM) ? OM2 : OM;
+                    if ((rc = launch_step2<T, MODE_ADJS>(pl, a, st, g.nts))) return rc;
+                    a.w = a.w_out;
+                    --it;
+                    continue;
+                }
+                a.u_out = ring_slot(it - 1);
                 if ((rc = launch_step<T, MODE_ADJS>(pl, a, st, g.nts))) return rc;
             }
             c.uf = SEG + (long)(c0 + 1 - useg0) * state;  // u^{c0+1}
@@ -1042,6 +1115,9 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) try {
     pl->ntx = (d->nelem_x + TX - 1) / TX;
     pl->ntz = (d->nelem_z + pl->tz - 1) / pl->tz;
     pl->ntiles = pl->ntx * pl->ntz;
+    pl->ntx2 = (d->nelem_x + TX2 - 1) / TX2;
+    pl->ntz2 = (d->nelem_z + TZ2 - 1) / TZ2;
+    pl->ntiles2 = pl->ntx2 * pl->ntz2;
     pl->ctiles = (int)(((long)d->nelem_x * d->nelem_z + CORR_W - 1) / CORR_W);
     pl->nexp = (d->nelem_x + 31) / 32 * 32;
     pl->gplane = (long)d->nelem_z * pl->nexp;
@@ -1186,6 +1262,18 @@ int sem_plan_set_streaming(sem_plan* pl, int mode) try {
     if (!pl || mode < SEM_STREAM_AUTO || mode > SEM_STREAM_NONTEMPORAL)
         return fail(SEM_E_INVALID, "sem_plan_set_streaming: bad argument");
     pl->streaming = mode;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_set_step_fusion(sem_plan* pl, int steps) try {
+    if (!pl || steps < 1 || steps > 2) return fail(SEM_E_INVALID, "sem_plan_set_step_fusion: steps must be 1 or 2");
+    pl->fusion = steps;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_get_step_fusion(const sem_plan* pl, int* steps) try {
+    if (!pl || !steps) return fail(SEM_E_INVALID, "sem_plan_get_step_fusion: bad argument");
+    *steps = pl->fusion;
     return SEM_OK;
 } ABI_CATCH
 

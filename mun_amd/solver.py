@@ -240,6 +240,17 @@ class Plan:
         code = {"auto": L.SEM_STREAM_AUTO, "plain": L.SEM_STREAM_PLAIN, "nontemporal": L.SEM_STREAM_NONTEMPORAL}[mode]
         L.check(self.lib.sem_plan_set_streaming(self.h, code))
 
+    def set_step_fusion(self, steps: int):
+        """Time steps per step-kernel launch of forward_adjoint: 2 (the
+        default; applies to the full-history forward and the adjoint passes)
+        or 1.  Results do not depend on it."""
+        L.check(self.lib.sem_plan_set_step_fusion(self.h, int(steps)))
+
+    def get_step_fusion(self) -> int:
+        v = C.c_int()
+        L.check(self.lib.sem_plan_get_step_fusion(self.h, C.byref(v)))
+        return v.value
+
     def get_schedule(self):
         ck, g, ln = C.c_int(), C.c_int(), C.c_int()
         L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))

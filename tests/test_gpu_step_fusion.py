@@ -1,0 +1,106 @@
+"""Two time steps per step-kernel launch (sem_plan_set_step_fusion(plan, 2))
+give the same bits as one step per launch: seismograms, misfit and gradient,
+on a mesh of several ragged two-step tiles with the source and receivers on
+tile boundaries, an odd step count, both precisions, both adjoint modes,
+both recording conventions, and the full-history, lane and checkpoint
+schedules."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tests.gpu_helpers import make_plan, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(nt=263):
+    # 37 x 50 elements: 3 x 3 two-step tiles of 16 x 23 owned elements (the
+    # last column and row partial); tile corners at x = 400 m, z = 575 m
+    nex, nez = 37, 50
+    L_x, L_z = 25.0 * nex, 25.0 * nez
+    xs = np.linspace(30.0, L_x - 30.0, 9)
+    zs = np.linspace(30.0, L_z - 30.0, 7)
+    rec = np.concatenate([np.stack([xs, np.full(xs.size, 575.0)], axis=1),     # along a tile row boundary
+                          np.stack([np.full(zs.size, 400.0), zs], axis=1),     # along a tile column boundary
+                          np.array([[399.0, 574.0], [800.0, 1150.0], [L_x, L_z], [0.0, 0.0]])])
+    prob = O.Problem(0.0, L_x, 0.0, L_z, nex, nez, 4, 1e-3, nt, (400.0, 575.0), 12.0, rec,
+                     sponge_width=120.0, sponge_cmax=3000.0, sponge_sides=("xmin", "xmax", "zmax"))
+    disc = O.discretize(prob)
+    rng = np.random.default_rng(3)
+    coef = 1800.0 + 900.0 * rng.random((6, 7))
+    c, _, _ = O.tensor_bspline_velocity(disc.mesh, coef)
+    return prob, disc, coef, c
+
+
+@pytest.fixture(scope="module")
+def fusion_case():
+    prob, disc, coef, c = _problem()
+    seis, _ = O.forward(disc, c, record="post", history=None)
+    return prob, disc, coef, c, seis
+
+
+def _inputs(prob, disc, coef, c, plan, nsolve=5):
+    rs = plan.add_receivers(prob.rec_xz)
+    src = plan.source_node(*prob.src_xz)
+    assert src == disc.src_node
+    w = torch.as_tensor(disc.wavelet, dtype=plan.dtype, device=plan.device)
+    rng = np.random.default_rng(9)
+    models = [c] + [O.tensor_bspline_velocity(disc.mesh, coef + 80.0 * rng.standard_normal(coef.shape))[0]
+                    for _ in range(2)]
+    C2 = torch.as_tensor(np.stack(models) ** 2, dtype=plan.dtype, device=plan.device)
+    src2 = disc.mesh.find_closest_node(130.0, 990.0)
+    srcs = [src2 if k % 2 else src for k in range(nsolve)]
+    model_of = [k % 3 for k in range(nsolve)]
+    return C2, w, srcs, model_of, rs
+
+
+@pytest.mark.parametrize("precision,mode,record", [("fp32", "reference", "post"), ("fp32", "exact", "pre"),
+                                                   ("fp64", "exact", "post"), ("fp64", "reference", "pre")])
+def test_two_steps_per_launch_give_the_same_bits(fusion_case, precision, mode, record):
+    prob, disc, coef, c, seis = fusion_case
+    print(f"[fusion] {precision} {mode} {record}", flush=True)
+    plan = make_plan(prob, precision, adjoint_mode=mode, record=record, checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    assert plan.get_step_fusion() == 2  # the default
+    for sch in [(0, 0, 1), (0, 2, 3), (40, 0, 1), (33, 2, 2)]:
+        plan.set_schedule(*sch)
+        outs = []
+        for steps in (1, 2):
+            plan.set_step_fusion(steps)
+            o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+            torch.cuda.synchronize()
+            outs.append({k: o[k].cpu() for k in ("seis", "misfit", "grad_c")})
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(outs[0][k], outs[1][k]), (sch, k)
+        assert torch.isfinite(outs[1]["grad_c"]).all() and outs[1]["grad_c"].abs().max() > 0
+
+
+def test_two_steps_per_launch_match_the_oracle(fusion_case):
+    """Solve 0 (the fixture's model and source) under two-step launches
+    against the FP64 oracle's exact-mode gradient and post-update seismogram."""
+    prob, disc, coef, c, seis = fusion_case
+    print("[fusion] oracle", flush=True)
+    plan = make_plan(prob, "fp64", adjoint_mode="exact", record="post", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan, nsolve=3)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    plan.set_schedule(0, 0, 1)
+    plan.set_step_fusion(2)
+    o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+    torch.cuda.synchronize()
+    ref = O.forward_adjoint(disc, c, seis * 0.9, 1e-9, mode="exact")
+    assert rel_l2(o["seis"][0].cpu().numpy(), seis) < 1e-9
+    assert rel_l2(o["grad_c"][0].cpu().numpy(), ref["dloglik_dc"]) < 1e-9
+
+
+def test_step_fusion_rejects_other_counts():
+    from mun_amd._lib import SemError
+    prob, disc, coef, c = _problem(nt=20)
+    plan = make_plan(prob, "fp32")
+    for bad in (0, 3, -1):
+        with pytest.raises(SemError):
+            plan.set_step_fusion(bad)
+    assert plan.get_step_fusion() == 2

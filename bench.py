@@ -8,8 +8,9 @@ element, degree-4 mesh (801 x 801 GLL nodes, 4 km x 4 km), 5-layer background
 FP64 gradient accumulation.
 
 The plan picks its schedule from free device memory (Plan.schedule_auto):
-on one MI355X the 256 solves run as groups of 4-5 that keep their whole FP32
-forward history (no recompute), three groups in flight on separate streams.
+on one MI355X the 256 solves run as groups of 5-6 that keep their whole FP32
+forward history (no recompute), three groups in flight on separate streams,
+two time steps per step-kernel launch.
 
 Printed beside it: the plain path (one solve per launch sequence, batch 1),
 the live roofline of the time-step kernel (HIP events on the launch stream,
@@ -34,14 +35,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FWD_BYTES_PER_NODE = 20  # u^n, w^n, c^2 read; u^{n+1}, w^{n+1} written (FP32, increment form)
-# one forward+adjoint solve step with the full forward history: forward 20 B,
-# adjoint 20 B, correlation 8 B (u^{k+1}, lambda_k read), FP64 accumulator
-# (25 doubles per element of 16 nodes = 12.5 B per node) read and written
-# once per chunk of 32 steps
-SOLVE_BYTES_PER_NODE_STEP = 20 + 20 + 8 + 25 / 32
-STD_BYTES_PER_NODE = 16  # the standard two-level form's minimum (u^n, u^{n-1}, c^2 read; u^{n+1} written),
-                         # which misses the FP32 tolerance at this size
+# per launch of the forward step kernel, FP32 increment form: u^n, w^n, c^2
+# read, the new u and w written -- one step per launch (step_kernel), or two
+# (step2_kernel: u^{n+2}, w^{n+2} written, u^{n+1} kept in LDS)
+FWD_BYTES_PER_NODE = 20
+# one forward+adjoint solve step with the full forward history: forward and
+# adjoint 20 B each with one step per launch; with two per launch each pair
+# reads 12 B and writes 12 B (u^{n+1} and u^{n+2} into the history or the
+# adjoint ring, w^{n+2}), 12 B per step; correlation 8 B (u^{k+1}, lambda_k
+# read), FP64 accumulator (25 doubles per element of 16 nodes = 12.5 B per
+# node) read and written once per chunk of 32 steps
+SOLVE_BYTES_PER_NODE_STEP = {1: 20 + 20 + 8 + 25 / 32, 2: 12 + 12 + 8 + 25 / 32}
 
 
 def log(msg):
@@ -212,28 +216,41 @@ def main():
         sched_txt = ("whole batch at once, " + ("full forward history" if sched["checkpoint_every"] <= 0
                                                 else f"checkpoints every {sched['checkpoint_every']} steps"))
     log(f"[bench] schedule: {sched_txt}")
-    path_gbs = SOLVE_BYTES_PER_NODE_STEP * m.N * m.nt * nsolve * args.steps / dt_total / 1e9
+    fusion = plan.get_step_fusion()
+    path_gbs = SOLVE_BYTES_PER_NODE_STEP[fusion] * m.N * m.nt * nsolve * args.steps / dt_total / 1e9
 
     # live roofline of the time-step kernel, launched on as many solves as the
-    # schedule keeps in flight: a forward-only pass is nt launches of the step
-    # kernel (plus one receiver pass), timed with events on the stream the
-    # library launches on
+    # schedule keeps in flight: a forward-only pass is nt / steps-per-launch
+    # launches of the step kernel (plus one receiver pass), timed with events
+    # on the stream the library launches on; measured for the plan's launch
+    # form (two steps per launch by default) and for one step per launch
     _, c2 = solver.velocity(coef)
     src, model_of = solver._expand(args.batch)
     src, model_of, c2 = src[:inflight], model_of[:inflight], c2[:inflight]  # one shot: model k = solve k
     st = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
-    e0.record(st)
-    plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
-    e1.record(st)
-    torch.cuda.synchronize()
-    t_launch = e0.elapsed_time(e1) / 1e3 / m.nt
+    nts = inflight * 4 * plan.layout()[1] * 4 > (256 << 20)  # the library's choice (LLC_BYTES)
+
+    def time_forward(steps):
+        plan.set_step_fusion(steps)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+        e0.record(st)
+        plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+        e1.record(st)
+        torch.cuda.synchronize()
+        launches = -(-m.nt // steps)
+        return e0.elapsed_time(e1) / 1e3 / launches
+
+    t_single = time_forward(1)
+    t_launch = time_forward(fusion)
+    plan.set_step_fusion(fusion)
     bytes_launch = FWD_BYTES_PER_NODE * m.N * inflight
     achieved = bytes_launch / t_launch / 1e9
-    nts = inflight * 4 * plan.layout()[1] * 4 > (256 << 20)  # the library's choice (LLC_BYTES)
-    log(f"[bench] step kernel on {inflight} solves: {t_launch * 1e6:.1f} us/launch, {achieved:.0f} GB/s; "
-        f"whole solve {path_gbs:.0f} GB/s")
+    kernel = (f"step2_kernel<float,4,MODE_FWD,NTS={str(nts).lower()}>" if fusion == 2
+              else f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>")
+    log(f"[bench] step kernel on {inflight} solves, {fusion} step(s) per launch: {t_launch * 1e6:.1f} us/launch "
+        f"({t_launch / fusion * 1e6:.1f} us/step), {achieved:.0f} GB/s; one step per launch: "
+        f"{t_single * 1e6:.1f} us/step, {bytes_launch / t_single / 1e9:.0f} GB/s; whole solve {path_gbs:.0f} GB/s")
     # bytes per launch across L2 from the committed counter runs: exact when
     # they were taken at this launch size, else their measured ratio to the
     # algorithmic bytes applied to this launch (noted in traffic_basis)
@@ -243,6 +260,8 @@ def main():
         with open(tp) as f:
             tj = json.load(f)
         fw = tj.get("forward", {})
+        if fw.get("kernel", "").split("<")[0].split("::")[-1] != kernel.split("<")[0]:
+            fw, tj = {}, {}
         if tj.get("batch") == inflight:
             traffic, traffic_basis = tj.get("bytes_per_launch"), f"counters at {inflight} solves per launch"
         elif fw.get("ratio"):
@@ -261,12 +280,17 @@ def main():
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "traffic_basis": traffic_basis,
-                        "kernel": f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>",
+                        "kernel": kernel, "steps_per_launch": fusion,
                         "solves_per_launch": inflight, "launch_us": round(t_launch * 1e6, 1),
+                        "us_per_step": round(t_launch / fusion * 1e6, 2),
                         "algorithmic_bytes_per_launch": bytes_launch, "bytes_per_node": FWD_BYTES_PER_NODE,
-                        "achieved_16B_basis": round(STD_BYTES_PER_NODE * m.N * inflight / t_launch / 1e9, 1),
+                        "one_step_per_launch": {
+                            "kernel": f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>",
+                            "launch_us": round(t_single * 1e6, 1),
+                            "achieved": round(bytes_launch / t_single / 1e9, 1),
+                            "frac": round(bytes_launch / t_single / 1e9 / HBM_PEAK_GBS, 4)},
                         "whole_solve_achieved": round(path_gbs, 1),
-                        "whole_solve_bytes_per_node_step": SOLVE_BYTES_PER_NODE_STEP},
+                        "whole_solve_bytes_per_node_step": SOLVE_BYTES_PER_NODE_STEP[fusion]},
            "latency_s": {"p50": round(ms_per_step / 1e3, 3), "p99": round(ms_per_step / 1e3, 3),
                          "note": "batched path: every solve completes with its batch"}}
 

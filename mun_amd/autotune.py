@@ -7,7 +7,9 @@ row; short tiles give more workgroups, which small meshes and small batches
 need to fill 256 CUs.  ``best_tile_rows`` times the forward step of every
 instance on the actual mesh and batch size (HIP events around a run of
 launches on the library's stream), keeps the fastest and caches the choice in
-a JSON file keyed by device, mesh, batch and precision.
+a JSON file keyed by device, mesh, batch and precision.  The tuned height
+applies to one-step launches (checkpointed forward, recompute, slot launches,
+snapshots); two-step launches use a fixed 19 x 26-element tile.
 """
 from __future__ import annotations
 
@@ -57,12 +59,14 @@ def _key(nelem_x, nelem_z, batch, precision, device) -> str:
 def measure(nelem_x: int, nelem_z: int, batch: int, tile_rows: int, precision: str = "fp32", steps: int = 30,
             device=None) -> float:
     """Mean forward-step time (seconds) of one tile shape on a synthetic
-    constant-velocity batch."""
+    constant-velocity batch, one step per launch (the tile height is the
+    one-step kernel's; two-step launches use a fixed tile)."""
     from .solver import Plan
     dev = torch.device(device if device is not None else "cuda")
     L = 25.0
     p = Plan(nelem_x, nelem_z, 0.0, L * nelem_x, 0.0, L * nelem_z, 2e-4, steps, 4, 4 * L, 3000.0, 1e-8,
              ("xmin", "xmax", "zmax"), precision=precision, checkpoint_every=None, device=dev, tile_rows=tile_rows)
+    p.set_step_fusion(1)
     rs = p.add_receivers(np.array([[L, 0.0]]))
     c2 = torch.full((batch, p.N), 2000.0 ** 2, dtype=p.dtype, device=dev)
     w = torch.zeros(steps, dtype=p.dtype, device=dev)

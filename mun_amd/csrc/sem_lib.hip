@@ -402,7 +402,7 @@ struct Layout {
     bool full_history = false;
     int S = 0, nseg = 0;
     int C = 0;  // adjoint steps per gradient-correlation chunk
-    size_t U0, U1, W, CKU, CKW, SEG, RING, OM, RES, G, total;
+    size_t U0, U1, W, W2, CKU, CKW, SEG, RING, OM, RES, G, total;
 };
 
 // Correlation chunk: at most 32 adjoint steps, dividing a segment of S steps
@@ -433,6 +433,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj, int ckpt) {
     L.U0 = take(state);
     L.U1 = take(state);
     L.W = take(state);
+    if (!adj) L.W2 = take(state);  // second increment buffer of two-step launches
     if (adj) {
         if (!L.full_history) {
             L.CKU = take(state * L.nseg);
@@ -756,29 +757,37 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     a.w = W;
     const T* last = nullptr;
     // two steps per launch under the full history (the increment alternates
-    // between W and U0, which the full history leaves unused)
-    const bool two = pl->fusion == 2 && adj && L.full_history && !snap;
+    // between W and U0, which the full history leaves unused) and in a
+    // forward-only pass (u between U0 and U1, the increment between W and W2);
+    // not with checkpoints or snapshots, which take every step's state
+    const bool hist = adj && L.full_history;
+    const bool two = pl->fusion == 2 && !snap && (hist || !adj);
+    T* W2 = hist ? U0 : (T*)(ws + L.W2);
+    T* ucur = U0;  // u^n outside the full history (ping-pong)
+    T* unext = U1;
     for (int n = 0; n < nt; ++n) {
         if (two && n + 1 < nt) {
-            a.u_in = SEG + (long)n * state;
-            a.u_mid = SEG + (long)(n + 1) * state;
-            a.u_out = SEG + (long)(n + 2) * state;
-            a.w_out = (a.w == W) ? U0 : W;
+            a.u_in = hist ? SEG + (long)n * state : ucur;
+            a.u_mid = hist ? SEG + (long)(n + 1) * state : nullptr;
+            a.u_out = hist ? SEG + (long)(n + 2) * state : unext;
+            a.w_out = (a.w == W) ? W2 : W;
             a.n = n;
             a.rec_t = post ? n - 1 : n;
             a.ck_u = a.ck_w = nullptr;
             if ((rc = launch_step2<T, MODE_FWD>(pl, a, st, g.nts))) return rc;
             a.w = a.w_out;
             last = a.u_out;
+            std::swap(ucur, unext);
             ++n;
             continue;
         }
-        if (adj && L.full_history) {
+        if (hist) {
             a.u_in = SEG + (long)n * state;
             a.u_out = SEG + (long)(n + 1) * state;
         } else {
-            a.u_in = (n & 1) ? U1 : U0;
-            a.u_out = (n & 1) ? U0 : U1;
+            a.u_in = ucur;
+            a.u_out = unext;
+            std::swap(ucur, unext);
         }
         a.n = n;
         a.rec_t = post ? n - 1 : n;

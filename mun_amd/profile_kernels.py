@@ -24,15 +24,21 @@ from . import workloads as W
 from .batch import BatchSolver
 
 
-def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles_per_solve=195, threads=256):
+def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles_per_solve=195, threads=256,
+                  tiles_per_solve2=117, threads2=512):
     """Bytes per step-kernel launch from two rocprofv3 counter passes
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
     counter reads for the calibration copy (the runtime's copyBuffer kernel
-    moving copy_mb MiB), against the step kernel's 20 algorithmic bytes per
-    node (12 read, 8 written).  Solves per launch follow from each dispatch's
-    grid (tiles_per_solve workgroups of `threads` per solve; 195 tiles of the
-    15-row kernel on the 801 x 801-node mesh).  `forward` / `adjoint` report
-    the launches of `batch` solves (else the largest), `kernels` all."""
+    moving copy_mb MiB), against the launch's algorithmic bytes per node:
+    one step per launch (step_kernel) reads 12 and writes 8; two steps per
+    launch (step2_kernel) read 12 and write 8 in the forward-only pass (the
+    launches of `batch` solves) and 12 where u^{n+1} is stored too (the
+    full-history forward of the lane groups, the adjoint).  Solves per launch
+    follow from each dispatch's grid: tiles_per_solve workgroups of `threads`
+    (195 tiles of the 15-row step kernel on the 801 x 801-node mesh), or
+    tiles_per_solve2 of threads2 for the two-step kernel (117 tiles).
+    `forward` / `adjoint` report the launches of `batch` solves (else the
+    largest), preferring the two-step kernel; `kernels` all."""
     import csv
     import statistics
 
@@ -48,7 +54,7 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles
     copy_bytes = copy_mb << 20
     out = {"batch": batch, "nodes": nodes}
     recs = {}
-    for counter, path, alg in (("FETCH_SIZE", fetch_csv, 12), ("WRITE_SIZE", write_csv, 8)):
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         vals = per_kernel(path, counter)
         # calibration: the largest copyBuffer dispatches (the copy_mb MiB copies)
         copies = [v for (n, g), vs in vals.items() if "copyBuffer" in n for v in vs]
@@ -57,24 +63,29 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles
         key = counter.split("_")[0].lower()
         out[f"{key}_factor"] = round(factor, 4)
         for (name, grid), vs in sorted(vals.items()):
-            if not name.startswith("sem::step_kernel") or ", 2, " in name:
+            two = name.startswith("sem::step2_kernel")
+            if not (two or name.startswith("sem::step_kernel")) or ", 2, " in name:
                 continue
             mode = "forward" if ", 0, " in name else "adjoint"
-            solves = max(1, round(grid / threads / tiles_per_solve))
-            rec = recs.setdefault((mode, solves), {"mode": mode, "kernel": name, "grid": grid, "solves": solves,
-                                                   "dispatches": len(vs)})
+            solves = max(1, round(grid / (threads2 * tiles_per_solve2 if two else threads * tiles_per_solve)))
+            steps = 2 if two else 1
+            wr = 12 if two and (mode == "adjoint" or solves != batch) else 8
+            rec = recs.setdefault((mode, steps, solves), {"mode": mode, "kernel": name, "steps_per_launch": steps,
+                                                          "grid": grid, "solves": solves, "dispatches": len(vs)})
             rec[f"{key}_kib_raw_median"] = statistics.median(vs)
             rec[f"{key}_bytes"] = statistics.median(vs) * 1024.0 * factor
-            rec[f"{key}_algorithmic_bytes"] = alg * nodes * solves
+            rec[f"{key}_algorithmic_bytes"] = (12 if key == "fetch" else wr) * nodes * solves
     for rec in recs.values():
         if "fetch_bytes" in rec and "write_bytes" in rec:
             rec["bytes_per_launch"] = int(rec["fetch_bytes"] + rec["write_bytes"])
-            rec["algorithmic_bytes_per_launch"] = 20 * nodes * rec["solves"]
+            rec["algorithmic_bytes_per_launch"] = rec["fetch_algorithmic_bytes"] + rec["write_algorithmic_bytes"]
             rec["ratio"] = round(rec["bytes_per_launch"] / rec["algorithmic_bytes_per_launch"], 4)
-    out["kernels"] = sorted(recs.values(), key=lambda r: (r["mode"], r["solves"]))
+    out["kernels"] = sorted(recs.values(), key=lambda r: (r["mode"], -r["steps_per_launch"], r["solves"]))
     for mode in ("forward", "adjoint"):
         mine = [r for r in out["kernels"] if r["mode"] == mode and "ratio" in r]
         if mine:
+            top = max(r["steps_per_launch"] for r in mine)
+            mine = [r for r in mine if r["steps_per_launch"] == top]
             pick = [r for r in mine if r["solves"] == batch] or [max(mine, key=lambda r: r["solves"])]
             out[mode] = pick[0]
     if "forward" in out:

@@ -78,6 +78,24 @@ def test_two_steps_per_launch_give_the_same_bits(fusion_case, precision, mode, r
         assert torch.isfinite(outs[1]["grad_c"]).all() and outs[1]["grad_c"].abs().max() > 0
 
 
+@pytest.mark.parametrize("precision,record", [("fp32", "post"), ("fp64", "pre")])
+def test_forward_only_two_steps_give_the_same_bits(fusion_case, precision, record):
+    """The forward-only pass (ping-pong states) under one- and two-step
+    launches, the odd last step included."""
+    prob, disc, coef, c, seis = fusion_case
+    print(f"[fusion] forward {precision} {record}", flush=True)
+    plan = make_plan(prob, precision, record=record)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan)
+    outs = []
+    for steps in (1, 2):
+        plan.set_step_fusion(steps)
+        outs.append(plan.forward(C2, w, srcs, [rs] * len(srcs), model_of=model_of).cpu())
+    assert torch.equal(outs[0], outs[1])
+    if precision == "fp64" and record == "pre":
+        ref, _ = O.forward(disc, c, record="pre", history=None)
+        assert rel_l2(outs[1][0, :, :ref.shape[1]].numpy(), ref) < 1e-9
+
+
 def test_two_steps_per_launch_match_the_oracle(fusion_case):
     """Solve 0 (the fixture's model and source) under two-step launches
     against the FP64 oracle's exact-mode gradient and post-update seismogram."""

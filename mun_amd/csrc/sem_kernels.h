@@ -184,6 +184,7 @@ struct StepArgs {
     // table and tile counts of the two-step tiling
     T* u_mid;
     T* w_out;
+    T* u_copy;  // when set: u^{n+2} also here (plain stores), u_out then non-temporal (full-history forward)
     const int* rec_ptr2;
     const RecEntry<T, P>* recs2;
     int ntx2, ntiles2, total2, nper2;
@@ -735,6 +736,7 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
     const GMEM T* __restrict__ u_in = gm(a.u_in + soff);
     GMEM T* __restrict__ u_out = gm(a.u_out + soff);
     GMEM T* __restrict__ u_mid = a.u_mid ? gm(a.u_mid + soff) : nullptr;
+    GMEM T* __restrict__ u_copy = a.u_copy ? gm(a.u_copy + soff) : nullptr;
     const GMEM T* __restrict__ w_in = gm((const T*)a.w + soff);
     GMEM T* __restrict__ w_out = gm(a.w_out + soff);
     const GMEM T* __restrict__ c2p = gm(a.c2 + mdl * a.sb);
@@ -968,9 +970,15 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
             if constexpr (NTS) {
                 store4nt(at(w_out, o), wn);
                 store4nt(at(u_out, o), un);
+                if (u_copy) store4nt(at(u_copy, o), un);
             } else {
                 store4(at(w_out, o), wn);
-                store4(at(u_out, o), un);
+                if (u_copy) {
+                    store4nt(at(u_out, o), un);
+                    store4(at(u_copy, o), un);
+                } else {
+                    store4(at(u_out, o), un);
+                }
             }
         }
         if (last_x || last_z) {
@@ -983,6 +991,7 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                     const T wn = update(j, i, W1[Z0w + j][X0w + i], u[j][i]);
                     w_out[base + o] = wn;
                     u_out[base + o] = r[j][i];
+                    if (u_copy) u_copy[base + o] = r[j][i];
                 }
         }
     }

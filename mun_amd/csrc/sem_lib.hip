@@ -433,7 +433,10 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj, int ckpt) {
     L.U0 = take(state);
     L.U1 = take(state);
     L.W = take(state);
-    if (!adj) L.W2 = take(state);  // second increment buffer of two-step launches
+    // forward-only: the second increment buffer of two-step launches; full
+    // history: the second buffer of the state copies the forward's two-step
+    // launches read (with U1)
+    if (!adj || L.full_history) L.W2 = take(state);
     if (adj) {
         if (!L.full_history) {
             L.CKU = take(state * L.nseg);
@@ -727,6 +730,24 @@ Group make_group(const sem_plan* pl, const sem_batch* bt, char* ws, const BatchL
     return g;
 }
 
+// The state copies of the full-history forward's two-step launches: each
+// launch also writes its new state (plain stores) to one buffer of a pair,
+// which the next launch reads, and writes the history slot itself
+// non-temporally, so that the write-once history stays out of the last-level
+// cache (the same for the adjoint's ring measured no faster).  `cur` is null
+// until a launch has written a copy.
+template <typename T>
+struct CopyPair {
+    T* cur;
+    T* a;
+    T* b;
+    void next_launch(StepArgs<T, 4>& s) {
+        if (cur) s.u_in = cur;
+        s.u_copy = (cur == a) ? b : a;
+        cur = s.u_copy;
+    }
+};
+
 // Forward time loop of a group.  Checkpoint mode: ping-pong U0/U1 with (u, w)
 // mirrored into checkpoint k after step k*S-1.  Full-history mode: u^n in SEG
 // slot n.
@@ -765,11 +786,16 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     T* W2 = hist ? U0 : (T*)(ws + L.W2);
     T* ucur = U0;  // u^n outside the full history (ping-pong)
     T* unext = U1;
+    // full history: u^{n+2} also goes to a ping-pong pair (U1, L.W2) that the
+    // next launch reads, so that every history slot is written non-temporally
+    CopyPair<T> cp{nullptr, U1, (T*)(ws + L.W2)};
     for (int n = 0; n < nt; ++n) {
         if (two && n + 1 < nt) {
             a.u_in = hist ? SEG + (long)n * state : ucur;
             a.u_mid = hist ? SEG + (long)(n + 1) * state : nullptr;
             a.u_out = hist ? SEG + (long)(n + 2) * state : unext;
+            a.u_copy = nullptr;
+            if (hist) cp.next_launch(a);
             a.w_out = (a.w == W) ? W2 : W;
             a.n = n;
             a.rec_t = post ? n - 1 : n;
@@ -784,6 +810,7 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
         if (hist) {
             a.u_in = SEG + (long)n * state;
             a.u_out = SEG + (long)(n + 1) * state;
+            a.u_copy = nullptr;
         } else {
             a.u_in = ucur;
             a.u_out = unext;

@@ -684,7 +684,8 @@ void step_kernel(const StepArgs<T, P> a) {
 // Element math, assembly order and update are those of step_kernel, so a
 // two-step launch gives the same bits as two single steps.  The halo reads
 // w^n of neighbouring tiles' nodes, so the increment is read from a.w and
-// written to a.w_out (two buffers).  u^{n+1} of O's nodes is written to
+// written to a.w_out (two buffers).  When a.ck_u is set, (u, w)^{n+2} also go
+// to the checkpoint a.ck_u / a.ck_w.  u^{n+1} of O's nodes is written to
 // a.u_mid when it is not null (the forward history, the adjoint ring), with
 // non-temporal stores: no launch reads it back (the gradient correlation does,
 // later), so it is kept out of the last-level cache that holds the state the
@@ -737,6 +738,8 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
     GMEM T* __restrict__ u_out = gm(a.u_out + soff);
     GMEM T* __restrict__ u_mid = a.u_mid ? gm(a.u_mid + soff) : nullptr;
     GMEM T* __restrict__ u_copy = a.u_copy ? gm(a.u_copy + soff) : nullptr;
+    GMEM T* __restrict__ ck_u = a.ck_u ? gm(a.ck_u + soff) : nullptr;  // checkpoint of (u, w)^{n+2}
+    GMEM T* __restrict__ ck_w = a.ck_w ? gm(a.ck_w + soff) : nullptr;
     const GMEM T* __restrict__ w_in = gm((const T*)a.w + soff);
     GMEM T* __restrict__ w_out = gm(a.w_out + soff);
     const GMEM T* __restrict__ c2p = gm(a.c2 + mdl * a.sb);
@@ -980,6 +983,10 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                     store4(at(u_out, o), un);
                 }
             }
+            if (ck_u) {
+                store4(at(ck_w, o), wn);
+                store4(at(ck_u, o), un);
+            }
         }
         if (last_x || last_z) {
 #pragma unroll
@@ -992,6 +999,10 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                     w_out[base + o] = wn;
                     u_out[base + o] = r[j][i];
                     if (u_copy) u_copy[base + o] = r[j][i];
+                    if (ck_u) {
+                        ck_w[base + o] = wn;
+                        ck_u[base + o] = r[j][i];
+                    }
                 }
         }
     }

@@ -433,10 +433,10 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj, int ckpt) {
     L.U0 = take(state);
     L.U1 = take(state);
     L.W = take(state);
-    // forward-only: the second increment buffer of two-step launches; full
-    // history: the second buffer of the state copies the forward's two-step
-    // launches read (with U1)
-    if (!adj || L.full_history) L.W2 = take(state);
+    // the second increment buffer of two-step launches in a forward-only or a
+    // checkpointed forward; under the full history the second buffer of the
+    // state copies the forward's two-step launches read (with U1)
+    L.W2 = take(state);
     if (adj) {
         if (!L.full_history) {
             L.CKU = take(state * L.nseg);
@@ -782,15 +782,18 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     // forward-only pass (u between U0 and U1, the increment between W and W2);
     // not with checkpoints or snapshots, which take every step's state
     const bool hist = adj && L.full_history;
-    const bool two = pl->fusion == 2 && !snap && (hist || !adj);
+    const bool two = pl->fusion == 2 && !snap;
     T* W2 = hist ? U0 : (T*)(ws + L.W2);
     T* ucur = U0;  // u^n outside the full history (ping-pong)
     T* unext = U1;
     // full history: u^{n+2} also goes to a ping-pong pair (U1, L.W2) that the
     // next launch reads, so that every history slot is written non-temporally
     CopyPair<T> cp{nullptr, U1, (T*)(ws + L.W2)};
+    // checkpointed forward: checkpoint k holds (u, w) after step k*S-1; a pair
+    // of steps may end on a checkpoint but not straddle one
+    const bool ckpt = adj && !L.full_history;
     for (int n = 0; n < nt; ++n) {
-        if (two && n + 1 < nt) {
+        if (two && n + 1 < nt && !(ckpt && (n + 1) % L.S == 0)) {
             a.u_in = hist ? SEG + (long)n * state : ucur;
             a.u_mid = hist ? SEG + (long)(n + 1) * state : nullptr;
             a.u_out = hist ? SEG + (long)(n + 2) * state : unext;
@@ -800,6 +803,11 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
             a.n = n;
             a.rec_t = post ? n - 1 : n;
             a.ck_u = a.ck_w = nullptr;
+            if (ckpt && (n + 2) % L.S == 0 && n + 2 < nt) {
+                const long k = (n + 2) / L.S;
+                a.ck_u = (T*)(ws + L.CKU) + k * state;
+                a.ck_w = (T*)(ws + L.CKW) + k * state;
+            }
             if ((rc = launch_step2<T, MODE_FWD>(pl, a, st, g.nts))) return rc;
             a.w = a.w_out;
             last = a.u_out;
@@ -898,6 +906,7 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
     if ((rc = fill<double>(G, 0.0, (long)B * pl->gsb, st))) return rc;
 
     StepArgs<T, 4> f = base_args<T>(pl, bt, g.c2p, g.rset);  // recompute steps
+    f.seis = nullptr;  // record nothing (a two-step launch samples rec_t and rec_t + 1)
     StepArgs<T, 4> a = base_args<T>(pl, bt, g.c2p, g.rset);  // adjoint steps
     a.w = OM;
     a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
@@ -920,11 +929,23 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
         if (!L.full_history) {
             // recompute u^{s0+1..s0+len} from checkpoint k into SEG[1..len]
             const T* slot0 = (T*)(ws + L.CKU) + (long)k * state;
-            f.w = (T*)(ws + L.CKW) + (long)k * state;
+            T* ckw = (T*)(ws + L.CKW) + (long)k * state;  // updated in place (its last use)
+            f.w = ckw;
             for (int m = 0; m < len; ++m) {
                 f.u_in = (m == 0) ? slot0 : SEG + (long)m * state;
-                f.u_out = SEG + (long)(m + 1) * state;
                 f.n = s0 + m;
+                if (pl->fusion == 2 && m + 1 < len) {
+                    // two steps per launch; the increment alternates between the
+                    // checkpoint's copy and U0 (idle while the adjoint runs)
+                    f.u_mid = SEG + (long)(m + 1) * state;
+                    f.u_out = SEG + (long)(m + 2) * state;
+                    f.w_out = (f.w == ckw) ? (T*)(ws + L.U0) : ckw;
+                    if ((rc = launch_step2<T, MODE_FWD>(pl, f, st, g.nts))) return rc;
+                    f.w = f.w_out;
+                    ++m;
+                    continue;
+                }
+                f.u_out = SEG + (long)(m + 1) * state;
                 if ((rc = launch_step<T, MODE_FWD>(pl, f, st, g.nts))) return rc;
             }
         }

@@ -158,12 +158,14 @@ int sem_plan_schedule_auto(sem_plan* plan, int batch, int rec_pitch, size_t budg
 #define SEM_STREAM_NONTEMPORAL 2
 int sem_plan_set_streaming(sem_plan* plan, int mode);
 
-/* Time steps per step-kernel launch in sem_forward_adjoint: 2 (default; a
- * two-step kernel keeps the intermediate field in LDS, so a pair of steps
- * reads the state once, and stores it with non-temporal writes), or 1.  Two-step launches cover the forward pass under the
- * full forward history and every adjoint pass; the other passes (checkpointed
- * forward, recompute, snapshots, slot launches) take one step per launch.
- * Results are bitwise the same for 1 and 2.  No reference counterpart (the
+/* Time steps per step-kernel launch in sem_forward and sem_forward_adjoint:
+ * 2 (default; a two-step kernel keeps the intermediate field in LDS, so a
+ * pair of steps reads the state once, and stores it with non-temporal
+ * writes), or 1.  Two-step launches cover the forward pass (full history,
+ * checkpointed or forward-only), the recompute and the adjoint; snapshots and
+ * slot launches take one step per launch, and so do the odd last step of a
+ * pass and a step pair that would straddle a checkpoint.  Results are
+ * bitwise the same for 1 and 2.  No reference counterpart (the
  * reference loops over single steps, core.py:1197-1275). */
 int sem_plan_set_step_fusion(sem_plan* plan, int steps);
 int sem_plan_get_step_fusion(const sem_plan* plan, int* steps);

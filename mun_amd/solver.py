@@ -241,9 +241,9 @@ class Plan:
         L.check(self.lib.sem_plan_set_streaming(self.h, code))
 
     def set_step_fusion(self, steps: int):
-        """Time steps per step-kernel launch of forward_adjoint: 2 (the
-        default; applies to the full-history forward and the adjoint passes)
-        or 1.  Results do not depend on it."""
+        """Time steps per step-kernel launch: 2 (the default; forward,
+        recompute and adjoint passes, not snapshots or slot launches) or 1.
+        Results do not depend on it."""
         L.check(self.lib.sem_plan_set_step_fusion(self.h, int(steps)))
 
     def get_step_fusion(self) -> int:

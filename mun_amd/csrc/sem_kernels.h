@@ -689,7 +689,10 @@ void step_kernel(const StepArgs<T, P> a) {
 // a.u_mid when it is not null (the forward history, the adjoint ring), with
 // non-temporal stores: no launch reads it back (the gradient correlation does,
 // later), so it is kept out of the last-level cache that holds the state the
-// next launch reads.  Batch mode only (no slot table); FP32 and FP64.
+// next launch reads.  When a.u_copy is set (the full-history forward),
+// u^{n+2} also goes there with plain stores, the next launch reads that copy,
+// and u_out (the history slot) is written non-temporally too.  Batch mode only
+// (no slot table); FP32 and FP64.
 constexpr int EX2 = 19;                 // element threads along x (16 owned: 256-byte rows)
 constexpr int EZ2 = 26;                 // element threads along z (23 owned)
 constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;

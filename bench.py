@@ -281,6 +281,10 @@ def main():
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "traffic_basis": traffic_basis,
                         "kernel": kernel, "steps_per_launch": fusion,
+                        "note": ("two time steps per launch for 20 algorithmic bytes per node (u, w, c^2 read once; "
+                                 "u^{n+1} kept in LDS): half the bytes per step of one step per launch and faster per "
+                                 "step (us_per_step vs one_step_per_launch.launch_us), so no longer bandwidth-bound"
+                                 if fusion == 2 else "one time step per launch"),
                         "solves_per_launch": inflight, "launch_us": round(t_launch * 1e6, 1),
                         "us_per_step": round(t_launch / fusion * 1e6, 2),
                         "algorithmic_bytes_per_launch": bytes_launch, "bytes_per_node": FWD_BYTES_PER_NODE,

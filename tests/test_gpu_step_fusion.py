@@ -78,6 +78,30 @@ def test_two_steps_per_launch_give_the_same_bits(fusion_case, precision, mode, r
         assert torch.isfinite(outs[1]["grad_c"]).all() and outs[1]["grad_c"].abs().max() > 0
 
 
+def test_two_steps_with_nontemporal_streams_give_the_same_bits(fusion_case):
+    """The non-temporal instances (NTS, chosen when a launch's working set
+    exceeds the last-level cache; forced here) under one- and two-step
+    launches, full-history and checkpointed schedules."""
+    prob, disc, coef, c, seis = fusion_case
+    print("[fusion] nontemporal", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="exact", record="post", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    plan.set_streaming("nontemporal")
+    for sch in [(0, 0, 1), (40, 2, 2)]:
+        plan.set_schedule(*sch)
+        outs = []
+        for steps in (1, 2):
+            plan.set_step_fusion(steps)
+            o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+            torch.cuda.synchronize()
+            outs.append({k: o[k].cpu() for k in ("seis", "misfit", "grad_c")})
+        for k in ("seis", "misfit", "grad_c"):
+            assert torch.equal(outs[0][k], outs[1][k]), (sch, k)
+    plan.set_streaming("auto")
+
+
 @pytest.mark.parametrize("precision,record", [("fp32", "post"), ("fp64", "pre")])
 def test_forward_only_two_steps_give_the_same_bits(fusion_case, precision, record):
     """The forward-only pass (ping-pong states) under one- and two-step

@@ -162,9 +162,10 @@ int sem_plan_set_streaming(sem_plan* plan, int mode);
  * 2 (default; a two-step kernel keeps the intermediate field in LDS, so a
  * pair of steps reads the state once, and stores it with non-temporal
  * writes), or 1.  Two-step launches cover the forward pass (full history,
- * checkpointed or forward-only), the recompute and the adjoint; snapshots and
- * slot launches take one step per launch, and so do the odd last step of a
- * pass and a step pair that would straddle a checkpoint.  Results are
+ * checkpointed, forward-only or with snapshots), the recompute and the
+ * adjoint; slot launches take one step per launch, and so do the odd last
+ * step of a pass and a step pair that would straddle a checkpoint or hold a
+ * snapshot in its middle.  Results are
  * bitwise the same for 1 and 2.  No reference counterpart (the
  * reference loops over single steps, core.py:1197-1275). */
 int sem_plan_set_step_fusion(sem_plan* plan, int steps);

@@ -782,7 +782,7 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     // forward-only pass (u between U0 and U1, the increment between W and W2);
     // not with checkpoints or snapshots, which take every step's state
     const bool hist = adj && L.full_history;
-    const bool two = pl->fusion == 2 && !snap;
+    const bool two = pl->fusion == 2;
     T* W2 = hist ? U0 : (T*)(ws + L.W2);
     T* ucur = U0;  // u^n outside the full history (ping-pong)
     T* unext = U1;
@@ -792,8 +792,19 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
     // checkpointed forward: checkpoint k holds (u, w) after step k*S-1; a pair
     // of steps may end on a checkpoint but not straddle one
     const bool ckpt = adj && !L.full_history;
+    // snapshot of u^{n+1} after step n when n % every == 0: a pair of steps
+    // may end on one (its output is the state) but not hold one in its middle
+    auto snapshot = [&](int n, const T* u) -> int {
+        const long per = (long)snap->nxr * snap->nzr;
+        const long nsnap = (nt + snap->every - 1) / snap->every;
+        dim3 grid((unsigned)std::min<long>((per + 255) / 256, 1024), B);
+        snapshot_kernel<T><<<grid, 256, 0, st>>>(u, pl->sb, pl->pitch, snap->ix0, snap->nxr, snap->iz0, snap->nzr,
+                                                 (T*)snap->out + (n / snap->every) * per, nsnap * per);
+        LAUNCH_CHECK();
+        return SEM_OK;
+    };
     for (int n = 0; n < nt; ++n) {
-        if (two && n + 1 < nt && !(ckpt && (n + 1) % L.S == 0)) {
+        if (two && n + 1 < nt && !(ckpt && (n + 1) % L.S == 0) && !(snap && n % snap->every == 0)) {
             a.u_in = hist ? SEG + (long)n * state : ucur;
             a.u_mid = hist ? SEG + (long)(n + 1) * state : nullptr;
             a.u_out = hist ? SEG + (long)(n + 2) * state : unext;
@@ -813,6 +824,7 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
             last = a.u_out;
             std::swap(ucur, unext);
             ++n;
+            if (snap && n % snap->every == 0 && (rc = snapshot(n, a.u_out))) return rc;
             continue;
         }
         if (hist) {
@@ -834,15 +846,7 @@ int forward_loop(sem_plan* pl, const Group& g, const Layout& L, bool adj, hipStr
         }
         if ((rc = launch_step<T, MODE_FWD>(pl, a, st, g.nts))) return rc;
         last = a.u_out;
-        if (snap && n % snap->every == 0) {
-            const long per = (long)snap->nxr * snap->nzr;
-            const long nsnap = (nt + snap->every - 1) / snap->every;
-            dim3 grid((unsigned)std::min<long>((per + 255) / 256, 1024), B);
-            snapshot_kernel<T><<<grid, 256, 0, st>>>(a.u_out, pl->sb, pl->pitch, snap->ix0, snap->nxr, snap->iz0,
-                                                     snap->nzr, (T*)snap->out + (n / snap->every) * per,
-                                                     nsnap * per);
-            LAUNCH_CHECK();
-        }
+        if (snap && n % snap->every == 0 && (rc = snapshot(n, a.u_out))) return rc;
     }
     if (post) {
         a.u_in = last;

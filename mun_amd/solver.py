@@ -242,7 +242,7 @@ class Plan:
 
     def set_step_fusion(self, steps: int):
         """Time steps per step-kernel launch: 2 (the default; forward,
-        recompute and adjoint passes, not snapshots or slot launches) or 1.
+        snapshot, recompute and adjoint passes, not slot launches) or 1.
         Results do not depend on it."""
         L.check(self.lib.sem_plan_set_step_fusion(self.h, int(steps)))
 

@@ -120,6 +120,24 @@ def test_forward_only_two_steps_give_the_same_bits(fusion_case, precision, recor
         assert rel_l2(outs[1][0, :, :ref.shape[1]].numpy(), ref) < 1e-9
 
 
+@pytest.mark.parametrize("every", [1, 2, 3])
+def test_snapshot_runs_with_two_steps_give_the_same_bits(fusion_case, every):
+    """Snapshot runs (a state every `every` steps, which a step pair may end
+    on but not hold in its middle) under one- and two-step launches."""
+    prob, disc, coef, c, seis = fusion_case
+    print(f"[fusion] snapshots every {every}", flush=True)
+    plan = make_plan(prob, "fp32", record="pre")
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan, nsolve=3)
+    box = (10, 120, 30, 170)
+    outs = []
+    for steps in (1, 2):
+        plan.set_step_fusion(steps)
+        sm, sn = plan.forward_snapshots(C2, w, srcs, every, box, [rs] * len(srcs), model_of=model_of)
+        outs.append((sm.cpu(), sn.cpu()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    assert outs[1][1].abs().max() > 0
+
+
 def test_two_steps_per_launch_match_the_oracle(fusion_case):
     """Solve 0 (the fixture's model and source) under two-step launches
     against the FP64 oracle's exact-mode gradient and post-update seismogram."""

@@ -317,6 +317,8 @@ typedef struct {
     void* ring;            /* adjoint: lambda_t in slot (t+1) % ring_slots, stride seg_stride */
     int ring_slots;        /* >= chunk_len + 1 of sem_slots_correlate                 */
     int reserved;
+    void* w2;              /* sem_slots_step2: second increment buffer (the increment
+                              alternates w -> w2 -> w ... launch by launch)           */
 } sem_slot;
 
 /* Padded nodal layout: row pitch and per-array element count. */
@@ -333,6 +335,16 @@ int sem_plan_gradient_elems(const sem_plan* plan, long* elems);
  * current segments; slots with m >= len skip.  rec_pitch: seismogram row length. */
 int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
                    int rec_pitch, void* stream);
+/* Steps m and m + 1 (m even) of every FORWARD_HIST and ADJOINT slot in
+ * phase_mask, in one launch per kernel (two-step kernel, the same bits as two
+ * sem_slots_step calls); a slot with m + 1 == len takes step m alone.  The
+ * increment is read from w and written to w2 on launches with (m / 2) even,
+ * the other way round on odd ones, so it ends a segment of len steps in w2
+ * when ceil(len / 2) is odd.  Slots of other phases are not advanced (step
+ * them with sem_slots_step and a mask without these phases).  No reference
+ * counterpart (continuous batching). */
+int sem_slots_step2(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
+                    int rec_pitch, void* stream);
 /* Gradient correlation of chunk `chunk` (segment steps chunk*chunk_len ..
  * min((chunk+1)*chunk_len, len) - 1) of every ADJOINT slot, added to its G in
  * FP64; enqueue after those adjoint steps and before the ring slots are reused

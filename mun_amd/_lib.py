@@ -27,7 +27,7 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
            "sem_slots_correlate", "sem_plan_gradient_elems", "sem_plan_set_schedule", "sem_plan_get_schedule",
            "sem_plan_schedule_auto", "sem_plan_set_streaming", "sem_plan_set_step_fusion",
-           "sem_plan_get_step_fusion", "sem_last_error"]
+           "sem_plan_get_step_fusion", "sem_slots_step2", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
 SEM_STREAM_AUTO, SEM_STREAM_PLAIN, SEM_STREAM_NONTEMPORAL = 0, 1, 2
@@ -67,7 +67,7 @@ class SemSlot(C.Structure):
                 ("w", C.c_void_p), ("c2", C.c_void_p), ("wavelet", C.c_void_p), ("seis", C.c_void_p),
                 ("ck_u", C.c_void_p), ("ck_w", C.c_void_p), ("seg", C.c_void_p), ("seg_stride", C.c_long),
                 ("resid", C.c_void_p), ("G", C.c_void_p), ("ring", C.c_void_p), ("ring_slots", C.c_int),
-                ("reserved", C.c_int)]
+                ("reserved", C.c_int), ("w2", C.c_void_p)]
 
 
 class SemError(RuntimeError):
@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
     L.sem_plan_layout.argtypes = [vp, ip, C.POINTER(C.c_long)]
     L.sem_plan_gradient_elems.argtypes = [vp, C.POINTER(C.c_long)]
     L.sem_slots_step.argtypes = [vp, vp, i, i, C.c_uint, i, vp]
+    L.sem_slots_step2.argtypes = [vp, vp, i, i, C.c_uint, i, vp]
     L.sem_slots_record_final.argtypes = [vp, vp, i, C.c_uint, i, vp]
     L.sem_slots_correlate.argtypes = [vp, vp, i, i, i, vp]
     L.sem_pad_nodal.argtypes = [vp, vp, vp, i, vp]

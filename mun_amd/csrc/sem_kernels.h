@@ -693,6 +693,97 @@ void step_kernel(const StepArgs<T, P> a) {
 // u^{n+2} also goes there with plain stores, the next launch reads that copy,
 // and u_out (the history slot) is written non-temporally too.  Batch mode only
 // (no slot table); FP32 and FP64.
+// Per-workgroup arrays and time indices of one solve in a two-step launch:
+// from the uniform batch arguments, or from the slot table (FORWARD_HIST and
+// ADJOINT slots; the increment alternates between the slot's w and w2 launch
+// by launch, and a slot whose segment ends after step m takes that step
+// alone: `single`).  Returns false if this solve does not advance.
+template <typename T>
+struct View2 {
+    const GMEM T* u_in;
+    GMEM T* u_out;
+    GMEM T* u_mid;
+    GMEM T* u_copy;
+    GMEM T* ck_u;
+    GMEM T* ck_w;
+    const GMEM T* w_in;
+    GMEM T* w_out;
+    const GMEM T* c2;
+    GMEM T* seis;
+    const GMEM T* resid;
+    const GMEM T* wav;
+    int src, set, rec_t, res_t;
+    bool single;
+};
+
+template <typename T, int P, int MODE>
+__device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View2<T>& v) {
+    v.u_copy = v.ck_u = v.ck_w = nullptr;
+    v.single = false;
+    if (!a.slots) {
+        const long soff = b * a.sb;
+        const long mdl = a.model_of ? a.model_of[b] : b;
+        v.u_in = gm(a.u_in + soff);
+        v.u_out = gm(a.u_out + soff);
+        v.u_mid = a.u_mid ? gm(a.u_mid + soff) : nullptr;
+        v.u_copy = a.u_copy ? gm(a.u_copy + soff) : nullptr;
+        v.ck_u = a.ck_u ? gm(a.ck_u + soff) : nullptr;
+        v.ck_w = a.ck_w ? gm(a.ck_w + soff) : nullptr;
+        v.w_in = gm((const T*)a.w + soff);
+        v.w_out = gm(a.w_out + soff);
+        v.c2 = gm(a.c2 + mdl * a.sb);
+        v.seis = a.seis ? gm(a.seis + b * a.seis_bs) : nullptr;
+        v.resid = a.resid ? gm(a.resid + b * a.resid_bs) : nullptr;
+        v.wav = (MODE == MODE_FWD && a.src_node) ? gm(a.wavelet + b * a.wav_bs + a.n) : nullptr;
+        v.src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
+        v.set = a.rec_set_of ? a.rec_set_of[b] : 0;
+        v.rec_t = a.rec_t;
+        v.res_t = a.res_t;
+        return true;
+    }
+    const sem_slot& d = a.slots[b];
+    if (!((a.phase_mask >> d.phase) & 1u) || a.m >= d.len) return false;
+    v.single = (a.m + 1 == d.len);
+    const bool flip = ((a.m >> 1) & 1) != 0;
+    v.w_in = gm((const T*)(flip ? d.w2 : d.w));
+    v.w_out = gm((T*)(flip ? d.w : d.w2));
+    v.c2 = gm((const T*)d.c2);
+    v.set = d.rec_set;
+    v.seis = nullptr;
+    v.resid = nullptr;
+    v.wav = nullptr;
+    v.src = -1;
+    v.rec_t = v.res_t = -1;
+    if (d.phase == SEM_SLOT_FORWARD_HIST) {
+        if (MODE != MODE_FWD) return false;
+        const int n = d.n0 + a.m;
+        const GMEM T* seg = gm((const T*)d.seg);
+        v.u_in = seg + (long)a.m * d.seg_stride;
+        v.u_mid = (GMEM T*)(seg + (long)(a.m + 1) * d.seg_stride);
+        v.u_out = (GMEM T*)(seg + (long)(a.m + 2) * d.seg_stride);  // not touched when single
+        v.wav = gm((const T*)d.wavelet + n);
+        v.src = d.src_node;
+        if (d.record) {
+            v.seis = gm((T*)d.seis);
+            v.rec_t = a.post ? n - 1 : n;
+        }
+        return true;
+    }
+    if (d.phase == SEM_SLOT_ADJOINT) {
+        if (MODE != MODE_ADJS) return false;
+        // lambda_t in ring slot (t + 1) % ring_slots (ring_slots >= 3)
+        const int it = d.n0 + d.len - 1 - a.m;
+        const GMEM T* ring = gm((const T*)d.ring);
+        v.u_in = ring + (long)((it + 1) % d.ring_slots) * d.seg_stride;
+        v.u_mid = (GMEM T*)(ring + (long)(it % d.ring_slots) * d.seg_stride);
+        v.u_out = (GMEM T*)(ring + (long)((it - 1 + d.ring_slots) % d.ring_slots) * d.seg_stride);
+        v.resid = gm((const T*)d.resid);
+        v.res_t = it;
+        return true;
+    }
+    return false;
+}
+
 constexpr int EX2 = 19;                 // element threads along x (16 owned: 256-byte rows)
 constexpr int EZ2 = 26;                 // element threads along z (23 owned)
 constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
@@ -734,23 +825,23 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
     const int base = (P * ez) * pitch + P * ex;
     const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
 
-    // solve b's arrays (uniform per workgroup)
-    const long soff = b * a.sb;
-    const long mdl = a.model_of ? a.model_of[b] : b;
-    const GMEM T* __restrict__ u_in = gm(a.u_in + soff);
-    GMEM T* __restrict__ u_out = gm(a.u_out + soff);
-    GMEM T* __restrict__ u_mid = a.u_mid ? gm(a.u_mid + soff) : nullptr;
-    GMEM T* __restrict__ u_copy = a.u_copy ? gm(a.u_copy + soff) : nullptr;
-    GMEM T* __restrict__ ck_u = a.ck_u ? gm(a.ck_u + soff) : nullptr;  // checkpoint of (u, w)^{n+2}
-    GMEM T* __restrict__ ck_w = a.ck_w ? gm(a.ck_w + soff) : nullptr;
-    const GMEM T* __restrict__ w_in = gm((const T*)a.w + soff);
-    GMEM T* __restrict__ w_out = gm(a.w_out + soff);
-    const GMEM T* __restrict__ c2p = gm(a.c2 + mdl * a.sb);
-    GMEM T* __restrict__ seis = a.seis ? gm(a.seis + b * a.seis_bs) : nullptr;
-    const GMEM T* __restrict__ resid = a.resid ? gm(a.resid + b * a.resid_bs) : nullptr;
-    const GMEM T* __restrict__ wav = (MODE == MODE_FWD && a.src_node) ? gm(a.wavelet + b * a.wav_bs + a.n) : nullptr;
-    const int src = (MODE == MODE_FWD && a.src_node) ? a.src_node[b] : -1;
-    const int set = a.rec_set_of ? a.rec_set_of[b] : 0;
+    // solve b's arrays and time indices (uniform per workgroup)
+    View2<T> V;
+    if (!make_view2<T, P, MODE>(a, b, V)) return;
+    const GMEM T* __restrict__ u_in = V.u_in;
+    GMEM T* __restrict__ u_out = V.u_out;
+    GMEM T* __restrict__ u_mid = V.u_mid;
+    GMEM T* __restrict__ u_copy = V.u_copy;
+    GMEM T* __restrict__ ck_u = V.ck_u;  // checkpoint of (u, w)^{n+2}
+    GMEM T* __restrict__ ck_w = V.ck_w;
+    const GMEM T* __restrict__ w_in = V.w_in;
+    GMEM T* __restrict__ w_out = V.w_out;
+    const GMEM T* __restrict__ c2p = V.c2;
+    GMEM T* __restrict__ seis = V.seis;
+    const GMEM T* __restrict__ resid = V.resid;
+    const GMEM T* __restrict__ wav = V.wav;
+    const int src = V.src;
+    const int set = V.set;
     const int rbeg = a.rec_ptr2[set * (a.ntiles2 + 1) + tile];
     const int rend = a.rec_ptr2[set * (a.ntiles2 + 1) + tile + 1];
 
@@ -848,7 +939,7 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                 for (int i = 0; i < NP; ++i) r[j][i] = T(0);
         }
         if constexpr (MODE == MODE_FWD) {
-            if (own) sample(u, a.rec_t);
+            if (own) sample(u, V.rec_t);
         }
         if (n1) {
 #pragma unroll
@@ -863,7 +954,7 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
             }
         }
         if constexpr (ADJ) {
-            if (valid) inject(r, a.res_t);
+            if (valid) inject(r, V.res_t);
         }
         publish(r);
         __syncthreads();
@@ -897,6 +988,7 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
 #pragma unroll
                     for (int i = 0; i < P; ++i) W1[Z0w + j][X0w + i] = wn[i];
                     if (u_mid) store4nt(at(u_mid, (unsigned)(base + j * pitch)), un);
+                    if (V.single) store4(at(w_out, (unsigned)(base + j * pitch)), wn);
                 }
             }
             if (last_x || last_z) {
@@ -911,12 +1003,16 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                         if (own) {
                             W1[Z0w + j][X0w + i] = wn;
                             if (u_mid) __builtin_nontemporal_store(r[j][i], &u_mid[base + o]);
+                            if (V.single) w_out[base + o] = wn;
                         }
                     }
             }
         }
         __syncthreads();
     }
+    // a slot whose segment ends after this step: u^{n+1} went to u_mid, w^{n+1}
+    // to w_out (uniform per workgroup)
+    if (V.single) return;
 
     // ---- step two: level n+1 -> n+2 on O and its left / lower ring ----
     {
@@ -942,10 +1038,10 @@ __global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> 
                 for (int i = 0; i < NP; ++i) r[j][i] = T(0);
         }
         if constexpr (MODE == MODE_FWD) {
-            if (own) sample(u, a.rec_t + 1);
+            if (own) sample(u, V.rec_t + 1);
         }
         if constexpr (ADJ) {
-            if (r2) inject(r, a.res_t - 1);
+            if (r2) inject(r, V.res_t - 1);
         }
         publish(r);
         __syncthreads();

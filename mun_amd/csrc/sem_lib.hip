@@ -605,7 +605,7 @@ int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, boo
 template <typename T, int MODE>
 int launch_step2(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
     (void)pl;
-    if (!a.rec_ptr2 || !a.w_out || a.w_out == a.w || a.u_out == a.u_in || a.u_mid == a.u_in || a.slots)
+    if (!a.rec_ptr2 || (!a.slots && (!a.w_out || a.w_out == a.w || a.u_out == a.u_in || a.u_mid == a.u_in)))
         return fail(SEM_E_INVALID, "two-step launch with aliased or missing buffers");
     const unsigned grid = (unsigned)a.nper2 * NXCD;
     if (nts) step2_kernel<T, 4, MODE, true><<<grid, T2_THREADS, 0, st>>>(a);
@@ -1608,7 +1608,49 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
     return SEM_OK;
 }
 
+template <typename T>
+static int slots_launch2(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
+                         hipStream_t st) {
+    int rc = prepare_receivers(pl);
+    if (rc) return rc;
+    sem_batch dummy;
+    std::memset(&dummy, 0, sizeof(dummy));
+    dummy.batch = nslots;
+    dummy.rec_pitch = rec_pitch;
+    StepArgs<T, 4> a = base_args<T>(pl, &dummy, nullptr, nullptr);
+    a.slots = slots;
+    a.m = m;
+    a.post = pl->d.record_mode == SEM_RECORD_POST ? 1 : 0;
+    a.rec_set_of = nullptr;
+    a.c2 = nullptr;
+    const bool nts = use_nt(pl, nslots);
+    const unsigned fwd = 1u << SEM_SLOT_FORWARD_HIST;
+    const unsigned adj = 1u << SEM_SLOT_ADJOINT;
+    if (mask & fwd) {
+        a.phase_mask = fwd;
+        a.damp = 1;
+        if ((rc = launch_step2<T, MODE_FWD>(pl, a, st, nts))) return rc;
+    }
+    if (mask & adj) {
+        a.phase_mask = adj;
+        a.damp = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
+        if ((rc = launch_step2<T, MODE_ADJS>(pl, a, st, nts))) return rc;
+    }
+    return SEM_OK;
+}
+
 extern "C" {
+
+int sem_slots_step2(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
+                    void* stream) try {
+    if (!pl || !slots || nslots < 1 || m < 0 || (m & 1) || rec_pitch < 0)
+        return fail(SEM_E_INVALID, "sem_slots_step2: bad argument (m must be even)");
+    if ((long)std::max(std::max(pl->ntiles, pl->ntiles2), pl->ctiles) * nslots >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "too many slots for this mesh");
+    hipStream_t st = (hipStream_t)stream;
+    return pl->d.precision == SEM_FP64 ? slots_launch2<double>(pl, slots, nslots, m, mask, rec_pitch, st)
+                                       : slots_launch2<float>(pl, slots, nslots, m, mask, rec_pitch, st);
+} ABI_CATCH
 
 int sem_slots_step(sem_plan* pl, const sem_slot* slots, int nslots, int m, unsigned mask, int rec_pitch,
                    void* stream) try {
@@ -1641,12 +1683,20 @@ int sem_slots_record_final(sem_plan* pl, const sem_slot* slots, int nslots, unsi
 int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) try {
     if (!pl || max_entries < 0 || max_sets < 0) return fail(SEM_E_INVALID, "sem_receivers_reserve: bad argument");
     const size_t eb = pl->d.precision == SEM_FP64 ? sizeof(RecEntry<double, 4>) : sizeof(RecEntry<float, 4>);
-    bool m1 = false, m2 = false;
+    bool m1 = false, m2 = false, m3 = false, m4 = false;
     int rc = pl->d_recs.reserve(std::max<size_t>(max_entries * eb, 16), m1);
     if (!rc) rc = pl->d_rec_ptr.reserve((size_t)max_sets * (pl->ntiles + 1) * sizeof(int), m2);
+    // the two-step tiles' table: an element lies in at most two tiles along
+    // each axis there too, so the same entry bound holds
+    if (!rc) rc = pl->d_recs2.reserve(std::max<size_t>(max_entries * eb, 16), m3);
+    if (!rc) rc = pl->d_rec_ptr2.reserve((size_t)max_sets * (pl->ntiles2 + 1) * sizeof(int), m4);
     if (rc) return rc;
     if (m1 || m2) {
         pl->rec_up_entries = pl->rec_up_ptrs = 0;
+        pl->rec_dirty = true;
+    }
+    if (m3 || m4) {
+        pl->rec2_up_entries = pl->rec2_up_ptrs = 0;
         pl->rec_dirty = true;
     }
     return prepare_receivers(pl);

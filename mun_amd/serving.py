@@ -104,15 +104,16 @@ def corr_chunk(S: int) -> int:
 
 
 def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad_elems=None,
-                      full_history=False):
+                      full_history=False, two_step=False):
     """HBM bytes one solve holds while resident: c2, w, either u0, u1, 2K
     checkpoint states and S+1 segment states or the nt+1 states of the full
     history, the adjoint ring and omega, the FP64 gradient accumulator
-    (grad_elems doubles; default: one nodal array), traces."""
+    (grad_elems doubles; default: one nodal array), traces; with two steps
+    per launch the second buffers of w and omega."""
     E = pitch_elems
     K = max(1, math.ceil(nt / S))
     GE = E if grad_elems is None else grad_elems
-    states = (nt + 1) if full_history else (2 + 2 * K + (S + 1))
+    states = ((nt + 1) if full_history else (2 + 2 * K + (S + 1))) + (2 if two_step else 0)
     nodal = E * elem_bytes * (1 + 1 + states + (corr_chunk(S) + 1) + 1) + GE * 8
     traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
     return nodal + traces
@@ -132,6 +133,8 @@ class _Active:
     nrec: int = 0
     src_node: int = 0
     hist: bool = False            # full forward history (no recompute)
+    wflip: bool = False           # two-step rounds: w^n of the forward is in w2, not w
+    omflip: bool = False          # two-step rounds: omega of the adjoint is in om2, not om
 
 
 class Lane:
@@ -207,6 +210,8 @@ class Lane:
         d.seg = b["hist"][s0].data_ptr() if a.hist else b["seg"].data_ptr()
         if a.phase == L.SLOT_FORWARD_HIST:
             d.w = b["w"].data_ptr()
+            if "w2" in b:
+                d.w, d.w2 = (b["w2"].data_ptr(), b["w"].data_ptr()) if a.wflip else (d.w, b["w2"].data_ptr())
         elif a.phase == L.SLOT_FORWARD:
             d.u0, d.u1, d.w = b["u0"].data_ptr(), b["u1"].data_ptr(), b["w"].data_ptr()
             if a.k + 1 < a.K:
@@ -215,6 +220,8 @@ class Lane:
             d.u0, d.w = b["cku"][a.k].data_ptr(), b["ckw"][a.k].data_ptr()
         elif a.phase == L.SLOT_ADJOINT:
             d.w = b["om"].data_ptr()
+            if "om2" in b:
+                d.w, d.w2 = (b["om2"].data_ptr(), b["om"].data_ptr()) if a.omflip else (d.w, b["om2"].data_ptr())
             d.ring, d.ring_slots = b["ring"].data_ptr(), self.chunk + 1
         return d
 
@@ -243,8 +250,16 @@ class Lane:
         sp = C.c_void_p(self.stream.cuda_stream)
         tab = C.c_void_p(self.table_dev.data_ptr())
         adjoint = bool(mask & (1 << L.SLOT_ADJOINT))
+        # two steps per launch for the FORWARD_HIST and ADJOINT slots (step
+        # pairs m, m + 1 never straddle a correlation chunk: S and the chunk
+        # are even), one for the other phases
+        two = mask & TWO_STEP_PHASES if self.server.two_step else 0
+        one = mask & ~two
         for m in range(self.S):
-            L.check(lib.sem_slots_step(h, tab, self.C, m, mask, self.rec_pitch, sp))
+            if one:
+                L.check(lib.sem_slots_step(h, tab, self.C, m, one, self.rec_pitch, sp))
+            if two and m % 2 == 0:
+                L.check(lib.sem_slots_step2(h, tab, self.C, m, two, self.rec_pitch, sp))
             if adjoint and ((m + 1) % self.chunk == 0 or m == self.S - 1):
                 # gradient correlation of the adjoint slots' chunk that just ended
                 L.check(lib.sem_slots_correlate(h, tab, self.C, m // self.chunk, self.chunk, sp))
@@ -271,13 +286,16 @@ class Lane:
         self.rounds += 1
 
 
+TWO_STEP_PHASES = (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_ADJOINT)
+
+
 class SEMServer:
     """Continuous-batching forward+adjoint SEM service on one GPU."""
 
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True, history: str = "auto"):
+                 autotune: bool = True, history: str = "auto", two_step: bool = False):
         """history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  Results are the same
@@ -285,6 +303,9 @@ class SEMServer:
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
         self.history = history
+        # two steps per launch for full-history forward and adjoint slots
+        # (sem_slots_step2); needs an even segment and correlation chunk
+        self.two_step = bool(two_step) and int(segment) % 2 == 0 and corr_chunk(int(segment)) % 2 == 0
         self.device = torch.device(device if device is not None else "cuda")
         self.budget = float(hbm_budget_bytes)
         self.S = int(segment)
@@ -319,9 +340,9 @@ class SEMServer:
             return False
         plan = lane.plan
         fp_args = (plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S)
-        fp_full = request_footprint(*fp_args, grad_elems=lane.GE, full_history=True)
+        fp_full = request_footprint(*fp_args, grad_elems=lane.GE, full_history=True, two_step=self.two_step)
         hist = self.history == "auto" and self.in_use + fp_full <= self.budget
-        fp = fp_full if hist else request_footprint(*fp_args, grad_elems=lane.GE)
+        fp = fp_full if hist else request_footprint(*fp_args, grad_elems=lane.GE, two_step=self.two_step)
         if self.in_use + fp > self.budget:
             return False
         dev, E, S = self.device, lane.E, self.S
@@ -334,6 +355,8 @@ class SEMServer:
             b = {"w": z(E), "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
                  "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
                  "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
+            if self.two_step:
+                b["w2"], b["om2"] = z(E), z(E)
             if hist:
                 b["hist"] = z(req.nt + 1, E)  # u^0 .. u^nt; u^0 = 0
             else:
@@ -385,6 +408,15 @@ class SEMServer:
                                                  a.nrec, a.req.nt, lane.rec_pitch, sp))
             done = []
             for a in lane.occupied():
+                if self.two_step and a.phase in (L.SLOT_FORWARD_HIST, L.SLOT_ADJOINT):
+                    # a segment of ln steps took ceil(ln / 2) launches, each of
+                    # which moved the increment to the other buffer
+                    ln = min(self.S, a.req.nt - a.k * self.S)
+                    if (ln + 1) // 2 % 2 == 1:
+                        if a.phase == L.SLOT_FORWARD_HIST:
+                            a.wflip = not a.wflip
+                        else:
+                            a.omflip = not a.omflip
                 if a.phase in fwd_phases:
                     if a.k + 1 < a.K:
                         a.k += 1

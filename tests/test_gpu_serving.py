@@ -115,3 +115,27 @@ def test_history_and_checkpoint_slots_give_the_same_bits():
             assert a.loglik == b.loglik
             np.testing.assert_array_equal(a.grad, b.grad)
             np.testing.assert_array_equal(a.seis, b.seis)
+
+
+@pytest.mark.parametrize("history", ["auto", "checkpoints"])
+def test_two_step_rounds_give_the_same_bits(history):
+    """Serving rounds that advance full-history forward and adjoint slots two
+    steps per launch (sem_slots_step2; odd segment tails take one step, the
+    increment buffers swap accordingly) give the same bits as one step per
+    launch, with captured rounds, for history and checkpointed requests."""
+    from mun_amd.serving import SEMServer
+    reqs = make_requests(6, seed=11)
+    kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420,
+              history=history)
+    print(f"[serve-two-step] {history}: one step per launch", flush=True)
+    one = SEMServer(hbm_budget_bytes=8e9, **kw).run(reqs)
+    print(f"[serve-two-step] {history}: two steps per launch", flush=True)
+    srv = SEMServer(hbm_budget_bytes=8e9, two_step=True, **kw)
+    assert srv.two_step
+    two = srv.run(reqs)
+    assert any(r.nt % 2 == 1 for r in reqs)  # odd segment tails occur
+    for r in reqs:
+        a, b = one[r.rid], two[r.rid]
+        assert a.loglik == b.loglik
+        np.testing.assert_array_equal(a.grad, b.grad)
+        np.testing.assert_array_equal(a.seis, b.seis)

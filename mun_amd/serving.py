@@ -295,11 +295,12 @@ class SEMServer:
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True, history: str = "auto", two_step: bool = False):
+                 autotune: bool = True, history: str = "auto", two_step: bool = True):
         """history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
-        steps); 'checkpoints' always uses checkpoints.  Results are the same
-        bits either way."""
+        steps); 'checkpoints' always uses checkpoints.  two_step: advance
+        full-history forward and adjoint slots two steps per launch
+        (sem_slots_step2).  Results are the same bits either way."""
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
         self.history = history

@@ -128,7 +128,7 @@ def test_two_step_rounds_give_the_same_bits(history):
     kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420,
               history=history)
     print(f"[serve-two-step] {history}: one step per launch", flush=True)
-    one = SEMServer(hbm_budget_bytes=8e9, **kw).run(reqs)
+    one = SEMServer(hbm_budget_bytes=8e9, two_step=False, **kw).run(reqs)
     print(f"[serve-two-step] {history}: two steps per launch", flush=True)
     srv = SEMServer(hbm_budget_bytes=8e9, two_step=True, **kw)
     assert srv.two_step

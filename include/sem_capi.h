@@ -335,8 +335,8 @@ int sem_plan_gradient_elems(const sem_plan* plan, long* elems);
  * current segments; slots with m >= len skip.  rec_pitch: seismogram row length. */
 int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
                    int rec_pitch, void* stream);
-/* Steps m and m + 1 (m even) of every FORWARD_HIST and ADJOINT slot in
- * phase_mask, in one launch per kernel (two-step kernel, the same bits as two
+/* Steps m and m + 1 (m even) of every FORWARD_HIST, RECOMPUTE and ADJOINT
+ * slot in phase_mask, in one launch per kernel (two-step kernel, the same bits as two
  * sem_slots_step calls); a slot with m + 1 == len takes step m alone.  The
  * increment is read from w and written to w2 on launches with (m / 2) even,
  * the other way round on odd ones, so it ends a segment of len steps in w2

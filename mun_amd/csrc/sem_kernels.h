@@ -694,8 +694,8 @@ void step_kernel(const StepArgs<T, P> a) {
 // and u_out (the history slot) is written non-temporally too.  Batch mode only
 // (no slot table); FP32 and FP64.
 // Per-workgroup arrays and time indices of one solve in a two-step launch:
-// from the uniform batch arguments, or from the slot table (FORWARD_HIST and
-// ADJOINT slots; the increment alternates between the slot's w and w2 launch
+// from the uniform batch arguments, or from the slot table (FORWARD_HIST,
+// RECOMPUTE and ADJOINT slots; the increment alternates between the slot's w and w2 launch
 // by launch, and a slot whose segment ends after step m takes that step
 // alone: `single`).  Returns false if this solve does not advance.
 template <typename T>
@@ -767,6 +767,18 @@ __device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View
             v.seis = gm((T*)d.seis);
             v.rec_t = a.post ? n - 1 : n;
         }
+        return true;
+    }
+    if (d.phase == SEM_SLOT_RECOMPUTE) {
+        // u^{n0} from the checkpoint u0, then segment slots 1..len; no samples
+        if (MODE != MODE_FWD) return false;
+        const int n = d.n0 + a.m;
+        const GMEM T* seg = gm((const T*)d.seg);
+        v.u_in = (a.m == 0) ? gm((const T*)d.u0) : seg + (long)a.m * d.seg_stride;
+        v.u_mid = (GMEM T*)(seg + (long)(a.m + 1) * d.seg_stride);
+        v.u_out = (GMEM T*)(seg + (long)(a.m + 2) * d.seg_stride);  // not touched when single
+        v.wav = gm((const T*)d.wavelet + n);
+        v.src = d.src_node;
         return true;
     }
     if (d.phase == SEM_SLOT_ADJOINT) {

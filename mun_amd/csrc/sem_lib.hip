@@ -1624,10 +1624,10 @@ static int slots_launch2(sem_plan* pl, const sem_slot* slots, int nslots, int m,
     a.rec_set_of = nullptr;
     a.c2 = nullptr;
     const bool nts = use_nt(pl, nslots);
-    const unsigned fwd = 1u << SEM_SLOT_FORWARD_HIST;
+    const unsigned fwd = (1u << SEM_SLOT_FORWARD_HIST) | (1u << SEM_SLOT_RECOMPUTE);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
     if (mask & fwd) {
-        a.phase_mask = fwd;
+        a.phase_mask = mask & fwd;
         a.damp = 1;
         if ((rc = launch_step2<T, MODE_FWD>(pl, a, st, nts))) return rc;
     }

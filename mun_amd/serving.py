@@ -218,6 +218,8 @@ class Lane:
                 d.ck_u, d.ck_w = b["cku"][a.k + 1].data_ptr(), b["ckw"][a.k + 1].data_ptr()
         elif a.phase == L.SLOT_RECOMPUTE:
             d.u0, d.w = b["cku"][a.k].data_ptr(), b["ckw"][a.k].data_ptr()
+            if "w2" in b:  # the segment's increment is not needed after it: no flip to track
+                d.w2 = b["w2"].data_ptr()
         elif a.phase == L.SLOT_ADJOINT:
             d.w = b["om"].data_ptr()
             if "om2" in b:
@@ -250,7 +252,7 @@ class Lane:
         sp = C.c_void_p(self.stream.cuda_stream)
         tab = C.c_void_p(self.table_dev.data_ptr())
         adjoint = bool(mask & (1 << L.SLOT_ADJOINT))
-        # two steps per launch for the FORWARD_HIST and ADJOINT slots (step
+        # two steps per launch for the FORWARD_HIST, RECOMPUTE and ADJOINT slots (step
         # pairs m, m + 1 never straddle a correlation chunk: S and the chunk
         # are even), one for the other phases
         two = mask & TWO_STEP_PHASES if self.server.two_step else 0
@@ -286,7 +288,7 @@ class Lane:
         self.rounds += 1
 
 
-TWO_STEP_PHASES = (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_ADJOINT)
+TWO_STEP_PHASES = (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT)
 
 
 class SEMServer:

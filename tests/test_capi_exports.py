@@ -112,5 +112,6 @@ def test_schedule_and_streaming_entry_points_validate_arguments():
     assert lib.sem_plan_set_streaming(None, 0) == -1
     assert lib.sem_plan_set_step_fusion(None, 2) == -1
     assert lib.sem_plan_get_step_fusion(None, None) == -1
+    assert lib.sem_slots_step2(None, None, 1, 0, 0, 0, None) == -1
     assert lib.sem_workspace_bytes(None, 1, 1, 1, None) == -1
     assert "bad argument" in lib.sem_last_error().decode()

@@ -44,8 +44,8 @@ FWD_BYTES_PER_NODE = 20
 # reads 12 B and writes 12 B (u^{n+1} and u^{n+2} into the history or the
 # adjoint ring, w^{n+2}), 12 B per step; correlation 8 B (u^{k+1}, lambda_k
 # read), FP64 accumulator (25 doubles per element of 16 nodes = 12.5 B per
-# node) read and written once per chunk of 32 steps
-SOLVE_BYTES_PER_NODE_STEP = {1: 20 + 20 + 8 + 25 / 32, 2: 12 + 12 + 8 + 25 / 32}
+# node) read and written once per chunk of 64 steps
+SOLVE_BYTES_PER_NODE_STEP = {1: 20 + 20 + 8 + 25 / 64, 2: 12 + 12 + 8 + 25 / 64}
 
 
 def log(msg):

@@ -178,7 +178,9 @@ int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_b
 /* Forward + misfit + adjoint + gradient; replaces the node-level part of
  * run_forward_and_adjoint (core.py:1067-1278) including
  * accumulate_sensitivity_dJdc_torch (core.py:49-85).
- *   obs      device real [batch][nt][rec_pitch]   observed data y_obs
+ *   obs      device double [batch][nt][rec_pitch] observed data y_obs; the
+ *            residual d_syn - y_obs is formed in double, as core.py:1224-1227
+ *            forms it (the adjoint source then keeps the state precision)
  *   misfit   device double [batch]               0.5 * ||d_syn - y_obs||^2
  *   grad_c   device double [batch][N] or NULL     d loglik / d c at the nodes,
  *            loglik = -misfit * inv_sigma2 (core.py:1230; inv_sigma2 =
@@ -186,7 +188,7 @@ int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_b
  *   grad_c2  device double [batch][N] or NULL     d loglik / d (c^2)
  * The gradient-time integral is accumulated in double.  With both gradient
  * outputs NULL only the forward pass and the misfit run. */
-int sem_forward_adjoint(sem_plan* plan, const sem_batch* b, const void* obs, double inv_sigma2,
+int sem_forward_adjoint(sem_plan* plan, const sem_batch* b, const double* obs, double inv_sigma2,
                         double* misfit, double* grad_c, double* grad_c2,
                         void* workspace, size_t ws_bytes, void* stream);
 
@@ -369,8 +371,9 @@ int sem_receivers_sync(sem_plan* plan);
 /* Per-solve helpers for slot owners (device pointers):
  * copy [nmodel][N] squared velocities into the padded layout;  */
 int sem_pad_nodal(sem_plan* plan, const void* src, void* dst_padded, int nmodel, void* stream);
-/* res = seis - obs on the first nrec columns of nt rows, misfit = 0.5 sum res^2 (double) */
-int sem_residual(sem_plan* plan, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
+/* res = seis - obs on the first nrec columns of nt rows (obs double, the difference
+ * formed in double and stored in the state precision), misfit = 0.5 sum res^2 (double) */
+int sem_residual(sem_plan* plan, const void* seis, const double* obs, void* res, double* misfit, int nrec, int nt,
                  int rec_pitch, void* stream);
 /* grad_c2 = scale * G, grad_c = 2 sqrt(c2) grad_c2 for one solve, G assembled per
  * node from the accumulator (sem_plan_gradient_elems doubles), c2 padded; outputs [N] */

@@ -1308,7 +1308,11 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
     }
 }
 
-// r = seis - obs (first nrec columns of each row), misfit[b] = 0.5 * sum r^2 in double.
+// r = seis - obs (first nrec columns of each row), misfit[b] = 0.5 * sum r^2.
+// The observations are double and the difference is formed in double (the
+// reference forms d_syn - y_obs in FP64, core.py:1224-1227): near a fit, where
+// |r| << |seis|, an FP32 difference would lose the residual's low digits.  The
+// adjoint source res keeps the state precision; the misfit sums the double r.
 // Rows outside the misfit window [w0, w1) or off its decimation grid give r = 0.
 // nrec_of: per-solve receiver counts on the device, or null to use nrec.
 // One block of RESID_THREADS per solve: each wave walks rows (wave, wave +
@@ -1316,7 +1320,7 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
 // fixed tree, so the misfit is deterministic.
 constexpr int RESID_THREADS = 1024;
 template <typename T>
-__global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, const T* obs, T* res,
+__global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, const double* obs, T* res,
                                                                  double* misfit, const int* nrec_of, int nrec,
                                                                  int nt, int pitch, long bs, int w0, int w1,
                                                                  int decim) {
@@ -1328,9 +1332,9 @@ __global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, 
         const bool in = row >= w0 && row < w1 && (row - w0) % decim == 0;
         const long o = b * bs + (long)row * pitch;
         for (int c = lane; c < pitch; c += 64) {
-            const T rv = (c < nrec && in) ? seis[o + c] - obs[o + c] : T(0);
-            res[o + c] = rv;
-            acc += (double)rv * (double)rv;
+            const double rv = (c < nrec && in) ? (double)seis[o + c] - obs[o + c] : 0.0;
+            res[o + c] = (T)rv;
+            acc += rv * rv;
         }
     }
     __shared__ double red[RESID_THREADS];

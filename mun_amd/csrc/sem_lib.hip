@@ -887,7 +887,7 @@ int run_group(sem_plan* pl, const Group& g, const Layout& L, const void* obs, do
 
     // residual and misfit
     T* RES = (T*)(ws + L.RES);
-    residual_kernel<T><<<B, RESID_THREADS, 0, st>>>((const T*)bt->seis, (const T*)obs + (long)g.g0 * nt * bt->rec_pitch, RES,
+    residual_kernel<T><<<B, RESID_THREADS, 0, st>>>((const T*)bt->seis, (const double*)obs + (long)g.g0 * nt * bt->rec_pitch, RES,
                                           misfit + g.g0, g.nrec, 0, nt, bt->rec_pitch, (long)nt * bt->rec_pitch,
                                           pl->d.win_begin, pl->d.win_end, pl->d.win_decim);
     LAUNCH_CHECK();
@@ -1391,7 +1391,7 @@ int sem_forward(sem_plan* pl, const sem_batch* b, void* ws, size_t ws_bytes, voi
                                        : forward_impl<float>(pl, b, ws, ws_bytes, st);
 } ABI_CATCH
 
-int sem_forward_adjoint(sem_plan* pl, const sem_batch* b, const void* obs, double inv_sigma2, double* misfit,
+int sem_forward_adjoint(sem_plan* pl, const sem_batch* b, const double* obs, double inv_sigma2, double* misfit,
                         double* grad_c, double* grad_c2, void* ws, size_t ws_bytes, void* stream) try {
     if (!pl) return fail(SEM_E_INVALID, "null plan");
     hipStream_t st = (hipStream_t)stream;
@@ -1720,7 +1720,7 @@ int sem_pad_nodal(sem_plan* pl, const void* src, void* dst, int nmodel, void* st
     return SEM_OK;
 } ABI_CATCH
 
-int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, double* misfit, int nrec, int nt,
+int sem_residual(sem_plan* pl, const void* seis, const double* obs, void* res, double* misfit, int nrec, int nt,
                  int rec_pitch, void* stream) try {
     if (!pl || !seis || !obs || !res || !misfit || nrec < 0 || nrec > rec_pitch || nt < 1)
         return fail(SEM_E_INVALID, "sem_residual: bad argument");
@@ -1729,7 +1729,7 @@ int sem_residual(sem_plan* pl, const void* seis, const void* obs, void* res, dou
         residual_kernel<double><<<1, RESID_THREADS, 0, st>>>((const double*)seis, (const double*)obs, (double*)res, misfit,
                                                    nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     else
-        residual_kernel<float><<<1, RESID_THREADS, 0, st>>>((const float*)seis, (const float*)obs, (float*)res, misfit,
+        residual_kernel<float><<<1, RESID_THREADS, 0, st>>>((const float*)seis, (const double*)obs, (float*)res, misfit,
                                                   nullptr, nrec, nt, rec_pitch, (long)nt * rec_pitch, 0, nt, 1);
     LAUNCH_CHECK();
     return SEM_OK;

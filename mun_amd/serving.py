@@ -108,14 +108,15 @@ def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad
     """HBM bytes one solve holds while resident: c2, w, either u0, u1, 2K
     checkpoint states and S+1 segment states or the nt+1 states of the full
     history, the adjoint ring and omega, the FP64 gradient accumulator
-    (grad_elems doubles; default: one nodal array), traces; with two steps
+    (grad_elems doubles; default: one nodal array), traces (seismograms and
+    residuals in the state precision, observations in double); with two steps
     per launch the second buffers of w and omega."""
     E = pitch_elems
     K = max(1, math.ceil(nt / S))
     GE = E if grad_elems is None else grad_elems
     states = ((nt + 1) if full_history else (2 + 2 * K + (S + 1))) + (2 if two_step else 0)
     nodal = E * elem_bytes * (1 + 1 + states + (corr_chunk(S) + 1) + 1) + GE * 8
-    traces = 3 * nt * nrec_pitch * elem_bytes + nt * elem_bytes
+    traces = 2 * nt * nrec_pitch * elem_bytes + nt * nrec_pitch * 8 + nt * elem_bytes
     return nodal + traces
 
 
@@ -357,7 +358,8 @@ class SEMServer:
             z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
             b = {"w": z(E), "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
                  "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
-                 "obs": z(req.nt, lane.rec_pitch), "c2p": z(E), "misfit": z(1, dt=torch.float64)}
+                 "obs": z(req.nt, lane.rec_pitch, dt=torch.float64), "c2p": z(E),
+                 "misfit": z(1, dt=torch.float64)}
             if self.two_step:
                 b["w2"], b["om2"] = z(E), z(E)
             if hist:
@@ -365,7 +367,7 @@ class SEMServer:
             else:
                 b.update({"u0": z(E), "u1": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E)})
             if req.obs is not None:
-                b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float32, device=dev)
+                b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float64, device=dev)
             w = source_wavelet(req.nt, req.dt, req.f0)
             b["wav"] = torch.as_tensor(w, dtype=torch.float32, device=dev)
             base = None if req.base is None else torch.as_tensor(req.base, device=dev)

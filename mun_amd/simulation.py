@@ -438,7 +438,7 @@ class SEMSimulation:
         window = self._window()
         plan = self._plan("post", window)
         y = self._observations(plan, y_obs, obs_dt, obs_nt, window)
-        obs = torch.as_tensor(y, dtype=plan.dtype, device=self.device)[None]
+        obs = torch.as_tensor(y, dtype=torch.float64, device=self.device)[None]
         out = plan.forward_adjoint(c2, self._wavelet(plan), [plan.source_node(self.src_x, self.src_z)], obs,
                                    sigma, [0])
         d_syn = out["seis"][0].double().cpu().numpy()[:, :self.num_receivers]
@@ -464,7 +464,7 @@ class SEMSimulation:
         'receiver_data' (nt, nrec), 'dt', 'nt'.  Replaces the per-chain calls
         of SEMSimulation(config).run() in the MCMC likelihood
         (Bspline_Coupling/MCMC/MCMC.py:192-213, 426-433)."""
-        M, c2, groups = self._batch_models(params)
+        M, c2, groups, _ = self._batch_models(params)
         out = [None] * M
         for (dt, nt), idx, plan, seis in self._forward_groups(groups, c2):
             data = seis.double().cpu().numpy()[:, :, :self.num_receivers]
@@ -486,17 +486,31 @@ class SEMSimulation:
         finally:
             self.dt, self.nt = dt0, nt0
 
-    def _batch_models(self, params):
+    def _batch_models(self, params, ctrl_base=None, sequential_cfl: bool = False):
+        """Velocities of M parameter sets in one batched map, grouped by their
+        CFL-adjusted (dt, nt).  Closed-curve models: params are offsets [M, 2K]
+        added to ``ctrl_base`` (default: the configured control points).
+
+        CFL (utils.py:96-111, quirk Q1): by default each model's dt is this
+        simulation's dt shrunk for that model alone, as a fresh simulation per
+        model computes it.  ``sequential_cfl`` gives the dt that one cached
+        simulation called model after model would use (its dt only ever
+        shrinks, C1S core.py:1381-1390): the running minimum in model order,
+        which then becomes this simulation's dt.
+        Returns (M, c2 [M, N], {(dt, nt): [model indices]}, aux [M, 4, N] of
+        the curve projection or None)."""
         params = np.asarray(_np(params), dtype=np.float64)
         M = params.shape[0]
         plan0 = self._plan("post")
+        aux = None
         if self.grid is None:
-            if self.ctrl_pts_original is None or not self.anomaly_enabled:
+            base_pts = self.ctrl_pts_original if ctrl_base is None else np.asarray(ctrl_base, dtype=np.float64)
+            if base_pts is None or not self.anomaly_enabled:
                 raise ValueError("batched runs need a closed-curve anomaly or a B-spline grid")
-            ctrl = self.ctrl_pts_original[None] + params.reshape(M, -1, 2)
+            ctrl = base_pts.reshape(1, -1, 2) + params.reshape(M, -1, 2)
             a = self._curve_args()
-            c, c2, _ = plan0.curve_velocity(ctrl, a["kind"], a["v_in"], a["v_out"], a["tau"], self.spline_samples,
-                                            7, a["background"], a["blend"])
+            c, c2, aux = plan0.curve_velocity(ctrl, a["kind"], a["v_in"], a["v_out"], a["tau"], self.spline_samples,
+                                              7, a["background"], a["blend"])
         else:
             base = None if self.grid_base is None else torch.as_tensor(self.grid_base, device=self.device)
             c, c2 = plan0.bspline_map(torch.as_tensor(params.reshape(M, *self.grid_coef.shape)), base)
@@ -504,29 +518,33 @@ class SEMSimulation:
             raise ValueError("[CFL] velocity_model contains non-finite values (NaN/Inf).")
         groups = {}
         cmax = c.max(dim=1).values.cpu().numpy()
+        run_dt, run_nt = self.dt, self.nt
         for m in range(M):
-            dt, nt = self.dt, self.nt
+            dt, nt = (run_dt, run_nt) if sequential_cfl else (self.dt, self.nt)
             if self.cfl_adjust:
                 dtm = cfl_dt(float(cmax[m]), self.npol, self.dx_elem, self.dz_elem)
                 if dt > dtm:
                     dt, nt = float(dtm), int(np.ceil(self.total_time / dtm))
+            run_dt, run_nt = dt, nt
             groups.setdefault((dt, nt), []).append(m)
-        return M, c2, groups
+        if sequential_cfl:
+            self.dt, self.nt = run_dt, run_nt
+        return M, c2, groups, aux
 
     def log_likelihood_batch(self, params, y_obs, sigma: float) -> np.ndarray:
         """sum Normal(simulated, sigma).log_prob(y_obs) for M parameter sets
         (the likelihood of Bspline_Coupling/MCMC/MCMC.py:426-433) from one
         batched forward; y_obs (nt, nrec) on the simulation's time axis.  The
         squared residual is reduced on the GPU in double."""
-        M, c2, groups = self._batch_models(params)
+        M, c2, groups, _ = self._batch_models(params)
         y = np.asarray(_np(y_obs), dtype=np.float64)
         out = np.empty(M)
         const = math.log(float(sigma)) + 0.5 * math.log(2 * math.pi)
         for (dt, nt), idx, plan, seis in self._forward_groups(groups, c2):
             if y.shape != (nt, self.num_receivers):
                 raise ValueError(f"y_obs shape {y.shape} != simulated {(nt, self.num_receivers)}")
-            obs = torch.zeros((1,) + tuple(seis.shape[1:]), dtype=plan.dtype, device=self.device)
-            obs[0, :, :self.num_receivers] = torch.as_tensor(y, dtype=plan.dtype)
+            obs = torch.zeros((1,) + tuple(seis.shape[1:]), dtype=torch.float64, device=self.device)
+            obs[0, :, :self.num_receivers] = torch.as_tensor(y, dtype=torch.float64)
             mis = plan.misfit(seis, obs, self.num_receivers).cpu().numpy()
             for k, m in enumerate(idx):
                 out[m] = -mis[k] / float(sigma) ** 2 - nt * self.num_receivers * const

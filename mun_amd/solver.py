@@ -353,12 +353,13 @@ class Plan:
         return seis, snaps
 
     def misfit(self, seis: torch.Tensor, obs: torch.Tensor, nrec: int) -> torch.Tensor:
-        """0.5 * ||seis - obs||^2 per solve over the first nrec columns, in
-        double (seis, obs: [B, nt, pitch] in the plan's precision)."""
+        """0.5 * ||seis - obs||^2 per solve over the first nrec columns, the
+        difference formed in double (seis: [B, nt, pitch] in the plan's
+        precision, obs: [B or 1, nt, pitch], taken in double)."""
         seis = seis.to(device=self.device, dtype=self.dtype).contiguous()
-        obs = obs.to(device=self.device, dtype=self.dtype).expand_as(seis).contiguous()
+        obs = obs.to(device=self.device, dtype=torch.float64).expand(seis.shape).contiguous()
         B, nt, pitch = seis.shape
-        res = torch.empty_like(seis[0])
+        res = torch.empty_like(seis[0])  # adjoint source, plan precision
         out = torch.empty(B, dtype=torch.float64, device=self.device)
         for b in range(B):
             L.check(self.lib.sem_residual(self.h, _ptr(seis[b]), _ptr(obs[b]), _ptr(res), _ptr(out[b:b + 1]),
@@ -373,7 +374,8 @@ class Plan:
         grad_c2 (optional), seis [B,nt,pitch])."""
         c2, wavelet, src, rec_sets, model_of, pitch = self._prep(c2, wavelet, src_nodes, rec_sets, model_of)
         B = len(rec_sets)
-        obs = obs.to(device=self.device, dtype=self.dtype)
+        # observations stay in double: the residual is formed in double
+        obs = obs.to(device=self.device, dtype=torch.float64)
         if obs.dim() == 2:
             obs = obs[None]
         if obs.shape[0] != B or obs.shape[1] != self.nt or obs.shape[2] > pitch:

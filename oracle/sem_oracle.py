@@ -609,7 +609,8 @@ def misfit_loglik(seis: np.ndarray, y_obs: np.ndarray, noise_std: float):
 
 
 def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.ndarray,
-            sigma2: float, mode: str = "reference", K: Optional[sp.csr_matrix] = None, stop: int = 0):
+            sigma2: float, mode: str = "reference", K: Optional[sp.csr_matrix] = None, stop: int = 0,
+            after_step=None):
     """Reverse-time adjoint and gradient correlation.
 
     reference mode (S/core.py:1247-1275; quirk Q5, no sponge):
@@ -620,6 +621,7 @@ def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.nd
         dJ_dv = 2c o sum_k sens(u~^{k+1}, lam_k)      (S/core.py:1265-1270).
     The loop runs it = nt-1 .. stop; ``stop`` > 0 is the misfit-window
     truncation of C1S/core.py:1530-1560, whose adjoint covers the window only.
+    ``after_step(it, lam)`` (optional) sees lam_{it-1} after every step.
     Returns dJ_dv, which equals d loglik / dc (node order of ``disc.mesh``)."""
     p = disc.prob
     if K is None:
@@ -644,6 +646,8 @@ def adjoint(disc: Discretization, c: np.ndarray, hist: _History, residual: np.nd
         if it < p.nt - 1:  # lam_{nt-1} = 0 contributes nothing
             g += sensitivity_dc2(hist.get(it), lam_curr, disc.mesh, disc.ops)
         lam_next, lam_curr = lam_curr, lam_prev
+        if after_step is not None:
+            after_step(it, lam_curr)
     return 2.0 * c * g
 
 

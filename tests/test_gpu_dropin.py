@@ -97,18 +97,36 @@ def test_run_forward_and_adjoint_matches_oracle(mode):
 
 
 def test_autograd_function_reference_signature():
+    """SEMLikelihoodAdjointFn hands the solver sqrt(noise_std^2 + 1e-12)
+    (RQS_Coupling.py:319-333).  At the driver's obs_noise_std = 5e-9
+    (RQS_Coupling.py:82) the log-likelihood and gradient are therefore those
+    of noise_std = 5e-9 scaled by (2.5e-17 + 1e-24) / (2.5e-17 + 1e-12 +
+    1e-24) (the core adds 1e-24, core.py:1086), and equal the oracle's at
+    sigma^2 = 2.5e-17 + 1e-12 + 1e-24."""
     from mun_amd.autograd import SEMLikelihoodAdjointFn, get_or_make_sim
     print("[autograd]", flush=True)
     cfg = reference_config("fp64")
     sim = get_or_make_sim(cfg)
     yobs = sim.run()["receiver_data"] * 1.05
+    ns = 5e-9
     z = torch.zeros(12, dtype=torch.float64, requires_grad=True)
-    ll = SEMLikelihoodAdjointFn.apply(z, torch.as_tensor(yobs), 1e-7, cfg, sim.dt, sim.nt)
+    ll = SEMLikelihoodAdjointFn.apply(z, torch.as_tensor(yobs), ns, cfg, sim.dt, sim.nt)
     ll.backward()
-    out = sim.run_forward_and_adjoint({"bspline_ctrl": CTRL, "y_obs": yobs, "obs_dt": sim.dt, "obs_nt": sim.nt,
-                                       "noise_std": 1e-7})
-    assert abs(ll.item() - out["loglik"]) <= 1e-12 * abs(out["loglik"])
-    np.testing.assert_allclose(z.grad.numpy(), out["grad_wrt_ctrl"], rtol=1e-12)
+    raw = sim.run_forward_and_adjoint({"bspline_ctrl": CTRL, "y_obs": yobs, "obs_dt": sim.dt, "obs_nt": sim.nt,
+                                       "noise_std": ns})
+    scale = (ns ** 2 + 1e-24) / (ns ** 2 + 1e-12 + 1e-24)
+    print(f"  scale {scale:.6e}: loglik {ll.item():.9e} vs raw x scale {raw['loglik'] * scale:.9e}")
+    assert abs(ll.item() - raw["loglik"] * scale) <= 1e-12 * abs(raw["loglik"] * scale)
+    np.testing.assert_allclose(z.grad.numpy(), raw["grad_wrt_ctrl"] * scale, rtol=1e-11)
+    # against the oracle at the reference's variance
+    disc = oracle_for(cfg, sim)
+    v, sd, us, no = O.closed_curve_velocity(disc.mesh.coords, CTRL, 2000.0, 3000.0, 10.0, 800, 7)
+    ref = O.forward_adjoint(disc, v, yobs, np.sqrt(ns ** 2 + 1e-12), mode="reference")
+    g12 = O.closed_curve_projection(-ref["dloglik_dc"], sd, us, no, 2000.0, 3000.0, 10.0)
+    el = abs(ll.item() - ref["loglik"]) / abs(ref["loglik"])
+    eg = rel_l2(z.grad.numpy(), g12)
+    print(f"  vs oracle: loglik {el:.2e} grad {eg:.2e}")
+    assert el < 1e-9 and eg < 1e-6
 
 
 def _grid_solver(precision, mode, nshot=1, nt=300, seed=0):
@@ -177,5 +195,7 @@ def test_batched_gradient_matches_oracle_and_shots_share_models():
             tot_g = tot_g + r["dloglik_dc"]
             assert rel_l2(seis[m, k, :, :20].double().cpu().numpy(), r["seis"]) < 1e-5
         gref = Bz.T @ tot_g.reshape(discs[0].mesh.NZ, discs[0].mesh.NX) @ Bx
-        assert abs(ll[m].item() - tot_ll) < 1e-4 * abs(tot_ll)
-        assert rel_l2(grad[m].cpu().numpy(), gref) < 1e-4
+        el = abs(ll[m].item() - tot_ll) / abs(tot_ll)
+        eg = rel_l2(grad[m].cpu().numpy(), gref)
+        print(f"  model {m}: loglik {el:.2e} grad {eg:.2e}")
+        assert el < 1e-5 and eg < 1e-5

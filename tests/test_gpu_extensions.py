@@ -49,13 +49,13 @@ def test_misfit_window_matches_oracle(prob_small, mode, precision):
                 prob.sponge_width, prob.sponge_cmax, prob.sponge_R, prob.sponge_sides, precision=precision,
                 adjoint_mode=mode, checkpoint_every=64, misfit_window=win)
     rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
-    obs = torch.as_tensor(yobs, dtype=plan.dtype, device=plan.device)[None]
+    obs = torch.as_tensor(yobs, dtype=torch.float64, device=plan.device)[None]
     out = plan.forward_adjoint(c2, w, [src], obs, 1e-9, [rs])
     torch.cuda.synchronize()
     e_ll = abs(float(out["loglik"][0]) - ref["loglik"]) / abs(ref["loglik"])
     e_g = rel_l2(out["grad_c"][0].cpu().numpy(), ref["dloglik_dc"])
     print(f"  loglik {e_ll:.2e} grad {e_g:.2e}")
-    tol = 1e-9 if precision == "fp64" else 1e-4
+    tol = 1e-9 if precision == "fp64" else 1e-5
     assert e_ll < tol and e_g < tol
 
 

@@ -51,7 +51,7 @@ def test_forward_adjoint_gradient(prob_small, mode, precision, tol):
     ref = O.forward_adjoint(disc, c, yobs, noise, mode=mode)
     plan = make_plan(prob, precision, adjoint_mode=mode, checkpoint_every=64)
     rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
-    obs = torch.as_tensor(yobs, dtype=plan.dtype, device=plan.device)[None]
+    obs = torch.as_tensor(yobs, dtype=torch.float64, device=plan.device)[None]
     out = plan.forward_adjoint(c2, w, [src], obs, noise, [rs])
     torch.cuda.synchronize()
     e_seis = rel_l2(out["seis"][0].double().cpu().numpy(), ref["seis"])
@@ -59,8 +59,8 @@ def test_forward_adjoint_gradient(prob_small, mode, precision, tol):
     e_g = rel_l2(out["grad_c"][0].cpu().numpy(), ref["dloglik_dc"])
     print(f"  seis {e_seis:.3e}  loglik {e_ll:.3e}  grad {e_g:.3e}")
     assert e_seis < tol
-    assert e_ll < 10 * tol
-    assert e_g < (tol if precision == "fp64" else 1e-4)
+    assert e_ll < tol
+    assert e_g < tol
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
@@ -73,7 +73,7 @@ def test_checkpoint_recompute_bit_identical(prob_small, precision):
     for ck in (None, 37, 100):
         plan = make_plan(prob, precision, adjoint_mode="exact", checkpoint_every=ck)
         rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
-        obs = torch.as_tensor(sp * 0.9, dtype=plan.dtype, device=plan.device)[None]
+        obs = torch.as_tensor(sp * 0.9, dtype=torch.float64, device=plan.device)[None]
         o = plan.forward_adjoint(c2, w, [src], obs, 1e-9, [rs])
         outs.append((o["grad_c"][0].cpu(), o["misfit"][0].item()))
     for g, m in outs[1:]:
@@ -93,7 +93,7 @@ def test_batch_independence(prob_small):
         ck, _, _ = O.tensor_bspline_velocity(disc.mesh, coef + 60.0 * rng.standard_normal(coef.shape))
         models.append(ck)
     C2 = torch.as_tensor(np.stack(models) ** 2, dtype=plan.dtype, device=plan.device)
-    obs = torch.as_tensor(sp * 1.1, dtype=plan.dtype, device=plan.device)[None]
+    obs = torch.as_tensor(sp * 1.1, dtype=torch.float64, device=plan.device)[None]
     alone = plan.forward_adjoint(C2[:1], w, [src], obs, 1e-9, [rs])
     srcs = [src, disc.mesh.find_closest_node(100.0, 60.0), src, src, src]
     mix = plan.forward_adjoint(C2[[1, 2, 0, 3, 4]], w, srcs, obs.expand(5, -1, -1), 1e-9, [rs] * 5)
@@ -127,7 +127,7 @@ def test_every_tile_height_matches_oracle(prob_small, tile_rows):
     ref = O.forward_adjoint(disc, c, sp * 0.8, 1e-9, mode="exact")
     plan = make_plan(prob, "fp64", adjoint_mode="exact", checkpoint_every=45, tile_rows=tile_rows)
     rs, src, w, c2 = gpu_setup(prob, disc, c, plan)
-    obs = torch.as_tensor(sp * 0.8, dtype=plan.dtype, device=plan.device)[None]
+    obs = torch.as_tensor(sp * 0.8, dtype=torch.float64, device=plan.device)[None]
     out = plan.forward_adjoint(c2, w, [src], obs, 1e-9, [rs])
     torch.cuda.synchronize()
     assert rel_l2(out["seis"][0].cpu().numpy(), ref["seis"]) < 1e-9
@@ -145,3 +145,29 @@ def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
     print("  20x20 batch 8:", rep)
     autotune._MEM.clear()
     assert autotune.best_tile_rows(20, 20, 8, "fp32", steps=10) == best  # from the file cache
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_residual_formed_in_double_near_a_fit(prob_small, precision):
+    """Near a fit (|r| / |d| = 1e-3) the residual d_syn - y_obs is formed in
+    double from the double observations, as core.py:1224-1227 forms it: the
+    misfit equals 0.5 sum ((double) d - y)^2 to rounding.  Casting y to FP32
+    first would lose the residual's low digits (about 6e-8 |d| per sample,
+    1e-4 of |r|)."""
+    prob = prob_small[0]
+    print(f"[residual near fit] {precision}", flush=True)
+    plan = make_plan(prob, precision)
+    rng = np.random.default_rng(5)
+    B, nt, pitch, nrec = 2, 300, 16, 13
+    d = torch.as_tensor(rng.standard_normal((B, nt, pitch)), dtype=plan.dtype, device=plan.device)
+    y = d.double() * (1.0 + 1e-3 * torch.as_tensor(rng.standard_normal((B, nt, pitch)), device=plan.device))
+    got = plan.misfit(d, y, nrec).cpu().numpy()
+    r = (d.double() - y)[:, :, :nrec].cpu().numpy()
+    want = 0.5 * np.sum(r * r, axis=(1, 2))
+    r32 = (d - y.to(plan.dtype)).double()[:, :, :nrec].cpu().numpy()
+    lossy = 0.5 * np.sum(r32 * r32, axis=(1, 2))
+    err = np.abs(got / want - 1).max()
+    print(f"  misfit rel err {err:.2e} (FP32-cast observations: {np.abs(lossy / want - 1).max():.2e})")
+    assert err < 1e-12
+    if precision == "fp32":
+        assert np.abs(lossy / want - 1).max() > 1e-7

@@ -42,7 +42,7 @@ def test_every_schedule_gives_the_same_bits(prob_sched, precision, shared_models
     plan = make_plan(prob, precision, adjoint_mode="exact", checkpoint_every=50)
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan, shared_models=shared_models)
     B = len(srcs)
-    obs = torch.as_tensor(seis * 0.9, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis * 0.9, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     # (checkpoint_every, group, lanes): whole batch with checkpoints, whole with
     # the full history, groups on 1-3 lanes with either policy
     schedules = [(50, 0, 1), (0, 0, 1), (0, 2, 3), (0, 3, 1), (37, 3, 2), (0, 1, 3), (50, 2, 2)]
@@ -58,8 +58,10 @@ def test_every_schedule_gives_the_same_bits(prob_sched, precision, shared_models
             assert torch.equal(o[k], outs[0][k]), (sch, k)
     # and solve 0 (the fixture's model) still matches the oracle
     ref = O.forward_adjoint(disc, c, seis * 0.9, 1e-9, mode="exact")
-    tol = 1e-9 if precision == "fp64" else 1e-4
-    assert rel_l2(outs[0]["grad_c"][0].numpy(), ref["dloglik_dc"]) < tol
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    e_g = rel_l2(outs[0]["grad_c"][0].numpy(), ref["dloglik_dc"])
+    print(f"  solve 0 vs oracle: grad {e_g:.2e}")
+    assert e_g < tol
 
 
 def test_schedule_auto_fits_the_budget(prob_sched):
@@ -68,7 +70,7 @@ def test_schedule_auto_fits_the_budget(prob_sched):
     plan = make_plan(prob, "fp32", adjoint_mode="reference", checkpoint_every=40)
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
     B, pitch = len(srcs), prob.rec_xz.shape[0]
-    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     ws_bytes = plan.workspace_bytes
     # a large budget: the whole batch with its full history, one group per lane
     assert plan.schedule_auto(B, pitch, 1 << 40) == {"checkpoint_every": 0, "group": -(-B // 3), "lanes": 3}
@@ -125,7 +127,7 @@ def test_streaming_hints_give_the_same_bits(prob_sched):
     plan = make_plan(prob, "fp32", adjoint_mode="exact", checkpoint_every=40)
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
     B = len(srcs)
-    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     outs = []
     for mode in ("plain", "nontemporal", "auto"):
         plan.set_streaming(mode)
@@ -145,7 +147,7 @@ def test_misfit_only_call_skips_the_adjoint(prob_sched):
     plan = make_plan(prob, "fp32", adjoint_mode="reference", checkpoint_every=40)
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
     B = len(srcs)
-    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     for sch in ((40, 0, 1), (0, 3, 3)):
         plan.set_schedule(*sch)
         full = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
@@ -168,7 +170,7 @@ def test_schedules_with_a_misfit_window(prob_sched, mode):
                 adjoint_mode=mode, checkpoint_every=30, misfit_window=(40, 200, 3))
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
     B = len(srcs)
-    obs = torch.as_tensor(seis * 0.95, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis * 0.95, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     outs = []
     for sch in ((30, 0, 1), (0, 2, 3), (45, 3, 2)):
         plan.set_schedule(*sch)
@@ -190,7 +192,7 @@ def test_schedules_with_per_solve_wavelets(prob_sched):
     C2, w, srcs, model_of, rs = _batch(prob, disc, coef, c, plan)
     B = len(srcs)
     W2 = torch.stack([w * (1.0 + 0.1 * k) for k in range(B)]).contiguous()
-    obs = torch.as_tensor(seis, dtype=plan.dtype, device=plan.device)[None].expand(B, -1, -1)
+    obs = torch.as_tensor(seis, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
     outs = []
     for sch in ((50, 0, 1), (0, 2, 3), (0, 3, 1)):
         plan.set_schedule(*sch)

@@ -83,9 +83,10 @@ def test_served_request_matches_oracle():
     c, Bz, Bx = O.tensor_bspline_velocity(d.mesh, r.coef)
     ref = O.forward_adjoint(d, c, r.obs, r.noise_std, mode="reference")
     gref = Bz.T @ ref["dloglik_dc"].reshape(d.mesh.NZ, d.mesh.NX) @ Bx
-    assert rel_l2(res.seis, ref["seis"]) < 1e-5
-    assert rel_l2(res.grad, gref) < 1e-4
-    assert abs(res.loglik - ref["loglik"]) < 1e-4 * abs(ref["loglik"])
+    es, eg = rel_l2(res.seis, ref["seis"]), rel_l2(res.grad, gref)
+    el = abs(res.loglik - ref["loglik"]) / abs(ref["loglik"])
+    print(f"  seis {es:.2e} grad {eg:.2e} loglik {el:.2e}")
+    assert es < 1e-5 and eg < 1e-5 and el < 1e-5
 
 
 def test_history_and_checkpoint_slots_give_the_same_bits():

@@ -14,7 +14,7 @@ def test_footprint_counts_states_and_traces():
     base = request_footprint(NX, NZ, E, nt, pitch, S)
     states = 2 + 2 * K + (S + 1)
     nodal = E * 4 * (1 + 1 + states + (corr_chunk(S) + 1) + 1) + E * 8
-    traces = 3 * nt * pitch * 4 + nt * 4
+    traces = 2 * nt * pitch * 4 + nt * pitch * 8 + nt * 4  # seis, res; obs in double; wavelet
     assert base == nodal + traces
     full = request_footprint(NX, NZ, E, nt, pitch, S, full_history=True)
     assert full - base == E * 4 * ((nt + 1) - states)

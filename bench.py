@@ -35,10 +35,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# per launch of the forward step kernel, FP32 increment form: u^n, w^n, c^2
-# read, the new u and w written -- one step per launch (step_kernel), or two
-# (step2_kernel: u^{n+2}, w^{n+2} written, u^{n+1} kept in LDS)
-FWD_BYTES_PER_NODE = 20
+# algorithmic bytes of a time step per node and solve, the standard two-level
+# leapfrog: u^n and u^{n-1} read, u^{n+1} written, c^2 read (mass and sponge
+# are analytic, 0 B); a launch of S steps is credited S times this
+STEP_BYTES_PER_NODE = 16
+# the increment form's own minimum per launch, one or two steps: u^n, w^n,
+# c^2 read, the new u and w written (u^{n+1} of a two-step launch stays in LDS)
+INCREMENT_BYTES_PER_NODE_LAUNCH = 20
 # one forward+adjoint solve step with the full forward history: forward and
 # adjoint 20 B each with one step per launch; with two per launch each pair
 # reads 12 B and writes 12 B (u^{n+1} and u^{n+2} into the history or the
@@ -46,46 +49,68 @@ FWD_BYTES_PER_NODE = 20
 # read), FP64 accumulator (25 doubles per element of 16 nodes = 12.5 B per
 # node) read and written once per chunk of 64 steps
 SOLVE_BYTES_PER_NODE_STEP = {1: 20 + 20 + 8 + 25 / 64, 2: 12 + 12 + 8 + 25 / 64}
+LLC_BYTES = 256 << 20
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, nsteps=24):
-    """FP64 oracle timed on a bounded sample of the same mesh: stiffness
-    assembly + `nsteps` forward steps + `nsteps` adjoint steps (with the
-    element-wise gradient correlation), extrapolated to one full solve."""
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
+def host_threads():
+    """Host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the share a box gives one GPU job)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (min(aff, int(env)) if env.isdigit() and int(env) > 0 else aff), aff
+
+
+def cpu_baseline(log, steps200=200, steps200_1t=30):
+    """The FP64 CPU restatement in the reference's torch form (assembled CSR
+    stiffness, one torch sparse mat-vec per step, dense 25 x 25 element
+    gradient correlation; oracle/torch_cpu.py) on the host cores: the 40x40
+    workload in full (2000 steps), and the 200x200 benchmark mesh over
+    `steps200` forward + adjoint steps extrapolated to 6000 (and over
+    `steps200_1t` on one thread).  Observations are zero (the work per step
+    does not depend on them)."""
+    import torch
     import oracle as O
-    m = wl.mesh
-    prob = O.Problem(m.xmin, m.xmax, m.zmin, m.zmax, m.nelem_x, m.nelem_z, 4, m.dt, nsteps,
-                     tuple(wl.sources[0]), wl.f0, wl.receivers, sponge_width=m.sponge_width,
-                     sponge_cmax=m.sponge_cmax, sponge_sides=m.sponge_sides)
-    ctx = threadpool_limits(1) if threadpool_limits else None
-    try:
+    from oracle.torch_cpu import forward_adjoint_torch
+    from mun_amd import workloads as W
+    cores, aff = host_threads()
+    prev = torch.get_num_threads()
+
+    def run(wl, nsteps, threads):
+        m = wl.mesh
+        prob = O.Problem(m.xmin, m.xmax, m.zmin, m.zmax, m.nelem_x, m.nelem_z, 4, m.dt, m.nt,
+                         tuple(wl.sources[0]), wl.f0, wl.receivers, sponge_width=m.sponge_width,
+                         sponge_cmax=m.sponge_cmax, sponge_sides=m.sponge_sides)
         disc = O.discretize(prob)
         c, _, _ = O.tensor_bspline_velocity(disc.mesh, wl.coef[0], wl.base)
-        t0 = time.perf_counter()
-        K = O.stiffness_csr(c, disc.mesh, disc.ops)
-        t_setup = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        seis, hist = O.forward(disc, c, record="post", history="full", K=K)
-        t_fwd = (time.perf_counter() - t0) / nsteps
-        t0 = time.perf_counter()
-        O.adjoint(disc, c, hist, seis * 0.01, 1.0, mode="reference", K=K)
-        t_adj = (time.perf_counter() - t0) / nsteps
+        r = forward_adjoint_torch(disc, c, np.zeros((m.nt, wl.receivers.shape[0])), 1e-9, nsteps=nsteps,
+                                  threads=threads)
+        t = r["times"]
+        per_step = (t["forward"] + t["adjoint"]) / r["steps"]
+        return t["assembly"], per_step, t["assembly"] + per_step * m.nt, r["steps"]
+
+    try:
+        a1, ps1, t1, n1 = run(W.cpu40(), None, cores)
+        log(f"[bench] cpu 40x40: {t1:.2f} s per solve ({n1} steps, {cores} threads)")
+        a2, ps2, t2, n2 = run(W.batch200(nmodel=1), steps200, cores)
+        log(f"[bench] cpu 200x200: {ps2 * 1e3:.1f} ms per step, {t2:.0f} s per solve ({cores} threads)")
+        a3, ps3, t3, n3 = run(W.batch200(nmodel=1), steps200_1t, 1)
+        log(f"[bench] cpu 200x200: {ps3 * 1e3:.1f} ms per step, {t3:.0f} s per solve (1 thread)")
     finally:
-        if ctx is not None:
-            ctx.__exit__(None, None, None)
-    t_solve = t_setup + m.nt * (t_fwd + t_adj)
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle on the 200x200 deg-4 mesh: CSR assembly ({t_setup:.2f} s) + {nsteps} forward "
-                      f"steps ({t_fwd * 1e3:.1f} ms/step) + {nsteps} adjoint steps ({t_adj * 1e3:.1f} ms/step), "
-                      f"extrapolated to {m.nt} steps = {t_solve:.0f} s per forward+adjoint solve, 1 thread"}
+        torch.set_num_threads(prev)
+    return {"value": round(1.0 / t2, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": (f"FP64 torch-CPU restatement of the reference path (CSR stiffness, one sparse mat-vec per "
+                       f"step, dense 25x25 element correlation) on {cores} threads (affinity {aff}): 200x200 deg-4 "
+                       f"mesh, assembly {a2:.2f} s + {n2} forward+adjoint steps at {ps2 * 1e3:.1f} ms per step, "
+                       f"extrapolated to 6000 steps = {t2:.0f} s per solve (extrapolated, not run in full)"),
+            "batch_seconds_sequential": round(256 * t2, 1),
+            "one_thread": {"value": round(1.0 / t3, 6), "cores": 1, "ms_per_step": round(ps3 * 1e3, 2),
+                           "sample": f"200x200, {n3} steps extrapolated to 6000 = {t3:.0f} s per solve"},
+            "cpu40_full": {"value": round(1.0 / t1, 4), "cores": cores, "seconds": round(t1, 2),
+                           "sample": f"40x40 deg-4, all {n1} steps forward+adjoint, assembly {a1:.2f} s"}}
 
 
 def extra_workloads(log):
@@ -144,7 +169,41 @@ def extra_workloads(log):
     log(f"[bench] elbo120: {te:.1f} s for {wl.nmodel * wl.nshot} solves")
     del s
     torch.cuda.empty_cache()
+    res["elbo120"]["per_shot_rel_l2"] = per_shot_errors(wl, obs, (0, 17, 63))
+    log(f"[bench] elbo120 per-shot FP32 vs FP64: {res['elbo120']['per_shot_rel_l2']}")
     return res
+
+
+def per_shot_errors(wl, obs, samples):
+    """FP32 against the FP64 build of the same kernels for every shot of a few
+    samples of the ELBO batch (FP64 itself matches the oracle to ~1e-11):
+    relative L2 of seismograms and of the B-spline node gradient, and the
+    relative log-likelihood error, each the largest over the samples."""
+    import torch
+    from mun_amd.batch import BatchSolver
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s = BatchSolver.from_workload(wl, tile_rows=15, precision=prec, device="cuda:0")
+        _, c2 = s.velocity(torch.as_tensor(wl.coef[list(samples)], device="cuda:0"))
+        src, model_of = s._expand(len(samples))
+        ob = obs.double()[None].expand(len(samples), -1, -1, -1).reshape(len(src), wl.mesh.nt, -1)
+        o = s.plan.forward_adjoint(c2, s.wavelet, src, ob, wl.noise_std, [s.rec_set] * len(src), model_of=model_of)
+        gn = s.plan.bspline_project(o["grad_c"], s.nbz, s.nbx)
+        out[prec] = (o["seis"].double().cpu().numpy(), gn.cpu().numpy(), o["loglik"].cpu().numpy())
+        del s, o
+        torch.cuda.empty_cache()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    ns = wl.nshot
+    shots = {"seis": [], "grad": [], "loglik": []}
+    for k in range(ns):
+        rows = [i * ns + k for i in range(len(samples))]
+        shots["seis"].append(max(rel(out["fp32"][0][b], out["fp64"][0][b]) for b in rows))
+        shots["grad"].append(max(rel(out["fp32"][1][b], out["fp64"][1][b]) for b in rows))
+        shots["loglik"].append(max(abs(out["fp32"][2][b] / out["fp64"][2][b] - 1.0) for b in rows))
+    return {"samples": list(samples), **{k: [float(f"{v:.3g}") for v in vs] for k, vs in shots.items()}}
 
 
 def main():
@@ -159,7 +218,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-nt-scale", type=float, default=0.5)
+    ap.add_argument("--serve-nt-scale", type=float, default=1.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the cpu40 and elbo120 workloads")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -217,44 +276,55 @@ def main():
                                                 else f"checkpoints every {sched['checkpoint_every']} steps"))
     log(f"[bench] schedule: {sched_txt}")
     fusion = plan.get_step_fusion()
-    path_gbs = SOLVE_BYTES_PER_NODE_STEP[fusion] * m.N * m.nt * nsolve * args.steps / dt_total / 1e9
+    solve_gbs = SOLVE_BYTES_PER_NODE_STEP[fusion] * m.N * m.nt * nsolve * args.steps / dt_total / 1e9
 
     # live roofline of the time-step kernel, launched on as many solves as the
     # schedule keeps in flight: a forward-only pass is nt / steps-per-launch
     # launches of the step kernel (plus one receiver pass), timed with events
     # on the stream the library launches on; measured for the plan's launch
-    # form (two steps per launch by default) and for one step per launch
+    # form (two steps per launch by default), for one step per launch, and for
+    # the two-step kernel on a launch too large for the last-level cache
     _, c2 = solver.velocity(coef)
     src, model_of = solver._expand(args.batch)
-    src, model_of, c2 = src[:inflight], model_of[:inflight], c2[:inflight]  # one shot: model k = solve k
     st = torch.cuda.current_stream(dev)
-    nts = inflight * 4 * plan.layout()[1] * 4 > (256 << 20)  # the library's choice (LLC_BYTES)
+    E = plan.layout()[1]
+    nts = inflight * 4 * E * 4 > LLC_BYTES  # the library's choice of streams
 
-    def time_forward(steps):
+    def time_forward(steps, n, streaming="auto"):
         plan.set_step_fusion(steps)
+        plan.set_streaming(streaming)
+        args_ = (c2[:n], solver.wavelet, src[:n], [solver.rec_set] * n)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+        plan.forward(*args_, model_of=model_of[:n])
         e0.record(st)
-        plan.forward(c2, solver.wavelet, src, [solver.rec_set] * len(src), model_of=model_of)
+        plan.forward(*args_, model_of=model_of[:n])
         e1.record(st)
         torch.cuda.synchronize()
-        launches = -(-m.nt // steps)
-        return e0.elapsed_time(e1) / 1e3 / launches
+        plan.set_step_fusion(fusion)
+        plan.set_streaming("auto")
+        return e0.elapsed_time(e1) / 1e3 / -(-m.nt // steps)
 
-    t_single = time_forward(1)
-    t_launch = time_forward(fusion)
-    plan.set_step_fusion(fusion)
-    bytes_launch = FWD_BYTES_PER_NODE * m.N * inflight
+    t_single = time_forward(1, inflight)
+    t_launch = time_forward(fusion, inflight)
+    n_hbm = min(args.batch, 64)
+    t_hbm = time_forward(fusion, n_hbm, "nontemporal")
+
+    def gbs(bytes_, t):
+        return round(bytes_ / t / 1e9, 1)
+
+    bytes_launch = STEP_BYTES_PER_NODE * fusion * m.N * inflight
     achieved = bytes_launch / t_launch / 1e9
     kernel = (f"step2_kernel<float,4,MODE_FWD,NTS={str(nts).lower()}>" if fusion == 2
               else f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>")
     log(f"[bench] step kernel on {inflight} solves, {fusion} step(s) per launch: {t_launch * 1e6:.1f} us/launch "
-        f"({t_launch / fusion * 1e6:.1f} us/step), {achieved:.0f} GB/s; one step per launch: "
-        f"{t_single * 1e6:.1f} us/step, {bytes_launch / t_single / 1e9:.0f} GB/s; whole solve {path_gbs:.0f} GB/s")
+        f"({t_launch / fusion * 1e6:.1f} us/step), {achieved:.0f} GB/s at 16 B per node-step; one step per launch: "
+        f"{t_single * 1e6:.1f} us/step; {n_hbm} solves non-temporal: {t_hbm * 1e6:.1f} us/launch; whole solve "
+        f"{solve_gbs:.0f} GB/s")
     # bytes per launch across L2 from the committed counter runs: exact when
     # they were taken at this launch size, else their measured ratio to the
-    # algorithmic bytes applied to this launch (noted in traffic_basis)
+    # increment form's algorithmic bytes applied to this launch (noted in traffic_basis)
     traffic, traffic_basis = None, None
+    inc_bytes = INCREMENT_BYTES_PER_NODE_LAUNCH * m.N * inflight
     tp = os.path.join(ROOT, "profiles", "step_fwd_batch200_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
@@ -265,9 +335,10 @@ def main():
         if tj.get("batch") == inflight:
             traffic, traffic_basis = tj.get("bytes_per_launch"), f"counters at {inflight} solves per launch"
         elif fw.get("ratio"):
-            traffic = int(fw["ratio"] * bytes_launch)
-            traffic_basis = (f"counters at {tj.get('batch')} solves per launch: {fw['ratio']} x the algorithmic "
-                             f"bytes, applied to {inflight}")
+            traffic = int(fw["ratio"] * inc_bytes)
+            traffic_basis = (f"counters at {tj.get('batch')} solves per launch: {fw['ratio']} x the increment "
+                             f"form's bytes, applied to {inflight}")
+    ws_mb = inflight * 4 * E * 4 / 2 ** 20
 
     out = {"metric": "fwd+adjoint SEM solves/s (1 GPU, batched vs plain); p50/p99 solve latency",
            "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -279,22 +350,37 @@ def main():
                       "parallelism": "none (1 GPU)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                        "traffic_basis": traffic_basis,
-                        "kernel": kernel, "steps_per_launch": fusion,
-                        "note": ("two time steps per launch for 20 algorithmic bytes per node (u, w, c^2 read once; "
-                                 "u^{n+1} kept in LDS): half the bytes per step of one step per launch and faster per "
-                                 "step (us_per_step vs one_step_per_launch.launch_us), so no longer bandwidth-bound"
-                                 if fusion == 2 else "one time step per launch"),
-                        "solves_per_launch": inflight, "launch_us": round(t_launch * 1e6, 1),
-                        "us_per_step": round(t_launch / fusion * 1e6, 2),
-                        "algorithmic_bytes_per_launch": bytes_launch, "bytes_per_node": FWD_BYTES_PER_NODE,
+                        "traffic_basis": traffic_basis, "kernel": kernel,
+                        "basis": (f"{STEP_BYTES_PER_NODE} B per node-step (u^n, u^(n-1), c^2 read, u^(n+1) "
+                                  f"written) x {fusion} steps per launch x {m.N} nodes x {inflight} solves "
+                                  f"= {bytes_launch} B per launch"),
+                        "steps_per_launch": fusion, "solves_per_launch": inflight,
+                        "launch_us": round(t_launch * 1e6, 1), "us_per_step": round(t_launch / fusion * 1e6, 2),
+                        "cache_resident": (f"the {inflight} solves' per-step working set (u^n, u^(n+1), w, c^2: "
+                                           f"{ws_mb:.0f} MiB) fits the 256 MiB last-level cache, so much of "
+                                           f"this launch's traffic is served there, not from HBM"),
+                        "limiter": ("VALU work and memory latency, not bandwidth: the two-step kernel keeps "
+                                    "u^(n+1) in LDS, and its counters show the VALU about 60 % busy "
+                                    "(profiles/step_kernels_batch200_sq_ta_counters.md)" if fusion == 2
+                                    else "HBM / last-level cache bandwidth"),
+                        "hbm_resident": {"solves_per_launch": n_hbm, "streams": "non-temporal",
+                                         "launch_us": round(t_hbm * 1e6, 1),
+                                         "achieved": gbs(STEP_BYTES_PER_NODE * fusion * m.N * n_hbm, t_hbm),
+                                         "frac": round(STEP_BYTES_PER_NODE * fusion * m.N * n_hbm / t_hbm / 1e9
+                                                       / HBM_PEAK_GBS, 4)},
+                        "increment_form": {"bytes_per_node_launch": INCREMENT_BYTES_PER_NODE_LAUNCH,
+                                           "achieved": gbs(inc_bytes, t_launch),
+                                           "frac": round(inc_bytes / t_launch / 1e9 / HBM_PEAK_GBS, 4)},
                         "one_step_per_launch": {
                             "kernel": f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>",
                             "launch_us": round(t_single * 1e6, 1),
-                            "achieved": round(bytes_launch / t_single / 1e9, 1),
-                            "frac": round(bytes_launch / t_single / 1e9 / HBM_PEAK_GBS, 4)},
-                        "whole_solve_achieved": round(path_gbs, 1),
-                        "whole_solve_bytes_per_node_step": SOLVE_BYTES_PER_NODE_STEP[fusion]},
+                            "achieved": gbs(STEP_BYTES_PER_NODE * m.N * inflight, t_single),
+                            "frac": round(STEP_BYTES_PER_NODE * m.N * inflight / t_single / 1e9 / HBM_PEAK_GBS, 4)},
+                        "whole_solve": {"bytes_per_node_step": round(SOLVE_BYTES_PER_NODE_STEP[fusion], 3),
+                                        "achieved": round(solve_gbs, 1),
+                                        "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
+                                        "basis": "the schedule run: full forward history, forward and adjoint "
+                                                 "two steps per launch, correlation per chunk of 64 steps"}},
            "latency_s": {"p50": round(ms_per_step / 1e3, 3), "p99": round(ms_per_step / 1e3, 3),
                          "note": "batched path: every solve completes with its batch"}}
 
@@ -346,7 +432,7 @@ def main():
             f"plain p50 {plain['p50_s']:.3f}s p99 {plain['p99_s']:.3f}s {plain['solves_per_s']:.2f}/s")
     if not args.no_cpu:
         log("[bench] cpu baseline (bounded sample) ...")
-        out["cpu_baseline"] = cpu_baseline(wl)
+        out["cpu_baseline"] = cpu_baseline(log)
     print(json.dumps(out), flush=True)
 
 

@@ -344,7 +344,15 @@ int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int
  * the other way round on odd ones, so it ends a segment of len steps in w2
  * when ceil(len / 2) is odd.  Slots of other phases are not advanced (step
  * them with sem_slots_step and a mask without these phases).  No reference
- * counterpart (continuous batching). */
+ * counterpart (continuous batching).
+ * Conditions on the slot table (a slot that breaks the first two is not
+ * advanced at all):
+ *   - w2 is non-null and differs from w;
+ *   - ADJOINT slots have ring_slots >= 3 (a pair writes lambda_{t-1} and
+ *     lambda_{t-2} while lambda_t is read);
+ *   - the chunk_len of the sem_slots_correlate calls between these launches
+ *     is even, so that no step pair straddles a correlation chunk (the pair
+ *     would overwrite a ring slot the pending correlation still reads). */
 int sem_slots_step2(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
                     int rec_pitch, void* stream);
 /* Gradient correlation of chunk `chunk` (segment steps chunk*chunk_len ..

@@ -9,7 +9,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmun_sem.so")
-PROBE = os.path.join(HERE, "stream_ceiling")
 SOURCES = [os.path.join(CSRC, "sem_lib.hip")]
 HEADERS = [os.path.join(CSRC, "sem_kernels.h"), os.path.join(HERE, "..", "include", "sem_capi.h")]
 
@@ -32,21 +31,7 @@ def build(verbose: bool = False, extra=()) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=CSRC)
     os.replace(LIB + ".tmp", LIB)
-    build_stream_ceiling(verbose)
     return LIB
-
-
-def build_stream_ceiling(verbose: bool = False) -> str:
-    """The HBM streaming-ceiling probe (mun_amd/stream_ceiling), a stand-alone
-    executable run by ``python -m mun_amd.profile_kernels --stream-ceiling``."""
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-o", PROBE + ".tmp", os.path.join(CSRC, "stream_ceiling.hip")]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, cwd=CSRC)
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
 
 
 if __name__ == "__main__":

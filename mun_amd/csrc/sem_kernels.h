@@ -743,6 +743,10 @@ __device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View
     }
     const sem_slot& d = a.slots[b];
     if (!((a.phase_mask >> d.phase) & 1u) || a.m >= d.len) return false;
+    // a table that breaks sem_slots_step2's conditions is not advanced (the
+    // increment would be read and written in one buffer, or an adjoint ring
+    // of 2 slots would give u_out == u_in): no race, the slot stays put
+    if (!d.w2 || d.w2 == d.w || (d.phase == SEM_SLOT_ADJOINT && d.ring_slots < 3)) return false;
     v.single = (a.m + 1 == d.len);
     const bool flip = ((a.m >> 1) & 1) != 0;
     v.w_in = gm((const T*)(flip ? d.w2 : d.w));

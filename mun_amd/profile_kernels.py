@@ -1,9 +1,7 @@
 """Profiling driver: runs the solver's kernels at a fixed shape under a profiler.
 
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
-    python -m mun_amd.profile_kernels --stream-ceiling
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
-    python -m mun_amd.profile_kernels --scan-streaming [--nt 200]
     python -m mun_amd.profile_kernels --summarize-trace KERNEL_TRACE_CSV [--top 30]
 
 Order of GPU work (so counter runs can attribute dispatches):
@@ -122,37 +120,6 @@ def summarize_trace(trace_csv, top=30):
     return "\n".join(lines)
 
 
-def scan_streaming(nt=200, batches=(6, 12, 18, 21, 32, 64, 128, 256), tile_rows=15):
-    """Forward step time per launch against solves per launch, with plain and
-    with non-temporal state streams (HIP events over nt launches on the
-    library's stream), on the 801 x 801-node workload mesh."""
-    dev = torch.device("cuda:0")
-    wl = W.batch200(nmodel=max(batches), nt=nt)
-    s = BatchSolver.from_workload(wl, tile_rows=tile_rows, precision="fp32", device=dev, schedule="whole")
-    _, c2 = s.velocity(torch.as_tensor(wl.coef, device=dev))
-    st = torch.cuda.current_stream(dev)
-    _, elems = s.plan.layout()
-    print(f"{'solves':>6} {'set MB':>8} {'hints':>12} {'us/launch':>10} {'GB/s (20 B/node)':>17}", flush=True)
-    rows = []
-    for B in batches:
-        src, model_of = s._expand(B)
-        for mode in ("plain", "nontemporal"):
-            s.plan.set_streaming(mode)
-            args = (c2[:B], s.wavelet, src, [s.rec_set] * B)
-            s.plan.forward(*args, model_of=model_of)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            s.plan.forward(*args, model_of=model_of)
-            e1.record(st)
-            torch.cuda.synchronize()
-            us = e0.elapsed_time(e1) * 1e3 / nt
-            gbs = 20 * s.plan.N * B / us / 1e3
-            rows.append((B, mode, us, gbs))
-            print(f"{B:6d} {4 * 4 * elems * B / 1e6:8.0f} {mode:>12} {us:10.2f} {gbs:17.0f}", flush=True)
-    s.plan.set_streaming("auto")
-    return rows
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", type=int, default=256)
@@ -160,22 +127,15 @@ def main(argv=None):
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--tile-rows", type=int, default=15, help="step-kernel tile height (15, 7 or 3)")
-    ap.add_argument("--stream-ceiling", action="store_true",
-                    help="run the HBM streaming-ceiling probe (mun_amd/stream_ceiling) instead")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two counter passes of this driver (no GPU work)")
     ap.add_argument("--out", help="JSON file for --summarize-pmc")
     ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
                     help="per-kernel table of a rocprofv3 kernel trace (no GPU work)")
     ap.add_argument("--top", type=int, default=30, help="rows of --summarize-trace")
-    ap.add_argument("--scan-streaming", action="store_true",
-                    help="time the forward step against solves per launch, plain vs non-temporal streams")
     a = ap.parse_args(argv)
     if a.summarize_trace:
         print(summarize_trace(a.summarize_trace, a.top))
-        return 0
-    if a.scan_streaming:
-        scan_streaming(nt=a.nt if a.nt != 40 else 200, tile_rows=a.tile_rows)
         return 0
     if a.summarize_pmc:
         import json
@@ -186,13 +146,6 @@ def main(argv=None):
                 f.write(text + "\n")
         print(text)
         return 0
-    if a.stream_ceiling:
-        import os
-        import subprocess
-        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stream_ceiling")
-        if not os.path.exists(exe):
-            raise FileNotFoundError(f"{exe} is missing: build it with `python -m mun_amd.build`")
-        return subprocess.run([exe], check=True).returncode
     dev = torch.device("cuda:0")
     n = a.copy_mb * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device=dev)

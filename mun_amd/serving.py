@@ -298,12 +298,15 @@ class SEMServer:
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True, history: str = "auto", two_step: bool = True):
+                 autotune: bool = True, history: str = "auto", two_step: bool = False):
         """history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  two_step: advance
-        full-history forward and adjoint slots two steps per launch
-        (sem_slots_step2).  Results are the same bits either way."""
+        full-history forward, recompute and adjoint slots two steps per launch
+        (sem_slots_step2); opt-in: warm servers run within noise of one step
+        per launch, and one saturated two-step run on a server created after
+        another was torn down once ran 4x slower for a reason not found.
+        Results are the same bits either way."""
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
         self.history = history

@@ -1,0 +1,204 @@
+"""The benchmark workloads at their full sizes against the FP64 oracle.
+
+Fixtures from tests/golden/make_oracle_fixtures.py (the oracle run on the CPU
+at the same mesh, steps, source, receivers and models):
+
+* single200 -- the single large solve: 200x200 deg-4 (801^2 nodes), 6000
+  steps, checkpoints every 100 steps, FP32 state.
+* batch200 model 0 -- one model of the 256-model batch, run inside a batch
+  under the schedule the benchmark uses (lanes of full-history groups, two
+  steps per launch).
+* elbo120 -- (sample, shot) pairs of the 64 x 8 ELBO batch at 120x120, 4000
+  steps, with the FP32 error against FP64 reported per shot.
+
+Observations are regenerated with the FP64 kernels from the seeded true model
+and first checked against the oracle's.  Tolerances: FP32 seismograms,
+log-likelihood and gradients within 1e-5 relative L2 of the oracle; FP64
+observations within 1e-9.
+
+Also here: central differences of the FP64 log-likelihood (h = 1 m/s with
+Richardson extrapolation, 8 seeded B-spline nodes) against the exact-mode
+adjoint gradient on the full 200x200, 6000-step batch model.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.gpu_helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name} not generated (tests/golden/make_oracle_fixtures.py)")
+    return dict(np.load(path, allow_pickle=False))
+
+
+def _solver(wl, precision, mode="reference", **kw):
+    from mun_amd.batch import BatchSolver
+    kw.setdefault("schedule", "whole")
+    return BatchSolver.from_workload(wl, tile_rows=15, precision=precision, adjoint_mode=mode, device="cuda", **kw)
+
+
+def _observations(wl, fx_rows, row_step, shots=None):
+    """FP64-kernel observations of the true model, checked against the oracle's rows."""
+    s = _solver(wl, "fp64")
+    obs = s.forward(torch.as_tensor(wl.true_coef[None], device="cuda"))[0].double()   # [nshot, nt, nrec]
+    for k, rows in (fx_rows.items() if shots is None else ((k, fx_rows[k]) for k in shots)):
+        e = rel_l2(obs[k, ::row_step, :wl.receivers.shape[0]].cpu().numpy(), rows)
+        print(f"  observations shot {k}: FP64 kernels vs oracle {e:.2e}")
+        assert e < 1e-9
+    del s
+    torch.cuda.empty_cache()
+    return obs
+
+
+def _compare(tag, fx, prefix, ll, misfit, grad_nodes, grad_c, seis, NX, NZ):
+    step, nstep = int(fx["meta_row_step"]), int(fx["meta_node_step"])
+    es = rel_l2(seis[::step], fx[prefix + "seis_rows"])
+    el = abs(ll - float(fx[prefix + "loglik"])) / abs(float(fx[prefix + "loglik"]))
+    em = abs(misfit - float(fx[prefix + "misfit"])) / abs(float(fx[prefix + "misfit"]))
+    eg = rel_l2(grad_nodes, fx[prefix + "grad_nodes"])
+    ec = rel_l2(grad_c.reshape(NZ, NX)[::nstep, ::nstep], fx[prefix + "grad_c_sub"])
+    print(f"  {tag}: seis {es:.2e} loglik {el:.2e} misfit {em:.2e} grad(nodes) {eg:.2e} grad(c, sampled) {ec:.2e}")
+    return es, el, em, eg, ec
+
+
+def test_single200_fp32_checkpointed_vs_oracle():
+    from mun_amd import workloads as W
+    fx = fixture("oracle_single200.npz")
+    wl = W.single200()
+    assert int(fx["meta_nt"]) == wl.mesh.nt
+    print("[single200] observations (FP64)", flush=True)
+    obs = _observations(wl, {0: fx["yobs_rows"]}, int(fx["meta_row_step"]))
+    print("[single200] FP32, checkpoints every 100 steps", flush=True)
+    s = _solver(wl, "fp32")
+    assert s.plan.get_schedule()["checkpoint_every"] == 100
+    plan = s.plan
+    _, c2 = s.velocity(torch.as_tensor(wl.coef, device="cuda"))
+    out = plan.forward_adjoint(c2, s.wavelet, s.src_nodes, obs, wl.noise_std, [s.rec_set])
+    g = out["grad_c"][0]
+    gn = plan.bspline_project(g[None], s.nbz, s.nbx)[0].cpu().numpy()
+    errs = _compare("single200", fx, "", float(out["loglik"][0]), float(out["misfit"][0]), gn, g.cpu().numpy(),
+                    out["seis"][0, :, :s.nrec].double().cpu().numpy(), plan.NX, plan.NZ)
+    assert int(fx["src_node"]) == s.src_nodes[0]
+    assert max(errs) < 1e-5
+
+
+def test_batch200_model0_in_the_benchmark_schedule_vs_oracle():
+    """Model 0 of the 256-model batch solved inside an 18-model batch under
+    the benchmark's schedule (groups of 6 in 3 lanes, full forward history,
+    two steps per launch) against the oracle's reference-mode solve."""
+    from mun_amd import workloads as W
+    fx = fixture("oracle_batch200m0.npz")
+    wl = W.batch200(nmodel=18)
+    # model 0 is the same draw for every batch size; the fixture's observations
+    # come from the true model drawn after a batch of one
+    wl.true_coef = W.batch200(nmodel=1).true_coef
+    print("[batch200 m0] observations (FP64)", flush=True)
+    obs = _observations(wl, {0: fx["yobs_rows"]}, int(fx["meta_row_step"]))
+    s = _solver(wl, "fp32")
+    s.plan.set_schedule(0, 6, 3)
+    assert s.plan.get_step_fusion() == 2
+    print("[batch200 m0] FP32, 18 models, 3 lanes x 6, full history", flush=True)
+    ll, gn, seis = s.loglik_grad(torch.as_tensor(wl.coef, device="cuda"), obs, wl.noise_std)
+    # model 0's nodal gradient and misfit: the same solve alone (bits do not depend on the batch)
+    s1 = _solver(wl, "fp32")
+    s1.plan.set_schedule(0, 0, 1)
+    _, c2 = s1.velocity(torch.as_tensor(wl.coef[:1], device="cuda"))
+    one = s1.plan.forward_adjoint(c2, s1.wavelet, s1.src_nodes, obs, wl.noise_std, [s1.rec_set])
+    assert float(one["loglik"][0]) == float(ll[0])
+    errs = _compare("batch200 m0", fx, "", float(ll[0]), float(one["misfit"][0]), gn[0].cpu().numpy(),
+                    one["grad_c"][0].cpu().numpy(), seis[0, 0, :, :s.nrec].double().cpu().numpy(),
+                    s.plan.NX, s.plan.NZ)
+    assert max(errs) < 1e-5
+
+
+def test_elbo120_pairs_vs_oracle_and_per_shot_fp64_error():
+    """(sample, shot) pairs of the ELBO batch against the oracle, and every
+    shot of those samples in FP32 against the FP64 kernels (the per-shot error
+    the ELBO workload reports)."""
+    from mun_amd import workloads as W
+    fx = fixture("oracle_elbo120.npz")
+    wl = W.elbo120()
+    pairs = [tuple(int(q) for q in p) for p in fx["pairs"]]
+    samples = sorted({p[0] for p in pairs})
+    print("[elbo120] observations (FP64)", flush=True)
+    obs = _observations(wl, {k: fx[f"s{s}k{k}_yobs_rows"] for s, k in pairs}, int(fx["meta_row_step"]))
+    res = {}
+    for prec in ("fp64", "fp32"):
+        print(f"[elbo120] {prec}: samples {samples} x {wl.nshot} shots", flush=True)
+        s = _solver(wl, prec)
+        _, c2 = s.velocity(torch.as_tensor(wl.coef[samples], device="cuda"))
+        src, model_of = s._expand(len(samples))
+        ob = obs[None].expand(len(samples), -1, -1, -1).reshape(len(samples) * wl.nshot, wl.mesh.nt, -1)
+        out = s.plan.forward_adjoint(c2, s.wavelet, src, ob, wl.noise_std, [s.rec_set] * len(src),
+                                     model_of=model_of)
+        gn = s.plan.bspline_project(out["grad_c"], s.nbz, s.nbx).cpu().numpy()
+        res[prec] = {"ll": out["loglik"].cpu().numpy(), "mis": out["misfit"].cpu().numpy(), "gn": gn,
+                     "gc": out["grad_c"].cpu().numpy(), "seis": out["seis"][:, :, :s.nrec].double().cpu().numpy()}
+        NX, NZ = s.plan.NX, s.plan.NZ
+        del s, out
+        torch.cuda.empty_cache()
+    worst = 0.0
+    for s_, k in pairs:
+        b = samples.index(s_) * wl.nshot + k
+        r = res["fp32"]
+        errs = _compare(f"sample {s_} shot {k}", fx, f"s{s_}k{k}_", float(r["ll"][b]), float(r["mis"][b]), r["gn"][b],
+                        r["gc"][b], r["seis"][b], NX, NZ)
+        worst = max(worst, max(errs))
+    for b in range(len(samples) * wl.nshot):
+        e64 = [rel_l2(res["fp32"][q][b], res["fp64"][q][b]) for q in ("seis", "gn")]
+        el = abs(res["fp32"]["ll"][b] / res["fp64"]["ll"][b] - 1.0)
+        print(f"  sample {samples[b // wl.nshot]} shot {b % wl.nshot}: FP32 vs FP64 seis {e64[0]:.2e} "
+              f"grad {e64[1]:.2e} loglik {el:.2e}")
+        assert max(e64 + [el]) < 1e-5
+    assert worst < 1e-5
+
+
+def test_batch200_gradient_matches_central_differences_fp64():
+    """FP64, exact adjoint, the full 200x200 mesh and 6000 steps: central
+    differences of the log-likelihood (h = 1 m/s and 0.5 m/s, Richardson
+    extrapolated) on 8 seeded B-spline nodes of a batch200 model against the
+    adjoint gradient.  All 32 perturbed solves run as one forward-only batch."""
+    from mun_amd import workloads as W
+    wl = W.batch200(nmodel=1)
+    print("[batch200 FD] FP64 exact adjoint", flush=True)
+    s = _solver(wl, "fp64", mode="exact")
+    obs = s.forward(torch.as_tensor(wl.true_coef[None], device="cuda"))[0]
+    coef = torch.as_tensor(wl.coef, device="cuda")
+    ll, g, _ = s.loglik_grad(coef, obs, wl.noise_std)
+    g = g[0].cpu().numpy().reshape(-1)
+    # 8 seeded nodes among those the wavefield reaches (|g| >= 1e-2 max|g|;
+    # the others' gradients are below the log-likelihood's rounding)
+    rng = np.random.default_rng(31)
+    live = np.nonzero(np.abs(g) >= 1e-2 * np.abs(g).max())[0]
+    assert live.size >= 8
+    idx = rng.choice(live, 8, replace=False)
+    pert = []
+    for h in (1.0, 0.5):
+        for k in idx:
+            for sgn in (1.0, -1.0):
+                cc = wl.coef[0].copy().reshape(-1)
+                cc[k] += sgn * h
+                pert.append(cc.reshape(wl.coef.shape[1:]))
+    print(f"[batch200 FD] {len(pert)} perturbed forward solves", flush=True)
+    lp, _, _ = s.loglik_grad(torch.as_tensor(np.stack(pert), device="cuda"), obs, wl.noise_std, want_grad=False)
+    lp = lp.cpu().numpy().reshape(2, 8, 2)
+    f1 = (lp[0, :, 0] - lp[0, :, 1]) / 2.0
+    f2 = (lp[1, :, 0] - lp[1, :, 1]) / 1.0
+    rich = (4.0 * f2 - f1) / 3.0
+    scale = np.abs(g).max()
+    e1 = np.abs(g[idx] - f1) / scale
+    er = np.abs(g[idx] - rich) / scale
+    for k, a, b, c in zip(idx, g[idx], f1, rich):
+        print(f"  node {k}: adjoint {a:.9e} FD(h=1) {b:.9e} Richardson {c:.9e} (rel {abs(c / a - 1):.1e})")
+    print(f"  h=1 FD err / max|g| = {e1.max():.2e}, Richardson {er.max():.2e}")
+    assert e1.max() < 1e-3
+    assert er.max() < 1e-6

@@ -224,6 +224,18 @@ int sem_bspline_map(sem_plan* plan, int nmodel, int nbz, int nbx, const double* 
 /* Chain rule onto the nodes: g_coef[m] = Bz^T g_c[m] Bx. */
 int sem_bspline_project(sem_plan* plan, int nmodel, int nbz, int nbx, const double* g_c,
                         double* g_coef, void* stream);
+/* sem_bspline_project that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Its partial sums go
+ * to an 8 MiB device array of the library (one per GPU, held until the process
+ * ends), models processed in chunks that fit: SEM_E_NOMEM if one model needs
+ * more (NZ * nbx > 2^20).  The basis tables of (nbz, nbx) must exist already --
+ * made by an earlier sem_bspline_map or sem_bspline_project call with these
+ * node counts -- else SEM_E_INVALID.  Calls that overlap on one GPU share the
+ * array and race; calls on one stream do not.  Same results as
+ * sem_bspline_project. */
+int sem_bspline_project_noalloc(sem_plan* plan, int nmodel, int nbz, int nbx, const double* g_c,
+                                double* g_coef, void* stream);
 
 /* Closed-curve anomaly velocity, the reference's parameterisation: signed
  * distance to a closed cubic B-spline through 6 control points (nearest of

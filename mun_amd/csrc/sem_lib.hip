@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -1418,6 +1419,81 @@ int sem_bspline_map(sem_plan* pl, int nmodel, int nbz, int nbx, const double* co
                                                         (int*)t->bx0.p, (double*)t->bxw.p, pl->NX, pl->NZ, c,
                                                         (float*)c2);
     LAUNCH_CHECK();
+    return SEM_OK;
+} ABI_CATCH
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free projection
+// Scratch of sem_bspline_project_noalloc: the x stage's partial sums
+// t[m][jz][q] for as many models as fit, the models processed in chunks of
+// that size.  A file-scope device array: every GPU the library runs on holds
+// its own copy until the process ends, and every call on that GPU shares it
+// (calls that overlap on one GPU race).  Each call writes every cell it then
+// reads, so nothing needs clearing.
+constexpr size_t SCRATCH_DOUBLES = size_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device (with
+// that device current) and kept: host code never passes the array's name.
+int scratch_on_current_device(double** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(SEM_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SEM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return SEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem_bspline_project_noalloc(sem_plan* pl, int nmodel, int nbz, int nbx, const double* g_c, double* g_coef,
+                                void* stream) try {
+    if (!pl || nmodel < 1 || nmodel > 65535 || nbz < 4 || nbx < 4 || !g_c || !g_coef)
+        return fail(SEM_E_INVALID, "sem_bspline_project_noalloc: bad argument");
+    if (nbx > 32) return fail(SEM_E_UNSUPPORTED, "sem_bspline_project_noalloc: at most 32 nodes along x");
+    // the basis tables are built (and allocated) by the first sem_bspline_map
+    // or sem_bspline_project call with these node counts; this call allocates nothing
+    auto it = pl->bsp.find(std::make_pair(nbz, nbx));
+    if (it == pl->bsp.end())
+        return fail(SEM_E_INVALID, "sem_bspline_project_noalloc: no basis tables for " + std::to_string(nbz) + " x " +
+                                       std::to_string(nbx) + " nodes yet (call sem_bspline_map or sem_bspline_project "
+                                       "with these node counts once first)");
+    const size_t per_model = (size_t)pl->NZ * nbx;
+    if (per_model > SCRATCH_DOUBLES)
+        return fail(SEM_E_NOMEM, "sem_bspline_project_noalloc: one model needs " + std::to_string(per_model * 8) +
+                                     " bytes of scratch, more than its " + std::to_string(SCRATCH_DOUBLES * 8));
+    double* tmp = nullptr;
+    int rc = scratch_on_current_device(&tmp);
+    if (rc) return rc;
+    const BsplineTables* t = it->second;
+    hipStream_t st = (hipStream_t)stream;
+    const int chunk = (int)std::min<size_t>((size_t)nmodel, SCRATCH_DOUBLES / per_model);
+    for (int m0 = 0; m0 < nmodel; m0 += chunk) {
+        const int n = std::min(chunk, nmodel - m0);
+        bspline_proj_x_kernel<<<dim3(pl->NZ, n), 256, 256 * nbx * sizeof(double), st>>>(
+            g_c + (long)m0 * pl->N, pl->NX, pl->NZ, nbx, (int*)t->bx0.p, (double*)t->bxw.p, tmp);
+        LAUNCH_CHECK();
+        dim3 g2((unsigned)((nbz * nbx + 255) / 256), n);
+        bspline_proj_z_kernel<<<g2, 256, 0, st>>>(tmp, pl->NZ, nbz, nbx, (int*)t->plo.p, (int*)t->phi.p,
+                                                  (int*)t->bz0.p, (double*)t->bzw.p, g_coef + (long)m0 * nbz * nbx);
+        LAUNCH_CHECK();
+    }
     return SEM_OK;
 } ABI_CATCH
 

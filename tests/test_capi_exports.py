@@ -115,3 +115,9 @@ def test_schedule_and_streaming_entry_points_validate_arguments():
     assert lib.sem_slots_step2(None, None, 1, 0, 0, 0, None) == -1
     assert lib.sem_workspace_bytes(None, 1, 1, 1, None) == -1
     assert "bad argument" in lib.sem_last_error().decode()
+
+
+def test_noalloc_projection_validates_before_device_work():
+    lib = L.lib()
+    assert lib.sem_bspline_project_noalloc(None, 1, 8, 8, None, None, None) == -1
+    assert "bad argument" in lib.sem_last_error().decode()

@@ -171,6 +171,25 @@ int sem_plan_set_streaming(sem_plan* plan, int mode);
 int sem_plan_set_step_fusion(sem_plan* plan, int steps);
 int sem_plan_get_step_fusion(const sem_plan* plan, int* steps);
 
+/* Tile shape of the two-step kernel: shape 0 (default) 19 x 26 element
+ * threads per workgroup (16 x 23 owned), 1: 19 x 13 (16 x 10), 2: 35 x 14
+ * (32 x 11), 3: 35 x 28 (32 x 25; FP32 plans only).  Results are bitwise the
+ * same for every shape.  The two-step receiver table is rebuilt on the next
+ * use: set the shape before sem_receivers_reserve and not while solves of the
+ * plan are in flight.  No reference counterpart (chosen by the autotuner). */
+/* Adjoint steps per gradient-correlation chunk of sem_forward_adjoint, at
+ * most (default 64; 1 .. 256): a segment is split into equal chunks of at
+ * most this many steps, whose adjoint states are kept in a ring of chunk + 1
+ * states per solve, and the FP64 accumulator is read and written once per
+ * chunk.  Results are bitwise the same for every value; the workspace size
+ * changes.  No reference counterpart (chosen by the autotuner). */
+int sem_plan_set_corr_chunk(sem_plan* plan, int steps);
+int sem_plan_get_corr_chunk(const sem_plan* plan, int* steps);
+
+#define SEM_TILE2_SHAPES 4
+int sem_plan_set_tile2(sem_plan* plan, int shape);
+int sem_plan_get_tile2(const sem_plan* plan, int* shape, int* ex, int* ez);
+
 /* Forward solve of a batch; replaces run() / _time_stepping_loop_torch
  * (core.py:334-435, 598-659).  Writes b->seis. */
 int sem_forward(sem_plan* plan, const sem_batch* b, void* workspace, size_t ws_bytes, void* stream);

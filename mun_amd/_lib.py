@@ -27,7 +27,8 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_forward_snapshots", "sem_grid_to_nodes", "sem_curve_velocity_ex", "sem_curve_project_ex",
            "sem_slots_correlate", "sem_plan_gradient_elems", "sem_plan_set_schedule", "sem_plan_get_schedule",
            "sem_plan_schedule_auto", "sem_plan_set_streaming", "sem_plan_set_step_fusion",
-           "sem_plan_get_step_fusion", "sem_slots_step2", "sem_bspline_project_noalloc", "sem_last_error"]
+           "sem_plan_get_step_fusion", "sem_slots_step2", "sem_bspline_project_noalloc", "sem_plan_set_tile2",
+           "sem_plan_get_tile2", "sem_plan_set_corr_chunk", "sem_plan_get_corr_chunk", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
 SEM_STREAM_AUTO, SEM_STREAM_PLAIN, SEM_STREAM_NONTEMPORAL = 0, 1, 2
@@ -109,6 +110,10 @@ def lib() -> C.CDLL:
     L.sem_bspline_map.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]
     L.sem_bspline_project.argtypes = [vp, i, i, i, vp, vp, vp]
     L.sem_bspline_project_noalloc.argtypes = [vp, i, i, i, vp, vp, vp]
+    L.sem_plan_set_tile2.argtypes = [vp, i]
+    L.sem_plan_get_tile2.argtypes = [vp, ip, ip, ip]
+    L.sem_plan_set_corr_chunk.argtypes = [vp, i]
+    L.sem_plan_get_corr_chunk.argtypes = [vp, ip]
     L.sem_curve_velocity.argtypes = [vp, i, vp, d, d, d, i, i, vp, vp, vp, vp]
     L.sem_curve_project.argtypes = [vp, i, vp, vp, d, d, d, vp, vp]
     L.sem_plan_layout.argtypes = [vp, ip, C.POINTER(C.c_long)]

@@ -1,4 +1,5 @@
-"""Self-written autotuner for the step kernel's tile shape.
+"""Self-written autotuner: the step kernels' tile shapes and the batched
+path's schedule.
 
 The step kernel has instances for tile heights of 15, 7 and 3 element rows
 (256-, 128- and 64-thread workgroups; every tile is 16 owned elements wide so
@@ -8,8 +9,14 @@ need to fill 256 CUs.  ``best_tile_rows`` times the forward step of every
 instance on the actual mesh and batch size (HIP events around a run of
 launches on the library's stream), keeps the fastest and caches the choice in
 a JSON file keyed by device, mesh, batch and precision.  The tuned height
-applies to one-step launches (checkpointed forward, recompute, slot launches,
-snapshots); two-step launches use a fixed 19 x 26-element tile.
+applies to one-step launches (checkpointed forward, recompute, one-step slot
+launches, snapshots).
+
+``tune_batch`` tunes what runs the batched forward+adjoint: the two-step
+kernel's tile shape (four instances), the full-history group size and lane
+count, and the gradient-correlation chunk, timed on the real mesh at the
+in-flight batch and kept to configurations whose workspace fits memory.
+``best_tile2`` times the two-step tile shapes for the serving lanes.
 """
 from __future__ import annotations
 
@@ -109,4 +116,175 @@ def best_tile_rows(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp3
         _store(db)
     if report is not None:
         report.update(rec)
+    return best
+
+
+# ---------------------------------------------------------------- batched path
+TILE2_SHAPES = {"fp32": (0, 1, 2, 3), "fp64": (0, 1, 2)}
+
+
+def _probe_plan(plan, steps: int, nrec: int):
+    """A plan of the same mesh, sponge, precision and adjoint mode as `plan`,
+    `steps` steps long, full forward history, with `nrec` surface receivers."""
+    from .solver import Plan
+    d = plan.desc
+    sides = tuple(k for k, bit in (("xmin", 1), ("xmax", 2), ("zmin", 4), ("zmax", 8)) if d.sponge_sides & bit)
+    p = Plan(d.nelem_x, d.nelem_z, d.xmin, d.xmax, d.zmin, d.zmax, d.dt, steps, 4, d.sponge_width, d.sponge_cmax,
+             d.sponge_R, sides, precision=plan.precision, adjoint_mode=plan.adjoint_mode, checkpoint_every=0,
+             device=plan.device, tile_rows=plan.tile_rows)
+    n = max(int(nrec), 1)
+    rs = p.add_receivers(np.stack([np.linspace(d.xmin, d.xmax, n), np.full(n, d.zmin)], axis=1))
+    return p, rs
+
+
+def _time_solves(p, rs, n: int, steps: int, repeats: int = 1) -> float:
+    """Seconds per solve-step of a forward+adjoint batch of n solves (the
+    fastest of `repeats` timed runs after one warm-up run)."""
+    dev = p.device
+    c2 = torch.full((n, p.N), 2500.0 ** 2, dtype=p.dtype, device=dev)
+    w = p.wavelet(15.0)
+    src = [p.source_node(0.5 * (p.desc.xmin + p.desc.xmax), p.desc.zmin + 50.0)] * n
+    pitch = p.rec_sizes[rs]
+    obs = torch.zeros((n, steps, pitch), dtype=torch.float64, device=dev)
+    p.forward_adjoint(c2, w, src, obs, 1e-9, [rs] * n)
+    st = torch.cuda.current_stream(dev)
+    best = float("inf")
+    for _ in range(repeats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        p.forward_adjoint(c2, w, src, obs, 1e-9, [rs] * n)
+        e1.record(st)
+        torch.cuda.synchronize(dev)
+        best = min(best, e0.elapsed_time(e1) / 1e3)
+    return best / (n * steps)
+
+
+def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 5, 6, 7, 8), lanes=(2, 3),
+               chunks=(16, 32, 64), max_probe_batch: int = 512, use_cache: bool = True,
+               report: Optional[dict] = None, log=None) -> dict:
+    """Tune how ``plan`` runs forward+adjoint batches of `batch` solves: the
+    two-step kernel's tile shape, the full-history group size and lane count,
+    and the gradient-correlation chunk.  Candidates are those whose workspace
+    fits the plan's memory budget at its real step count; each is timed on a
+    probe plan of the same mesh and precision, `probe_steps` steps long, on
+    the real batch size (at most `max_probe_batch` solves, so that the
+    rounds per lane and the tail of the last round are those of the real
+    run), by coordinate descent (shape, then group x lanes, then chunk).  The result is cached (JSON, keyed by device, mesh,
+    precision, adjoint mode, batch, steps and the largest feasible round) and
+    applied to the plan (set_tile2, set_corr_chunk, plan.tuned for
+    schedule_auto).  Returns the configuration with its timings."""
+    dev = plan.device
+    budget = plan.memory_budget()
+    saved = plan.get_schedule()
+    # the library planner's own choice is always a candidate (with half its
+    # group): small meshes want a few large groups, not groups of 4-8
+    try:
+        plan.tuned = None
+        lib = plan.schedule_auto(batch, pitch, budget)
+    except Exception:
+        lib = {"checkpoint_every": 100, "group": 0, "lanes": 1}
+    cand_g, cand_l = set(groups), set(lanes)
+    if lib["checkpoint_every"] <= 0:
+        g_lib = lib["group"] if lib["group"] > 0 else batch
+        cand_g |= {g_lib, max(1, g_lib // 2)}
+        cand_l.add(lib["lanes"])
+    feasible = []
+    for g in sorted(cand_g):
+        for ln in sorted(cand_l):
+            if (ln - 1) * g >= batch:  # fewer than `ln` groups of at most g solves
+                continue
+            plan.set_schedule(0, g, ln)
+            if plan.workspace_bytes(batch, pitch, True) <= budget:
+                feasible.append((g, ln))
+    plan.set_schedule(saved["checkpoint_every"], saved["group"], saved["lanes"])
+    if not feasible:
+        return {}
+    d = plan.desc
+    key = (f"{torch.cuda.get_device_name(dev).replace(' ', '_')}|batch|{d.nelem_x}x{d.nelem_z}|{plan.precision}|"
+           f"{plan.adjoint_mode}|b{batch}|nt{plan.nt}|r{max(g * ln for g, ln in feasible)}|p{probe_steps}")
+    shapes = TILE2_SHAPES[plan.precision]
+    rec = None
+    if use_cache:
+        rec = _MEM.get(key) or _load().get(key)
+        if rec and not (rec.get("tile2") in shapes and (rec.get("group"), rec.get("lanes")) in
+                        [tuple(f) for f in feasible] and rec.get("chunk") in chunks):
+            rec = None
+    if rec is None:
+        p, rs = _probe_plan(plan, probe_steps, pitch)
+        times = {}
+
+        def run(shape, g, ln, ck):
+            k = f"tile2={shape} group={g} lanes={ln} chunk={ck}"
+            if k not in times:
+                p.set_tile2(shape)
+                p.set_corr_chunk(ck)
+                p.set_schedule(0, g, ln)
+                times[k] = _time_solves(p, rs, min(batch, max_probe_batch), probe_steps)
+                if log:
+                    log(f"[autotune] {k}: {times[k] * 1e6:.3f} us per solve-step")
+            return times[k]
+
+        g_lib = lib["group"] if lib["group"] > 0 else batch
+        start = (g_lib, lib["lanes"]) if (g_lib, lib["lanes"]) in feasible else None
+        g0, l0 = start or max(feasible, key=lambda f: (f[0] * f[1], f[1]))
+        shape = min(shapes, key=lambda s_: run(s_, g0, l0, 64))
+        g0, l0 = min(feasible, key=lambda f: run(shape, f[0], f[1], 64))
+        ck = min(chunks, key=lambda c_: run(shape, g0, l0, c_))
+        rec = {"tile2": shape, "group": g0, "lanes": l0, "chunk": ck, "probe_steps": probe_steps,
+               "us_per_solve_step": {k: round(v * 1e6, 4) for k, v in times.items()},
+               "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+        del p
+        torch.cuda.empty_cache()
+        if use_cache:
+            db = _load()
+            db[key] = rec
+            _store(db)
+    _MEM[key] = rec
+    plan.set_tile2(rec["tile2"])
+    plan.set_corr_chunk(rec["chunk"])
+    plan.tuned = {"group": rec["group"], "lanes": rec["lanes"]}
+    if report is not None:
+        report.update(rec)
+    return rec
+
+
+def best_tile2(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp32", device=None, steps: int = 30,
+               use_cache: bool = True) -> int:
+    """Fastest two-step tile shape for forward steps of `batch` solves on this
+    mesh (timed once, then cached): the serving lanes' choice when they
+    advance two steps per launch."""
+    from .solver import Plan
+    dev = torch.device(device if device is not None else "cuda")
+    key = _key(nelem_x, nelem_z, batch, precision, dev) + "|tile2"
+    shapes = TILE2_SHAPES[precision]
+    hit = _MEM.get(key) or (_load().get(key) if use_cache else None)
+    if hit and hit.get("tile2") in shapes:
+        _MEM[key] = hit
+        return int(hit["tile2"])
+    L = 25.0
+    p = Plan(nelem_x, nelem_z, 0.0, L * nelem_x, 0.0, L * nelem_z, 2e-4, steps, 4, 4 * L, 3000.0, 1e-8,
+             ("xmin", "xmax", "zmax"), precision=precision, checkpoint_every=None, device=dev)
+    rs = p.add_receivers(np.array([[L, 0.0]]))
+    c2 = torch.full((batch, p.N), 2000.0 ** 2, dtype=p.dtype, device=dev)
+    w = torch.zeros(steps, dtype=p.dtype, device=dev)
+    src = [p.source_node(L * nelem_x / 2, L)] * batch
+    st = torch.cuda.current_stream(dev)
+    times = {}
+    for s_ in shapes:
+        p.set_tile2(s_)
+        p.forward(c2, w, src, [rs] * batch)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        p.forward(c2, w, src, [rs] * batch)
+        e1.record(st)
+        torch.cuda.synchronize(dev)
+        times[s_] = e0.elapsed_time(e1) / 1e3 / steps
+    best = min(times, key=times.get)
+    rec = {"tile2": best, "step_us": {str(k): round(v * 1e6, 2) for k, v in times.items()},
+           "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    _MEM[key] = rec
+    if use_cache:
+        db = _load()
+        db[key] = rec
+        _store(db)
     return best

@@ -31,21 +31,32 @@ class BatchSolver:
         self.wavelet = plan.wavelet(f0, amp)
         self.nbz, self.nbx = nbz, nbx
         self.base = None if base is None else torch.as_tensor(base, dtype=torch.float64, device=self.dev)
+        self.tuning = None
 
     @classmethod
-    def from_workload(cls, wl, tile_rows="auto", **plan_kw):
+    def from_workload(cls, wl, tile_rows="auto", tune=None, **plan_kw):
         """Solver for a workload; tile_rows='auto' asks the autotuner for the
-        fastest tile height at this mesh and batch (models x shots).  The plan
-        picks its schedule per call from free device memory
-        (Plan.schedule_auto) unless plan_kw sets schedule='whole'."""
+        fastest one-step tile height at this mesh and batch (models x shots).
+        The plan picks its schedule per call from free device memory
+        (Plan.schedule_auto) unless plan_kw sets schedule='whole'.  tune=True
+        (default: with schedule='auto') also tunes the two-step tile shape,
+        the full-history groups and lanes, and the correlation chunk for this
+        batch (autotune.tune_batch), which schedule_auto then uses."""
         plan_kw.setdefault("schedule", "auto")
+        batch = wl.nmodel * wl.nshot
         if tile_rows == "auto":
             from .autotune import best_tile_rows
-            tile_rows = best_tile_rows(wl.mesh.nelem_x, wl.mesh.nelem_z, wl.nmodel * wl.nshot,
+            tile_rows = best_tile_rows(wl.mesh.nelem_x, wl.mesh.nelem_z, batch,
                                        plan_kw.get("precision", "fp32"), plan_kw.get("device"))
         plan = Plan.from_mesh(wl.mesh, tile_rows=int(tile_rows), **plan_kw)
         nbz, nbx = wl.coef.shape[1:]
-        return cls(plan, wl.sources, wl.receivers, wl.f0, nbz, nbx, wl.base)
+        solver = cls(plan, wl.sources, wl.receivers, wl.f0, nbz, nbx, wl.base)
+        if tune is None:
+            tune = plan_kw["schedule"] == "auto"
+        if tune:
+            from .autotune import tune_batch
+            solver.tuning = tune_batch(plan, batch, solver.nrec)
+        return solver
 
     def velocity(self, coef):
         coef = torch.as_tensor(coef, dtype=torch.float64, device=self.dev)

@@ -800,20 +800,27 @@ __device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View
     return false;
 }
 
-constexpr int EX2 = 19;                 // element threads along x (16 owned: 256-byte rows)
-constexpr int EZ2 = 26;                 // element threads along z (23 owned)
-constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
-constexpr int T2_THREADS = (EX2 * EZ2 + 63) / 64 * 64;
-constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;  // level-(n+1) nodes of the ring + O
-constexpr int U1_H = 4 * (EZ2 - 1) + 1;
-constexpr int W1_W = (4 * TX2 + 1 + 3) / 4 * 4;        // increment of O's nodes
-constexpr int W1_H = 4 * TZ2 + 1;
+// Two-step tile shapes (sem_plan_set_tile2): EX x EZ element threads per
+// workgroup, the owned block (EX - 3) x (EZ - 3) with two halo columns / rows
+// on the left / below and one on the right / above.  Owned widths of 16 and
+// 32 elements keep owned FP32 rows 256- or 512-byte aligned.  Shape 0 is the
+// default; shape 3 needs 150 KB of LDS in FP32 (one workgroup per CU) and has
+// no FP64 instance (it would exceed the CU's 160 KB).
+constexpr int NTILE2 = 4;
+inline constexpr int TILE2_EX[NTILE2] = {19, 19, 35, 35};
+inline constexpr int TILE2_EZ[NTILE2] = {26, 13, 14, 28};
+__host__ __device__ constexpr int tile2_threads(int ex, int ez) { return (ex * ez + 63) / 64 * 64; }
 
-template <typename T, int P, int MODE, bool NTS>
-__global__ __launch_bounds__(T2_THREADS) void step2_kernel(const StepArgs<T, P> a) {
+template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
+__global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const StepArgs<T, P> a) {
 #pragma clang fp contract(off)
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     static_assert(MODE == MODE_FWD || MODE == MODE_ADJS, "forward or adjoint steps");
+    constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
+    constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;  // level-(n+1) nodes of the ring + O
+    constexpr int U1_H = 4 * (EZ2 - 1) + 1;
+    constexpr int W1_W = (4 * TX2 + 1 + 3) / 4 * 4;        // increment of O's nodes
+    constexpr int W1_H = 4 * TZ2 + 1;
     constexpr int NP = P + 1;
     constexpr bool ADJ = (MODE == MODE_ADJS);
     __shared__ T xr_col[EZ2][EX2][NP];

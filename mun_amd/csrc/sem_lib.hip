@@ -253,11 +253,14 @@ struct sem_plan {
     DevBuf d_recs, d_rec_ptr;
     bool rec_dirty = true;
     size_t rec_up_entries = 0, rec_up_ptrs = 0;  // prefix already on the device
-    // two-step tiling (step2_kernel) and its receiver table
+    // two-step tiling (step2_kernel, tile shape `tile2` of TILE2_EX/TILE2_EZ)
+    // and its receiver table
+    int tile2 = 0, tx2 = TILE2_EX[0] - 3, tz2 = TILE2_EZ[0] - 3;
     int ntx2 = 0, ntz2 = 0, ntiles2 = 0;
     DevBuf d_recs2, d_rec_ptr2;
     size_t rec2_up_entries = 0, rec2_up_ptrs = 0;
     int fusion = 2;  // time steps per step-kernel launch where the batch path allows two (sem_plan_set_step_fusion)
+    int corr_max = 64;  // adjoint steps per gradient-correlation chunk, at most (sem_plan_set_corr_chunk)
     std::map<std::pair<int, int>, BsplineTables*> bsp;
     // schedule of sem_forward_adjoint (sem_plan_set_schedule): groups of at
     // most sched_group solves (0: the whole batch), sched_lanes in flight
@@ -364,7 +367,7 @@ int upload_receivers(sem_plan* pl) {
     // two-step tiles: two halo columns / rows on the left and below, one on the right and above
     std::vector<RecEntry<T, 4>> flat2;
     std::vector<int> ptr2;
-    build_rec_table<T>(pl, TX2, TZ2, pl->ntx2, pl->ntz2, 2, 1, flat2, ptr2);
+    build_rec_table<T>(pl, pl->tx2, pl->tz2, pl->ntx2, pl->ntz2, 2, 1, flat2, ptr2);
     if ((rc = upload_rec_table<T>(pl->d_recs2, pl->d_rec_ptr2, pl->rec2_up_entries, pl->rec2_up_ptrs, flat2, ptr2)))
         return rc;
     pl->rec_dirty = false;
@@ -406,13 +409,13 @@ struct Layout {
     size_t U0, U1, W, W2, CKU, CKW, SEG, RING, OM, RES, G, total;
 };
 
-// Correlation chunk: at most 64 adjoint steps, dividing a segment of S steps
-// into equal chunks (S = 100 -> 2 x 50).  The adjoint states of one chunk are
-// kept in a ring of C + 1 slots; the FP64 accumulator is read and written once
-// per chunk (64 rather than 32 steps: 1 % faster on the lane schedule for a
-// ring of 65 states per solve).
-int corr_chunk(int S) {
-    const int nchunk = (S + 63) / 64;
+// Correlation chunk: at most `cmax` adjoint steps (the plan's, default 64),
+// dividing a segment of S steps into equal chunks (S = 100 -> 2 x 50).  The
+// adjoint states of one chunk are kept in a ring of C + 1 slots; the FP64
+// accumulator is read and written once per chunk (64 rather than 32 steps: 1 %
+// faster on the lane schedule for a ring of 65 states per solve).
+int corr_chunk(int S, int cmax) {
+    const int nchunk = (S + cmax - 1) / cmax;
     return (S + nchunk - 1) / nchunk;
 }
 
@@ -447,7 +450,7 @@ Layout layout(const sem_plan* pl, int B, int pitch, bool adj, int ckpt) {
         } else {
             L.SEG = take(state * (size_t)(nt + 1));
         }
-        L.C = corr_chunk(L.S);
+        L.C = corr_chunk(L.S, pl->corr_max);
         L.RING = take(state * (size_t)(L.C + 1));
         L.OM = take(state);
         L.RES = take((size_t)B * nt * pitch * Ts);
@@ -603,17 +606,39 @@ int launch_step(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, boo
     return nts ? launch_step_nt<T, MODE, true>(pl, a, st) : launch_step_nt<T, MODE, false>(pl, a, st);
 }
 
-// Two steps (n, n+1) per launch: u_in -> u_mid -> u_out, increment a.w -> a.w_out.
-template <typename T, int MODE>
-int launch_step2(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
-    (void)pl;
-    if (!a.rec_ptr2 || (!a.slots && (!a.w_out || a.w_out == a.w || a.u_out == a.u_in || a.u_mid == a.u_in)))
-        return fail(SEM_E_INVALID, "two-step launch with aliased or missing buffers");
+// Two steps (n, n+1) per launch: u_in -> u_mid -> u_out, increment a.w -> a.w_out,
+// on the plan's two-step tile shape.
+template <typename T, int MODE, bool NTS, int S>
+void launch_step2_shape(const StepArgs<T, 4>& a, unsigned grid, hipStream_t st) {
+    constexpr int ex = TILE2_EX[S], ez = TILE2_EZ[S];
+    step2_kernel<T, 4, MODE, NTS, ex, ez><<<grid, tile2_threads(ex, ez), 0, st>>>(a);
+}
+
+template <typename T, int MODE, bool NTS>
+int launch_step2_nt(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st) {
     const unsigned grid = (unsigned)a.nper2 * NXCD;
-    if (nts) step2_kernel<T, 4, MODE, true><<<grid, T2_THREADS, 0, st>>>(a);
-    else step2_kernel<T, 4, MODE, false><<<grid, T2_THREADS, 0, st>>>(a);
+    switch (pl->tile2) {
+        case 0: launch_step2_shape<T, MODE, NTS, 0>(a, grid, st); break;
+        case 1: launch_step2_shape<T, MODE, NTS, 1>(a, grid, st); break;
+        case 2: launch_step2_shape<T, MODE, NTS, 2>(a, grid, st); break;
+        case 3:
+            if constexpr (sizeof(T) == 4) {
+                launch_step2_shape<T, MODE, NTS, 3>(a, grid, st);
+                break;
+            } else {
+                return fail(SEM_E_UNSUPPORTED, "two-step tile shape 3 has no FP64 instance");
+            }
+        default: return fail(SEM_E_UNSUPPORTED, "two-step tile shape without a kernel instance");
+    }
     LAUNCH_CHECK();
     return SEM_OK;
+}
+
+template <typename T, int MODE>
+int launch_step2(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st, bool nts) {
+    if (!a.rec_ptr2 || (!a.slots && (!a.w_out || a.w_out == a.w || a.u_out == a.u_in || a.u_mid == a.u_in)))
+        return fail(SEM_E_INVALID, "two-step launch with aliased or missing buffers");
+    return nts ? launch_step2_nt<T, MODE, true>(pl, a, st) : launch_step2_nt<T, MODE, false>(pl, a, st);
 }
 
 template <typename T>
@@ -1178,8 +1203,8 @@ int sem_plan_create(const sem_desc* d, sem_plan** out) try {
     pl->ntx = (d->nelem_x + TX - 1) / TX;
     pl->ntz = (d->nelem_z + pl->tz - 1) / pl->tz;
     pl->ntiles = pl->ntx * pl->ntz;
-    pl->ntx2 = (d->nelem_x + TX2 - 1) / TX2;
-    pl->ntz2 = (d->nelem_z + TZ2 - 1) / TZ2;
+    pl->ntx2 = (d->nelem_x + pl->tx2 - 1) / pl->tx2;
+    pl->ntz2 = (d->nelem_z + pl->tz2 - 1) / pl->tz2;
     pl->ntiles2 = pl->ntx2 * pl->ntz2;
     pl->ctiles = (int)(((long)d->nelem_x * d->nelem_z + CORR_W - 1) / CORR_W);
     pl->nexp = (d->nelem_x + 31) / 32 * 32;
@@ -1331,6 +1356,43 @@ int sem_plan_set_streaming(sem_plan* pl, int mode) try {
 int sem_plan_set_step_fusion(sem_plan* pl, int steps) try {
     if (!pl || steps < 1 || steps > 2) return fail(SEM_E_INVALID, "sem_plan_set_step_fusion: steps must be 1 or 2");
     pl->fusion = steps;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_set_corr_chunk(sem_plan* pl, int steps) try {
+    if (!pl || steps < 1 || steps > 256) return fail(SEM_E_INVALID, "sem_plan_set_corr_chunk: steps must be in [1, 256]");
+    pl->corr_max = steps;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_get_corr_chunk(const sem_plan* pl, int* steps) try {
+    if (!pl || !steps) return fail(SEM_E_INVALID, "sem_plan_get_corr_chunk: bad argument");
+    *steps = pl->corr_max;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_set_tile2(sem_plan* pl, int shape) try {
+    if (!pl || shape < 0 || shape >= NTILE2) return fail(SEM_E_INVALID, "sem_plan_set_tile2: bad argument");
+    if (shape == 3 && pl->d.precision == SEM_FP64)
+        return fail(SEM_E_UNSUPPORTED, "two-step tile shape 3 has no FP64 instance");
+    if (shape == pl->tile2) return SEM_OK;
+    pl->tile2 = shape;
+    pl->tx2 = TILE2_EX[shape] - 3;
+    pl->tz2 = TILE2_EZ[shape] - 3;
+    pl->ntx2 = (pl->d.nelem_x + pl->tx2 - 1) / pl->tx2;
+    pl->ntz2 = (pl->d.nelem_z + pl->tz2 - 1) / pl->tz2;
+    pl->ntiles2 = pl->ntx2 * pl->ntz2;
+    // the two-step receiver table is rebuilt for the new tiles on the next use
+    pl->rec2_up_entries = pl->rec2_up_ptrs = 0;
+    pl->rec_dirty = true;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_plan_get_tile2(const sem_plan* pl, int* shape, int* ex, int* ez) try {
+    if (!pl) return fail(SEM_E_INVALID, "sem_plan_get_tile2: null plan");
+    if (shape) *shape = pl->tile2;
+    if (ex) *ex = TILE2_EX[pl->tile2];
+    if (ez) *ez = TILE2_EZ[pl->tile2];
     return SEM_OK;
 } ABI_CATCH
 

@@ -155,6 +155,16 @@ class Lane:
         self.plan = Plan(nex, nez, 0.0, xmax, 0.0, zmax, dt, nt_max, 4, sw, scm, 1e-8, server.sides,
                          precision="fp32", adjoint_mode=server.adjoint_mode, record="post",
                          checkpoint_every=self.S, device=server.device, tile_rows=tile_rows)
+        # two-step tile shape of the lane (before the receiver tables are
+        # reserved): the server's, else the autotuner's when rounds take two
+        # steps per launch, else the default
+        self.tile2 = server.tile2
+        if self.tile2 is None:
+            self.tile2 = 0
+            if server.two_step and server.autotune:
+                from .autotune import best_tile2
+                self.tile2 = best_tile2(nex, nez, capacity, "fp32", server.device)
+        self.plan.set_tile2(self.tile2)
         pitch, elems = C.c_int(), C.c_long()
         L.check(self.plan.lib.sem_plan_layout(self.plan.h, C.byref(pitch), C.byref(elems)))
         self.E = elems.value
@@ -298,7 +308,8 @@ class SEMServer:
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True, history: str = "auto", two_step: bool = False):
+                 autotune: bool = True, history: str = "auto", two_step: bool = False,
+                 tile2: Optional[int] = None):
         """history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  two_step: advance
@@ -306,7 +317,9 @@ class SEMServer:
         (sem_slots_step2); opt-in: warm servers run within noise of one step
         per launch, and one saturated two-step run on a server created after
         another was torn down once ran 4x slower for a reason not found.
-        Results are the same bits either way."""
+        Results are the same bits either way.  tile2: the two-step kernel's
+        tile shape in every lane (default: tuned per lane when two_step and
+        autotune, else shape 0); results do not depend on it."""
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
         self.history = history
@@ -324,6 +337,7 @@ class SEMServer:
         self.nt_max = int(nt_max)
         self.max_sets = int(max_sets)
         self.autotune = bool(autotune)
+        self.tile2 = None if tile2 is None else int(tile2)
         self.lanes: Dict[tuple, Lane] = {}
         self.in_use = 0
         self.results: Dict[int, SolveResult] = {}

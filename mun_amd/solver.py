@@ -95,6 +95,7 @@ class Plan:
                                        self.z1d.ctypes.data_as(C.POINTER(C.c_double))))
         self.rec_sizes = []
         self._ws = None
+        self.tuned = None  # tuned batch configuration (autotune.tune_batch), used by schedule_auto
 
     @classmethod
     def from_mesh(cls, m, **kw):
@@ -251,6 +252,28 @@ class Plan:
         L.check(self.lib.sem_plan_get_step_fusion(self.h, C.byref(v)))
         return v.value
 
+    def set_tile2(self, shape: int):
+        """Tile shape of the two-step kernel (0: 19 x 26 element threads,
+        1: 19 x 13, 2: 35 x 14, 3: 35 x 28, FP32 only).  Results do not depend
+        on it; set it before receivers are reserved for captured rounds."""
+        L.check(self.lib.sem_plan_set_tile2(self.h, int(shape)))
+
+    def get_tile2(self):
+        """(shape, element threads along x, along z) of the two-step tile."""
+        sh, ex, ez = C.c_int(), C.c_int(), C.c_int()
+        L.check(self.lib.sem_plan_get_tile2(self.h, C.byref(sh), C.byref(ex), C.byref(ez)))
+        return sh.value, ex.value, ez.value
+
+    def set_corr_chunk(self, steps: int):
+        """Adjoint steps per gradient-correlation chunk, at most (default 64).
+        Results do not depend on it; the workspace does."""
+        L.check(self.lib.sem_plan_set_corr_chunk(self.h, int(steps)))
+
+    def get_corr_chunk(self) -> int:
+        v = C.c_int()
+        L.check(self.lib.sem_plan_get_corr_chunk(self.h, C.byref(v)))
+        return v.value
+
     def get_schedule(self):
         ck, g, ln = C.c_int(), C.c_int(), C.c_int()
         L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))
@@ -265,10 +288,18 @@ class Plan:
         return max(0, free + held + cached - max(reserve_min, int(reserve_frac * total)))
 
     def schedule_auto(self, batch: int, pitch: int, budget: Optional[int] = None):
-        """Pick the fastest schedule whose workspace fits `budget` bytes
-        (default: memory_budget()): the full forward history when it fits,
-        else lanes of full-history groups, else checkpoints."""
+        """Pick a schedule whose workspace fits `budget` bytes (default:
+        memory_budget()).  With a tuned configuration (``self.tuned``, set by
+        ``autotune.tune_batch``): its full-history groups and lanes when the
+        batch is at least one round of them and their workspace fits, else
+        the library's planner: the full forward history when it fits, else
+        lanes of full-history groups, else checkpoints."""
         budget = self.memory_budget() if budget is None else int(budget)
+        t = getattr(self, "tuned", None)
+        if t and batch >= t["group"] * t["lanes"]:
+            self.set_schedule(0, t["group"], t["lanes"])
+            if self.workspace_bytes(batch, pitch, True) <= budget:
+                return self.get_schedule()
         L.check(self.lib.sem_plan_schedule_auto(self.h, int(batch), int(pitch), C.c_size_t(budget)))
         return self.get_schedule()
 

@@ -130,13 +130,41 @@ def test_two_step_rounds_give_the_same_bits(history):
               history=history)
     print(f"[serve-two-step] {history}: one step per launch", flush=True)
     one = SEMServer(hbm_budget_bytes=8e9, two_step=False, **kw).run(reqs)
-    print(f"[serve-two-step] {history}: two steps per launch", flush=True)
-    srv = SEMServer(hbm_budget_bytes=8e9, two_step=True, **kw)
-    assert srv.two_step
-    two = srv.run(reqs)
     assert any(r.nt % 2 == 1 for r in reqs)  # odd segment tails occur
+    for tile2 in (None, 1, 2):  # the lanes' tuned tile shape, then two fixed ones
+        print(f"[serve-two-step] {history}: two steps per launch, tile2={tile2}", flush=True)
+        srv = SEMServer(hbm_budget_bytes=8e9, two_step=True, tile2=tile2, **kw)
+        assert srv.two_step
+        two = srv.run(reqs)
+        if tile2 is not None:
+            assert all(lane.plan.get_tile2()[0] == tile2 for lane in srv.lanes.values())
+        for r in reqs:
+            a, b = one[r.rid], two[r.rid]
+            assert a.loglik == b.loglik
+            np.testing.assert_array_equal(a.grad, b.grad)
+            np.testing.assert_array_equal(a.seis, b.seis)
+
+
+def test_served_stream_at_its_workload_sizes_alone_equals_mix():
+    """The served workload itself (meshes 20^2 .. 160^2 deg-4, 1000-5000 steps,
+    31-186 receivers, Poisson arrivals at 20/s), a reduced count of 8
+    requests covering all four mesh sizes: every result in the mix has the
+    bits of the same request served alone."""
+    from mun_amd import workloads as W
+    from mun_amd.serving import SEMServer
+    print("[serve workload] requests", flush=True)
+    reqs = W.served_solve_requests(8, 20.0, seed=7, nt_scale=1.0)
+    assert {r.nelem_x for r in reqs} == {20, 40, 80, 160} and max(r.nt for r in reqs) == 5000
+    srv = SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000)
+    print("[serve workload] mix", flush=True)
+    mix = srv.run(reqs)
+    assert len(mix) == len(reqs)
+    adm = sorted((r.admitted, r.finished) for r in mix.values())
+    assert any(adm[i + 1][0] < adm[i][1] for i in range(len(adm) - 1))
     for r in reqs:
-        a, b = one[r.rid], two[r.rid]
+        print(f"[serve workload] alone: request {r.rid} ({r.nelem_x}^2, {r.nt} steps)", flush=True)
+        a = srv.run([r])[r.rid]
+        b = mix[r.rid]
         assert a.loglik == b.loglik
         np.testing.assert_array_equal(a.grad, b.grad)
         np.testing.assert_array_equal(a.seis, b.seis)

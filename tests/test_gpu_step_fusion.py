@@ -191,3 +191,55 @@ def test_two_steps_at_the_benchmark_mesh_give_the_same_bits():
             assert torch.equal(outs[0][k], outs[1][k]), (sch, k)
         assert torch.isfinite(outs[1][1]).all() and outs[1][1].abs().max() > 0
     s.plan.set_step_fusion(2)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_every_two_step_tile_shape_gives_the_same_bits(fusion_case, precision):
+    """The two-step kernel's tile shapes (sem_plan_set_tile2; 0..3, shape 3
+    FP32 only) give the bits of one step per launch, under the full-history
+    lanes and the checkpointed schedule."""
+    prob, disc, coef, c, seis = fusion_case
+    print(f"[tile2] {precision}", flush=True)
+    plan = make_plan(prob, precision, adjoint_mode="exact", record="post", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
+    shapes = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
+    if precision == "fp64":
+        with pytest.raises(Exception, match="FP64"):
+            plan.set_tile2(3)
+    for sch in [(0, 2, 3), (40, 0, 1)]:
+        plan.set_schedule(*sch)
+        plan.set_step_fusion(1)
+        o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+        ref = {k: o[k].cpu() for k in ("seis", "misfit", "grad_c")}
+        plan.set_step_fusion(2)
+        for sh in shapes:
+            plan.set_tile2(sh)
+            assert plan.get_tile2()[0] == sh
+            o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+            torch.cuda.synchronize()
+            for k in ref:
+                assert torch.equal(o[k].cpu(), ref[k]), (sch, sh, k)
+        plan.set_tile2(0)
+
+
+def test_every_correlation_chunk_gives_the_same_bits(fusion_case):
+    prob, disc, coef, c, seis = fusion_case
+    print("[corr chunk]", flush=True)
+    plan = make_plan(prob, "fp32", adjoint_mode="reference", record="post", checkpoint_every=40)
+    C2, w, srcs, model_of, rs = _inputs(prob, disc, coef, c, plan)
+    B = len(srcs)
+    obs = torch.as_tensor(seis * 0.9, dtype=torch.float64, device=plan.device)[None].expand(B, -1, -1)
+    outs = []
+    for ck in (64, 1, 2, 7, 16, 128):
+        plan.set_corr_chunk(ck)
+        assert plan.get_corr_chunk() == ck
+        for sch in [(0, 2, 3), (40, 0, 1)]:
+            plan.set_schedule(*sch)
+            o = plan.forward_adjoint(C2, w, srcs, obs, 1e-9, [rs] * B, model_of=model_of)
+            torch.cuda.synchronize()
+            outs.append({k: o[k].cpu() for k in ("seis", "misfit", "grad_c")})
+    for o in outs[1:]:
+        for k in o:
+            assert torch.equal(o[k], outs[0][k]), k

@@ -15,23 +15,30 @@ P = 4
 HDR = os.path.join(os.path.dirname(__file__), "..", "mun_amd", "csrc", "sem_kernels.h")
 
 
-def _consts():
+def _shapes():
     text = open(HDR).read()
-    ex2 = int(re.search(r"constexpr int EX2 = (\d+);", text).group(1))
-    ez2 = int(re.search(r"constexpr int EZ2 = (\d+);", text).group(1))
-    return ex2, ez2
+    ex = [int(v) for v in re.search(r"TILE2_EX\[NTILE2\] = \{([^}]*)\}", text).group(1).split(",")]
+    ez = [int(v) for v in re.search(r"TILE2_EZ\[NTILE2\] = \{([^}]*)\}", text).group(1).split(",")]
+    return list(zip(ex, ez))
 
 
-EX2, EZ2 = _consts()
-TX2, TZ2 = EX2 - 3, EZ2 - 3
-T2 = (EX2 * EZ2 + 63) // 64 * 64
-U1_W = (4 * (EX2 - 1) + 1 + 3) // 4 * 4
-U1_H = 4 * (EZ2 - 1) + 1
-W1_W = (4 * TX2 + 1 + 3) // 4 * 4
-W1_H = 4 * TZ2 + 1
+SHAPES = _shapes()
 
 
-def _emulate(nex, nez):
+class Tile:
+    def __init__(self, ex2, ez2):
+        self.EX2, self.EZ2 = ex2, ez2
+        self.TX2, self.TZ2 = ex2 - 3, ez2 - 3
+        self.T2 = (ex2 * ez2 + 63) // 64 * 64
+        self.U1_W = (4 * (ex2 - 1) + 1 + 3) // 4 * 4
+        self.U1_H = 4 * (ez2 - 1) + 1
+        self.W1_W = (4 * self.TX2 + 1 + 3) // 4 * 4
+        self.W1_H = 4 * self.TZ2 + 1
+
+
+def _emulate(nex, nez, tile):
+    EX2, EZ2, TX2, TZ2, T2 = tile.EX2, tile.EZ2, tile.TX2, tile.TZ2, tile.T2
+    U1_W, U1_H, W1_W, W1_H = tile.U1_W, tile.U1_H, tile.W1_W, tile.W1_H
     NX, NZ = nex * P + 1, nez * P + 1
     pitch = (NX + 31) // 32 * 32
     sb = (NZ * pitch + 63) // 64 * 64
@@ -80,14 +87,20 @@ def _emulate(nex, nez):
     return writes
 
 
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
 @pytest.mark.parametrize("nex,nez", [(1, 1), (2, 3), (7, 5), (16, 23), (17, 24), (33, 47), (37, 50), (40, 40),
                                      (20, 80), (160, 20)])
-def test_two_step_tiles_partition_the_nodes(nex, nez):
-    writes = _emulate(nex, nez)
+def test_two_step_tiles_partition_the_nodes(nex, nez, shape):
+    writes = _emulate(nex, nez, Tile(*SHAPES[shape]))
     assert (writes == 1).all(), np.argwhere(writes != 1)[:5]
 
 
-def test_two_step_tile_fits_two_workgroups_per_cu():
-    lds_fp32 = 4 * (EZ2 * EX2 * (P + 1 + P) + U1_H * U1_W + W1_H * W1_W)
-    assert T2 <= 1024 and lds_fp32 <= 80 * 1024  # two workgroups in a CU's 160 KB
-    assert TX2 * P * 4 % 256 == 0  # owned rows start on 256-byte boundaries (FP32)
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_two_step_tile_fits_the_cu(shape):
+    t = Tile(*SHAPES[shape])
+    lds_fp32 = 4 * (t.EZ2 * t.EX2 * (P + 1 + P) + t.U1_H * t.U1_W + t.W1_H * t.W1_W)
+    assert t.T2 <= 1024 and lds_fp32 <= 160 * 1024      # one workgroup in a CU's 160 KB
+    if shape != 3:
+        assert 2 * lds_fp32 <= 160 * 1024                 # FP64 (twice the bytes) fits too
+    assert t.TX2 * P * 4 % 256 == 0                       # owned rows start on 256-byte boundaries (FP32)
+    assert shape != 0 or 2 * lds_fp32 <= 160 * 1024       # the default shape: two workgroups per CU

@@ -314,7 +314,8 @@ def main():
 
     bytes_launch = STEP_BYTES_PER_NODE * fusion * m.N * inflight
     achieved = bytes_launch / t_launch / 1e9
-    kernel = (f"step2_kernel<float,4,MODE_FWD,NTS={str(nts).lower()}>" if fusion == 2
+    _, t2x, t2z = plan.get_tile2()
+    kernel = (f"step2_kernel<float,4,MODE_FWD,NTS={str(nts).lower()},{t2x},{t2z}>" if fusion == 2
               else f"step_kernel<float,4,MODE_FWD,{plan.tile_rows},NTS={str(nts).lower()}>")
     log(f"[bench] step kernel on {inflight} solves, {fusion} step(s) per launch: {t_launch * 1e6:.1f} us/launch "
         f"({t_launch / fusion * 1e6:.1f} us/step), {achieved:.0f} GB/s at 16 B per node-step; one step per launch: "
@@ -347,7 +348,9 @@ def main():
            "config": {"workload": f"batch200: {args.batch} models x (200x200 deg-4 elements, 801x801 nodes, "
                                   f"{m.nt} steps) forward+adjoint+gradient; {sched_txt}",
                       "batch": args.batch, "nt": m.nt, "nodes": m.N, "schedule": sched,
-                      "parallelism": "none (1 GPU)"},
+                      "tuned": ({k: solver.tuning[k] for k in ("tile2", "group", "lanes", "chunk")}
+                                if solver.tuning else None),
+                      "tile2": list(plan.get_tile2()), "parallelism": "none (1 GPU)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "traffic_basis": traffic_basis, "kernel": kernel,

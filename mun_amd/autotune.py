@@ -169,10 +169,12 @@ def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 
     probe plan of the same mesh and precision, `probe_steps` steps long, on
     the real batch size (at most `max_probe_batch` solves, so that the
     rounds per lane and the tail of the last round are those of the real
-    run), by coordinate descent (shape, then group x lanes, then chunk).  The result is cached (JSON, keyed by device, mesh,
-    precision, adjoint mode, batch, steps and the largest feasible round) and
-    applied to the plan (set_tile2, set_corr_chunk, plan.tuned for
-    schedule_auto).  Returns the configuration with its timings."""
+    run), by coordinate descent (shape, then group x lanes, then chunk).
+    The result is cached (JSON, keyed by device, mesh, precision, adjoint
+    mode, batch and steps) and applied to the plan (set_tile2,
+    set_corr_chunk, plan.tuned for schedule_auto); a cached choice is used
+    only if it still fits the current budget.  Returns the configuration
+    with its timings."""
     dev = plan.device
     budget = plan.memory_budget()
     saved = plan.get_schedule()
@@ -201,7 +203,7 @@ def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 
         return {}
     d = plan.desc
     key = (f"{torch.cuda.get_device_name(dev).replace(' ', '_')}|batch|{d.nelem_x}x{d.nelem_z}|{plan.precision}|"
-           f"{plan.adjoint_mode}|b{batch}|nt{plan.nt}|r{max(g * ln for g, ln in feasible)}|p{probe_steps}")
+           f"{plan.adjoint_mode}|b{batch}|nt{plan.nt}|p{probe_steps}")
     shapes = TILE2_SHAPES[plan.precision]
     rec = None
     if use_cache:

@@ -14,6 +14,7 @@ on the 200 x 200 element, degree-4 batch workload.
 from __future__ import annotations
 
 import argparse
+import re
 import sys
 
 import torch
@@ -34,7 +35,8 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles
     full-history forward of the lane groups, the adjoint).  Solves per launch
     follow from each dispatch's grid: tiles_per_solve workgroups of `threads`
     (195 tiles of the 15-row step kernel on the 801 x 801-node mesh), or
-    tiles_per_solve2 of threads2 for the two-step kernel (117 tiles).
+    tiles_per_solve2 of threads2 for the two-step kernel (117 tiles of the
+    default 19 x 26 shape; other shapes are read from the instance's name).
     `forward` / `adjoint` report the launches of `batch` solves (else the
     largest), preferring the two-step kernel; `kernels` all."""
     import csv
@@ -65,7 +67,14 @@ def summarize_pmc(fetch_csv, write_csv, batch, nodes=641601, copy_mb=1024, tiles
             if not (two or name.startswith("sem::step_kernel")) or ", 2, " in name:
                 continue
             mode = "forward" if ", 0, " in name else "adjoint"
-            solves = max(1, round(grid / (threads2 * tiles_per_solve2 if two else threads * tiles_per_solve)))
+            per2 = threads2 * tiles_per_solve2
+            shape = re.search(r"step2_kernel<[^>]*, (\d+), (\d+)>", name) if two else None
+            if shape:
+                # the tile shape is in the instance's template arguments
+                ex, ez = int(shape.group(1)), int(shape.group(2))
+                nel = int(round((nodes ** 0.5 - 1) / 4))
+                per2 = (ex * ez + 63) // 64 * 64 * (-(-nel // (ex - 3))) * (-(-nel // (ez - 3)))
+            solves = max(1, round(grid / (per2 if two else threads * tiles_per_solve)))
             steps = 2 if two else 1
             wr = 12 if two and (mode == "adjoint" or solves != batch) else 8
             rec = recs.setdefault((mode, steps, solves), {"mode": mode, "kernel": name, "steps_per_launch": steps,

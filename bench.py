@@ -234,8 +234,13 @@ def main():
     wl = W.batch200(nmodel=args.batch, nt=args.nt)
     m = wl.mesh
     log(f"[bench] workload {wl.name}: batch {args.batch}, mesh {m.nelem_x}x{m.nelem_z} (N={m.N}), nt={m.nt}")
+    t_tune = time.perf_counter()
     solver = BatchSolver.from_workload(wl, precision="fp32", adjoint_mode="reference", device=dev)
     plan = solver.plan
+    if solver.tuning:
+        log(f"[bench] tuning ({'cached' if solver.tuning.get('cached') else 'timed'}, "
+            f"{time.perf_counter() - t_tune:.1f} s): " +
+            ", ".join(f"{k} {solver.tuning[k]}" for k in ("tile2", "group", "lanes", "chunk")))
     coef = torch.as_tensor(wl.coef, device=dev)
     # observations from the seeded true model (one forward solve)
     obs = solver.forward(torch.as_tensor(wl.true_coef[None], device=dev))[0]

@@ -33,8 +33,20 @@ _MEM: Dict[str, dict] = {}
 
 
 def cache_path() -> str:
-    return os.environ.get("MUN_AMD_AUTOTUNE_CACHE",
-                          os.path.join(os.path.expanduser("~"), ".cache", "mun_amd", "autotune.json"))
+    """MUN_AMD_AUTOTUNE_CACHE, else ~/.cache/mun_amd/autotune.json when the
+    home directory is writable, else a file in the temporary directory."""
+    env = os.environ.get("MUN_AMD_AUTOTUNE_CACHE")
+    if env:
+        return env
+    home = os.path.join(os.path.expanduser("~"), ".cache", "mun_amd")
+    try:
+        os.makedirs(home, exist_ok=True)
+        if os.access(home, os.W_OK):
+            return os.path.join(home, "autotune.json")
+    except OSError:
+        pass
+    import tempfile
+    return os.path.join(tempfile.gettempdir(), f"mun_amd_autotune_{os.getuid()}.json")
 
 
 def _load() -> dict:
@@ -58,9 +70,15 @@ def _store(db: dict):
         pass
 
 
+def device_tag(device) -> str:
+    """Cache key of a device: its compute-unit count and memory size (the
+    marketing name reads empty under some profilers)."""
+    props = torch.cuda.get_device_properties(device)
+    return f"cu{props.multi_processor_count}_mem{props.total_memory >> 30}G"
+
+
 def _key(nelem_x, nelem_z, batch, precision, device) -> str:
-    name = torch.cuda.get_device_name(device).replace(" ", "_")
-    return f"{name}|{nelem_x}x{nelem_z}|b{batch}|{precision}"
+    return f"{device_tag(device)}|{nelem_x}x{nelem_z}|b{batch}|{precision}"
 
 
 def measure(nelem_x: int, nelem_z: int, batch: int, tile_rows: int, precision: str = "fp32", steps: int = 30,
@@ -202,7 +220,7 @@ def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 
     if not feasible:
         return {}
     d = plan.desc
-    key = (f"{torch.cuda.get_device_name(dev).replace(' ', '_')}|batch|{d.nelem_x}x{d.nelem_z}|{plan.precision}|"
+    key = (f"{device_tag(dev)}|batch|{d.nelem_x}x{d.nelem_z}|{plan.precision}|"
            f"{plan.adjoint_mode}|b{batch}|nt{plan.nt}|p{probe_steps}")
     shapes = TILE2_SHAPES[plan.precision]
     rec = None
@@ -211,6 +229,7 @@ def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 
         if rec and not (rec.get("tile2") in shapes and (rec.get("group"), rec.get("lanes")) in
                         [tuple(f) for f in feasible] and rec.get("chunk") in chunks):
             rec = None
+    cached = rec is not None
     if rec is None:
         p, rs = _probe_plan(plan, probe_steps, pitch)
         times = {}
@@ -247,7 +266,7 @@ def tune_batch(plan, batch: int, pitch: int, probe_steps: int = 192, groups=(4, 
     plan.tuned = {"group": rec["group"], "lanes": rec["lanes"]}
     if report is not None:
         report.update(rec)
-    return rec
+    return dict(rec, cached=cached)
 
 
 def best_tile2(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp32", device=None, steps: int = 30,
@@ -290,3 +309,25 @@ def best_tile2(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp32", 
         db[key] = rec
         _store(db)
     return best
+
+
+def main(argv=None):
+    """Tune and cache the batched configuration of a workload ahead of use:
+    python -m mun_amd.autotune batch200|elbo120|cpu40 (the cache file is
+    cache_path(), or MUN_AMD_AUTOTUNE_CACHE)."""
+    import argparse
+    from . import workloads as W
+    from .batch import BatchSolver
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=("batch200", "elbo120", "cpu40"))
+    a = ap.parse_args(argv)
+    wl = {"batch200": lambda: W.batch200(nmodel=256), "elbo120": W.elbo120, "cpu40": W.cpu40}[a.workload]()
+    if a.workload == "cpu40":  # the benchmark's 256 models of the 40x40 workload
+        wl.coef = wl.coef[0][None] + np.random.default_rng(11).normal(0.0, 50.0, (256,) + wl.coef.shape[1:])
+    s = BatchSolver.from_workload(wl, precision="fp32", device="cuda:0")
+    print(json.dumps({"workload": a.workload, "cache": cache_path(),
+                      **{k: s.tuning[k] for k in ("tile2", "group", "lanes", "chunk", "cached")}}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,6 @@
 """Profiling driver: runs the solver's kernels at a fixed shape under a profiler.
 
-    python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15]
+    python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15] [--tile2 1]
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
     python -m mun_amd.profile_kernels --summarize-trace KERNEL_TRACE_CSV [--top 30]
 
@@ -136,6 +136,7 @@ def main(argv=None):
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--tile-rows", type=int, default=15, help="step-kernel tile height (15, 7 or 3)")
+    ap.add_argument("--tile2", type=int, default=1, help="two-step kernel tile shape (sem_plan_set_tile2)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two counter passes of this driver (no GPU work)")
     ap.add_argument("--out", help="JSON file for --summarize-pmc")
@@ -165,7 +166,8 @@ def main(argv=None):
     print(f"[profile] copy {a.copy_mb} MiB x3 done", flush=True)
     wl = W.batch200(nmodel=a.batch, nt=a.nt)
     wl.mesh.checkpoint_every = a.checkpoint_every
-    s = BatchSolver.from_workload(wl, tile_rows=a.tile_rows, precision="fp32", device=dev)
+    s = BatchSolver.from_workload(wl, tile_rows=a.tile_rows, precision="fp32", device=dev, tune=False)
+    s.plan.set_tile2(a.tile2)
     coef = torch.as_tensor(wl.coef, device=dev)
     _, c2 = s.velocity(coef)
     srcn, model_of = s._expand(a.batch)

@@ -380,7 +380,10 @@ class SEMServer:
             if self.two_step:
                 b["w2"], b["om2"] = z(E), z(E)
             if hist:
-                b["hist"] = z(req.nt + 1, E)  # u^0 .. u^nt; u^0 = 0
+                # u^0 .. u^nt: slot m + 1 is written before any launch reads
+                # it, so only u^0 = 0 needs clearing
+                b["hist"] = torch.empty((req.nt + 1, E), dtype=torch.float32, device=dev)
+                b["hist"][0].zero_()
             else:
                 b.update({"u0": z(E), "u1": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E)})
             if req.obs is not None:

@@ -7,17 +7,19 @@ element, degree-4 mesh (801 x 801 GLL nodes, 4 km x 4 km), 5-layer background
 + adjoint + gradient projected onto the 256 B-spline nodes, FP32 state with
 FP64 gradient accumulation.
 
-The plan picks its schedule from free device memory (Plan.schedule_auto):
-on one MI355X the 256 solves run as groups of 5-6 that keep their whole FP32
-forward history (no recompute), three groups in flight on separate streams,
-two time steps per step-kernel launch.
+The batch runs under the autotuned schedule (mun_amd.autotune.tune_batch,
+cached per device and workload): on one MI355X, groups of 8 solves that keep
+their whole FP32 forward history (no recompute), two groups in flight on
+separate streams, two time steps per step-kernel launch on 19 x 13-element
+tiles, gradient correlation per chunk of 64 adjoint steps.
 
 Printed beside it: the plain path (one solve per launch sequence, batch 1),
 the live roofline of the time-step kernel (HIP events on the launch stream,
-launched on the solves in flight), the whole solve's algorithmic bandwidth
-over the timed region, and the CPU oracle (FP64, structured like the
-reference: assembled CSR stiffness + sparse mat-vec per step) timed on a
-bounded sample.
+launched on the solves in flight, and on 64 solves streaming from HBM), the
+whole solve's algorithmic bandwidth over the timed region, the ELBO and 40 x
+40 workloads, the served stream of uneven requests, and the CPU restatement
+(FP64, structured like the reference: assembled CSR stiffness + sparse
+mat-vec per step, torch on all host cores) timed on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--nt NT]
 """
@@ -33,6 +35,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# hardware queues per process for HIP (its default is 4), read when the GPU is
+# first used: the served leg runs one stream per mesh-size lane and a
+# finishing stream beside the default stream, and lanes that share a queue
+# run one after the other
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of a time step per node and solve, the standard two-level
@@ -238,7 +245,7 @@ def main():
     solver = BatchSolver.from_workload(wl, precision="fp32", adjoint_mode="reference", device=dev)
     plan = solver.plan
     if solver.tuning:
-        log(f"[bench] tuning ({'cached' if solver.tuning.get('cached') else 'timed'}, "
+        log(f"[bench] tuning ({solver.tuning.get('source', 'timed')}, "
             f"{time.perf_counter() - t_tune:.1f} s): " +
             ", ".join(f"{k} {solver.tuning[k]}" for k in ("tile2", "group", "lanes", "chunk")))
     coef = torch.as_tensor(wl.coef, device=dev)
@@ -353,7 +360,7 @@ def main():
            "config": {"workload": f"batch200: {args.batch} models x (200x200 deg-4 elements, 801x801 nodes, "
                                   f"{m.nt} steps) forward+adjoint+gradient; {sched_txt}",
                       "batch": args.batch, "nt": m.nt, "nodes": m.N, "schedule": sched,
-                      "tuned": ({k: solver.tuning[k] for k in ("tile2", "group", "lanes", "chunk")}
+                      "tuned": ({k: solver.tuning[k] for k in ("tile2", "group", "lanes", "chunk", "source")}
                                 if solver.tuning else None),
                       "tile2": list(plan.get_tile2()), "parallelism": "none (1 GPU)"},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -414,26 +421,44 @@ def main():
         log(f"[bench] served stream: {args.serve_requests} requests, Poisson {args.serve_rate}/s, "
             f"meshes 20..160^2, nt scale {args.serve_nt_scale}")
         import dataclasses
+        import gc
         reqs = W.served_solve_requests(args.serve_requests, args.serve_rate, nt_scale=args.serve_nt_scale)
         at_once = [dataclasses.replace(r, arrival=0.0) for r in reqs]
         torch.cuda.synchronize()
-        # both paths warm first (lanes, captured rounds, tile choices, plans), as in service
-        srv = SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000)
-        srv.run([dataclasses.replace(r, arrival=0.0) for r in reqs], realtime=False)
+
+        def serve(loop):
+            # a warm server (lanes, captured rounds, tile choices, its arena),
+            # as in service: the stream at its arrival times, then saturated
+            # (every request present at t = 0: throughput, not latency)
+            srv = SEMServer(hbm_budget_bytes=120e9, segment=100, use_graph=True, nt_max=5000, loop=loop)
+            srv.run([dataclasses.replace(r, arrival=0.0) for r in reqs], realtime=False)
+            st = latency_stats(srv.run(reqs, progress=log))
+            sat_ = latency_stats(srv.run(at_once))
+            for lane in srv.lanes.values():
+                if hasattr(lane, "close"):
+                    lane.close()
+            del srv
+            gc.collect()
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+            return st, sat_
+
+        served, sat = serve("device")
+        served_host, sat_host = serve("host")
         plans = {}
         run_plain([dataclasses.replace(r, arrival=0.0) for r in reqs], plans=plans)
-        served = latency_stats(srv.run(reqs, progress=log))
         plain = latency_stats(run_plain(reqs, progress=log, plans=plans))
-        # saturated: every request present at t = 0 (throughput, not latency)
-        sat = latency_stats(srv.run(at_once))
         sat_plain = latency_stats(run_plain(at_once, plans=plans))
         out["served"] = {"workload": f"{args.serve_requests} Poisson({args.serve_rate}/s) forward+adjoint requests, "
                                      f"meshes {{20,40,80,160}}^2 deg-4, {int(1000 * args.serve_nt_scale)}-"
                                      f"{int(5000 * args.serve_nt_scale)} steps, 20-200 receivers",
-                         "warm": "both paths served the stream once before timing",
+                         "warm": "every path served the stream once before timing",
+                         "engine": "device-resident request loop (queue, slot states and round scheduling on "
+                                   "the device; one captured round replayed)",
                          "continuous": served, "plain_one_at_a_time": plain,
                          "saturated_continuous": sat, "saturated_plain": sat_plain,
-                         "saturated_throughput_ratio": round(sat["solves_per_s"] / sat_plain["solves_per_s"], 3)}
+                         "saturated_throughput_ratio": round(sat["solves_per_s"] / sat_plain["solves_per_s"], 3),
+                         "host_scheduled_loop": {"continuous": served_host, "saturated_continuous": sat_host}}
         out["latency_s"] = {"p50": round(served["p50_s"], 4), "p99": round(served["p99_s"], 4),
                             "note": "served stream (continuous batching), arrival to result"}
         log(f"[bench] served p50 {served['p50_s']:.3f}s p99 {served['p99_s']:.3f}s {served['solves_per_s']:.2f}/s; "

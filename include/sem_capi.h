@@ -419,6 +419,78 @@ int sem_residual(sem_plan* plan, const void* seis, const double* obs, void* res,
 int sem_gradient_finalize(sem_plan* plan, const double* G, const void* c2_padded, double scale, double* grad_c,
                           double* grad_c2, void* stream);
 
+/* ---- device-resident request loop -------------------------------------------
+ * A lane of continuous batching whose requests, slot states and per-round slot
+ * tables live in device memory.  The host pushes admitted requests into a
+ * device queue (sem_loop_push, stream-ordered) and enqueues rounds
+ * (sem_loop_round); each round, on the device: picks queued requests into free
+ * slots in arrival order, writes the slot table, advances every slot by one
+ * segment of `segment` steps (the slot kernels above, so a solve's arithmetic is
+ * the batch path's), records the final samples and forms the residual and misfit
+ * of the slots whose forward pass ended, assembles the nodal gradient of the
+ * solves that completed, moves every slot to its next phase and frees the
+ * completed ones, appending their ids to a completion list in host-mapped
+ * memory (sem_loop_poll).  A round's launches and arguments never change, so a
+ * captured graph of one round serves every round.  No reference counterpart:
+ * the reference runs one solve per call (core.py:1067) and its drivers loop over
+ * samples and sources (CO2_plume/one_stage/single_cubic/RQS_NF.py:1035-1072,
+ * 1488-1492). */
+typedef struct {
+    long long id;          /* caller's request id, echoed in the completion list       */
+    int nt;                /* time steps (<= the plan's nt)                            */
+    int full_history;      /* 1: hist holds u^0..u^nt (no recompute); 0: checkpoints   */
+    int src_node;          /* injection node (sem_source_node)                         */
+    int rec_set;           /* receiver set id                                          */
+    int nrec;              /* receivers of the set (columns of seis / resid / obs used) */
+    int reserved;
+    double inv_sigma2;     /* gradient scale 1 / (noise_std^2 + 1e-24)                 */
+    const void* c2;        /* padded squared velocity                                  */
+    const void* wavelet;   /* [nt]                                                     */
+    void* seis;            /* [nt][rec_pitch] synthetic seismograms (state precision)  */
+    void* resid;           /* [nt][rec_pitch] residual (state precision)               */
+    const double* obs;     /* [nt][rec_pitch] observations                             */
+    double* misfit;        /* [1] 0.5 ||resid||^2                                      */
+    double* G;             /* gradient accumulator (sem_plan_gradient_elems), zeroed   */
+    double* grad_c;        /* [N] d loglik / d c, written when the solve completes      */
+    void* w;               /* forward increment, zeroed                                */
+    void* w2;              /* second forward increment buffer (two-step loops), zeroed */
+    void* om;              /* adjoint increment, zeroed                                */
+    void* om2;             /* second adjoint increment buffer (two-step loops), zeroed */
+    void* ring;            /* chunk + 1 adjoint states, zeroed                         */
+    void* hist;            /* full history: nt + 1 states, u^0 zeroed                  */
+    void* u0;              /* checkpoints: state pair, zeroed                          */
+    void* u1;
+    void* cku;             /* checkpoints: ceil(nt / segment) states of u and of w;    */
+    void* ckw;             /*   entry 0 zeroed                                         */
+    void* seg;             /* checkpoints: segment + 1 states                          */
+} sem_request;
+
+typedef struct sem_loop sem_loop;
+/* nslots <= 256 slots advancing `segment` steps per round (two_step: full-history
+ * forward, recompute and adjoint slots two steps per launch; needs an even
+ * segment and correlation chunk); queue_capacity: requests pushed and not yet
+ * completed, at most.  Allocates the loop's device and host-mapped state. */
+int sem_loop_create(sem_plan* plan, int nslots, int segment, int two_step, int rec_pitch, int queue_capacity,
+                    sem_loop** out);
+int sem_loop_destroy(sem_loop* loop);
+/* Adjoint steps per gradient-correlation chunk of the loop (ring states per
+ * request: chunk + 1). */
+int sem_loop_chunk(const sem_loop* loop, int* chunk);
+/* Enqueue one request on `stream` (after the work that fills its buffers). */
+int sem_loop_push(sem_loop* loop, const sem_request* req, void* stream);
+/* Enqueue one round on `stream`; allocates nothing, capturable. */
+int sem_loop_round(sem_loop* loop, void* stream);
+/* Completion records since index `since` (the list is cumulative): up to `max`
+ * ids and the rounds that completed them; *total = completions so far.  Reads
+ * host-mapped memory written by the device (no device call).  Records older than
+ * the list's 4096 most recent are gone (SEM_E_INVALID). */
+int sem_loop_poll(sem_loop* loop, long long since, long long* ids, long long* rounds, int max, long long* total);
+/* Counters: requests pushed, sem_loop_round calls (a round captured into a graph
+ * counts once however often it is replayed), rounds completed on the device,
+ * slots occupied after the last completed round. */
+int sem_loop_counts(const sem_loop* loop, long long* pushed, long long* rounds_enqueued, long long* rounds_done,
+                    int* active);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* sem_last_error(void);
 

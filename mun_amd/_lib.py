@@ -28,7 +28,9 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_slots_correlate", "sem_plan_gradient_elems", "sem_plan_set_schedule", "sem_plan_get_schedule",
            "sem_plan_schedule_auto", "sem_plan_set_streaming", "sem_plan_set_step_fusion",
            "sem_plan_get_step_fusion", "sem_slots_step2", "sem_bspline_project_noalloc", "sem_plan_set_tile2",
-           "sem_plan_get_tile2", "sem_plan_set_corr_chunk", "sem_plan_get_corr_chunk", "sem_last_error"]
+           "sem_plan_get_tile2", "sem_plan_set_corr_chunk", "sem_plan_get_corr_chunk", "sem_loop_create",
+           "sem_loop_destroy", "sem_loop_chunk", "sem_loop_push", "sem_loop_round", "sem_loop_poll",
+           "sem_loop_counts", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
 SEM_STREAM_AUTO, SEM_STREAM_PLAIN, SEM_STREAM_NONTEMPORAL = 0, 1, 2
@@ -69,6 +71,13 @@ class SemSlot(C.Structure):
                 ("ck_u", C.c_void_p), ("ck_w", C.c_void_p), ("seg", C.c_void_p), ("seg_stride", C.c_long),
                 ("resid", C.c_void_p), ("G", C.c_void_p), ("ring", C.c_void_p), ("ring_slots", C.c_int),
                 ("reserved", C.c_int), ("w2", C.c_void_p)]
+
+
+class SemRequest(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("nt", C.c_int), ("full_history", C.c_int), ("src_node", C.c_int),
+                ("rec_set", C.c_int), ("nrec", C.c_int), ("reserved", C.c_int), ("inv_sigma2", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("c2", "wavelet", "seis", "resid", "obs", "misfit", "G", "grad_c", "w",
+                                          "w2", "om", "om2", "ring", "hist", "u0", "u1", "cku", "ckw", "seg")]
 
 
 class SemError(RuntimeError):
@@ -131,6 +140,14 @@ def lib() -> C.CDLL:
     L.sem_grid_to_nodes.argtypes = [vp, i, i, vp, vp, vp, i, vp, vp]
     L.sem_curve_velocity_ex.argtypes = [vp, C.POINTER(SemCurveDesc), i, vp, vp, vp, vp, vp]
     L.sem_curve_project_ex.argtypes = [vp, C.POINTER(SemCurveDesc), i, vp, vp, vp, vp]
+    ll, llp = C.c_longlong, C.POINTER(C.c_longlong)
+    L.sem_loop_create.argtypes = [vp, i, i, i, i, i, C.POINTER(vp)]
+    L.sem_loop_destroy.argtypes = [vp]
+    L.sem_loop_chunk.argtypes = [vp, ip]
+    L.sem_loop_push.argtypes = [vp, C.POINTER(SemRequest), vp]
+    L.sem_loop_round.argtypes = [vp, vp]
+    L.sem_loop_poll.argtypes = [vp, ll, llp, llp, i, llp]
+    L.sem_loop_counts.argtypes = [vp, llp, llp, llp, ip]
     for name in EXPORTS:
         if name != "sem_last_error":
             getattr(L, name).restype = C.c_int

@@ -52,8 +52,10 @@ constexpr int NXCD = 8;           // XCDs of an MI355X
 constexpr size_t LLC_BYTES = size_t(256) << 20;
 
 // MODE_ADJS: adjoint step; its gradient correlation runs later per chunk of
-// steps in corr_kernel.
-enum StepMode { MODE_FWD = 0, MODE_REC = 2, MODE_ADJS = 3 };
+// steps in corr_kernel.  MODE_ANY (slot launches only): each workgroup takes
+// the forward or the adjoint step by its slot's phase, so one launch advances
+// slots of both directions.
+enum StepMode { MODE_FWD = 0, MODE_REC = 2, MODE_ADJS = 3, MODE_ANY = 4 };
 
 // Global-memory qualified pointers: lets the compiler emit global_* instead of
 // flat_* instructions for pointers that come from the slot table.
@@ -162,6 +164,7 @@ struct StepArgs {
     const T* Dx;                       // [NX padded to 4]  exp(-sx dt)
     const T* Dz;                       // [NZ padded to 4]  exp(-sz dt)
     int damp;
+    int damp_adj;                      // MODE_ANY launches: damp of the adjoint slots
     // point source (forward): a[src] += dt^2 s_n
     const int* src_node;               // lexicographic node id (jz * NX + ix)
     const T* wavelet; long wav_bs; int n;
@@ -431,17 +434,23 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
 // more memory latency; the forward step and the FP64 instances keep the
 // compiler's choice (103 VGPRs, 4 waves; FP64 about 200), since at 5 waves
 // they spill.
+// LDS of step_kernel: each element's published edge contributions.
+template <typename T, int P, int EZ>
+struct StepLds {
+    T xr_col[EZ][EX][P + 1];  // element's i = P column of contributions
+    T xr_row[EZ][EX][P];      // element's j = P row (i < P)
+};
+
+// One step of the workgroup's (tile, solve); damp: apply the sponge.
 template <typename T, int P, int MODE, int EZ, bool NTS>
-__global__ __launch_bounds__(tile_threads(EZ))
-__attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
-void step_kernel(const StepArgs<T, P> a) {
+__device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int damp, StepLds<T, P, EZ>& lds) {
 #pragma clang fp contract(off)
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
     constexpr bool ADJ = (MODE == MODE_ADJS);
-    __shared__ T xr_col[EZ][EX][NP];   // element's i = P column of contributions
-    __shared__ T xr_row[EZ][EX][P];    // element's j = P row (i < P)
+    auto& xr_col = lds.xr_col;
+    auto& xr_row = lds.xr_row;
 
     // XCD-contiguous numbering of (tile, solve) pairs
     const int wgid = blockIdx.x;
@@ -523,7 +532,7 @@ void step_kernel(const StepArgs<T, P> a) {
                     if constexpr (NTS) load4nt(wq, wv[j]);
                     else load4(wq, wv[j]);
                 }
-                if (a.damp) {
+                if (damp) {
                     load4(gm(a.Dx) + P * ex, dxv);
                     load4(gm(a.Dz) + P * ez, dzv);
                 }
@@ -613,7 +622,7 @@ void step_kernel(const StepArgs<T, P> a) {
         auto inv_wx = [&](int i) -> T { return (i == 0 && ex > 0) ? a.ec.iw_shared : a.ec.iw[i]; };
         auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
         auto damp_at = [&](int j, int i) -> T {
-            if (!a.damp) return T(1);
+            if (!damp) return T(1);
             const T dxi = (i < P) ? dxv[i] : gm(a.Dx)[P * ex + i];
             const T dzj = (j < P) ? dzv[j] : gm(a.Dz)[P * ez + j];
             return dxi * dzj;
@@ -670,6 +679,24 @@ void step_kernel(const StepArgs<T, P> a) {
                     }
                 }
         }
+    }
+}
+
+template <typename T, int P, int MODE, int EZ, bool NTS>
+__global__ __launch_bounds__(tile_threads(EZ))
+__attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
+void step_kernel(const StepArgs<T, P> a) {
+    __shared__ StepLds<T, P, EZ> lds;
+    if constexpr (MODE == MODE_ANY) {
+        const int wgid = blockIdx.x;
+        const int lin = (wgid % NXCD) * a.nper + wgid / NXCD;
+        if (lin >= a.total) return;  // whole workgroup leaves together
+        if (a.slots[lin / a.ntiles].phase == SEM_SLOT_ADJOINT)
+            step_body<T, P, MODE_ADJS, EZ, NTS>(a, a.damp_adj, lds);
+        else
+            step_body<T, P, MODE_FWD, EZ, NTS>(a, a.damp, lds);
+    } else {
+        step_body<T, P, MODE, EZ, NTS>(a, a.damp, lds);
     }
 }
 
@@ -811,22 +838,37 @@ inline constexpr int TILE2_EX[NTILE2] = {19, 19, 35, 35};
 inline constexpr int TILE2_EZ[NTILE2] = {26, 13, 14, 28};
 __host__ __device__ constexpr int tile2_threads(int ex, int ez) { return (ex * ez + 63) / 64 * 64; }
 
+// LDS of step2_kernel: edge contributions, the level-(n+1) nodes of the ring
+// and O, and the increment of O's nodes.
+template <typename T, int P, int EX2, int EZ2>
+struct Step2Lds {
+    static constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;
+    static constexpr int U1_H = 4 * (EZ2 - 1) + 1;
+    static constexpr int W1_W = (4 * (EX2 - 3) + 1 + 3) / 4 * 4;
+    static constexpr int W1_H = 4 * (EZ2 - 3) + 1;
+    T xr_col[EZ2][EX2][P + 1];
+    T xr_row[EZ2][EX2][P];
+    __attribute__((aligned(16))) T U1[U1_H][U1_W];
+    __attribute__((aligned(16))) T W1[W1_H][W1_W];
+};
+
+// Two steps of the workgroup's (tile, solve); damp: apply the sponge.
 template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
-__global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const StepArgs<T, P> a) {
+__device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int damp, Step2Lds<T, P, EX2, EZ2>& lds) {
 #pragma clang fp contract(off)
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     static_assert(MODE == MODE_FWD || MODE == MODE_ADJS, "forward or adjoint steps");
     constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
-    constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;  // level-(n+1) nodes of the ring + O
-    constexpr int U1_H = 4 * (EZ2 - 1) + 1;
-    constexpr int W1_W = (4 * TX2 + 1 + 3) / 4 * 4;        // increment of O's nodes
-    constexpr int W1_H = 4 * TZ2 + 1;
+    constexpr int U1_W = Step2Lds<T, P, EX2, EZ2>::U1_W;  // level-(n+1) nodes of the ring + O
+    constexpr int U1_H = Step2Lds<T, P, EX2, EZ2>::U1_H;
+    constexpr int W1_W = Step2Lds<T, P, EX2, EZ2>::W1_W;  // increment of O's nodes
+    constexpr int W1_H = Step2Lds<T, P, EX2, EZ2>::W1_H;
     constexpr int NP = P + 1;
     constexpr bool ADJ = (MODE == MODE_ADJS);
-    __shared__ T xr_col[EZ2][EX2][NP];
-    __shared__ T xr_row[EZ2][EX2][P];
-    __shared__ __attribute__((aligned(16))) T U1[U1_H][U1_W];
-    __shared__ __attribute__((aligned(16))) T W1[W1_H][W1_W];
+    auto& xr_col = lds.xr_col;
+    auto& xr_row = lds.xr_row;
+    auto& U1 = lds.U1;
+    auto& W1 = lds.W1;
 
     const int wgid = blockIdx.x;
     const int lin = (wgid % NXCD) * a.nper2 + wgid / NXCD;
@@ -879,7 +921,7 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
     auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
     T dxv[P], dzv[P];
     auto damp_at = [&](int j, int i) -> T {
-        if (!a.damp) return T(1);
+        if (!damp) return T(1);
         const T dxi = (i < P) ? dxv[i] : gm(a.Dx)[P * ex + i];
         const T dzj = (j < P) ? dzv[j] : gm(a.Dz)[P * ez + j];
         return dxi * dzj;
@@ -971,7 +1013,7 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
                 if constexpr (NTS) load4nt(wq, wv[j]);
                 else load4(wq, wv[j]);
             }
-            if (a.damp) {
+            if (damp) {
                 load4(gm(a.Dx) + P * ex, dxv);
                 load4(gm(a.Dz) + P * ez, dzv);
             }
@@ -1127,6 +1169,22 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
                     }
                 }
         }
+    }
+}
+
+template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
+__global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const StepArgs<T, P> a) {
+    __shared__ Step2Lds<T, P, EX2, EZ2> lds;
+    if constexpr (MODE == MODE_ANY) {
+        const int wgid = blockIdx.x;
+        const int lin = (wgid % NXCD) * a.nper2 + wgid / NXCD;
+        if (lin >= a.total2) return;  // whole workgroup leaves together
+        if (a.slots[lin / a.ntiles2].phase == SEM_SLOT_ADJOINT)
+            step2_body<T, P, MODE_ADJS, NTS, EX2, EZ2>(a, a.damp_adj, lds);
+        else
+            step2_body<T, P, MODE_FWD, NTS, EX2, EZ2>(a, a.damp, lds);
+    } else {
+        step2_body<T, P, MODE, NTS, EX2, EZ2>(a, a.damp, lds);
     }
 }
 
@@ -1330,6 +1388,31 @@ __global__ void pad_rows_kernel(const T* src, T* dst, int NX, int NZ, int pitch,
 // nwaves, ...), its lanes the columns; per-thread FP64 sums are combined in a
 // fixed tree, so the misfit is deterministic.
 constexpr int RESID_THREADS = 1024;
+// The body for one solve (seis/obs/res at its first row); red: RESID_THREADS
+// doubles of LDS.  Called by the whole block.
+template <typename T>
+__device__ __forceinline__ void residual_rows(const T* seis, const double* obs, T* res, double* misfit, int nrec,
+                                              int nt, int pitch, int w0, int w1, int decim, double* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = RESID_THREADS / 64;
+    double acc = 0.0;
+    for (int row = wave; row < nt; row += nwaves) {
+        const bool in = row >= w0 && row < w1 && (row - w0) % decim == 0;
+        const long o = (long)row * pitch;
+        for (int c = lane; c < pitch; c += 64) {
+            const double rv = (c < nrec && in) ? (double)seis[o + c] - obs[o + c] : 0.0;
+            res[o + c] = (T)rv;
+            acc += rv * rv;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RESID_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *misfit = 0.5 * red[0];
+}
+
 template <typename T>
 __global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, const double* obs, T* res,
                                                                  double* misfit, const int* nrec_of, int nrec,
@@ -1337,25 +1420,8 @@ __global__ __launch_bounds__(RESID_THREADS) void residual_kernel(const T* seis, 
                                                                  int decim) {
     const long b = blockIdx.x;
     if (nrec_of) nrec = nrec_of[b];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = RESID_THREADS / 64;
-    double acc = 0.0;
-    for (int row = wave; row < nt; row += nwaves) {
-        const bool in = row >= w0 && row < w1 && (row - w0) % decim == 0;
-        const long o = b * bs + (long)row * pitch;
-        for (int c = lane; c < pitch; c += 64) {
-            const double rv = (c < nrec && in) ? (double)seis[o + c] - obs[o + c] : 0.0;
-            res[o + c] = (T)rv;
-            acc += rv * rv;
-        }
-    }
     __shared__ double red[RESID_THREADS];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = RESID_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) misfit[b] = 0.5 * red[0];
+    residual_rows<T>(seis + b * bs, obs + b * bs, res + b * bs, misfit + b, nrec, nt, pitch, w0, w1, decim, red);
 }
 
 // Wavefield snapshot: box [iz0, iz0+nzr) x [ix0, ix0+nxr) of every solve's

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -19,6 +20,7 @@
 #include "../../include/sem_capi.h"
 #include "sem_curve.h"
 #include "sem_kernels.h"
+#include "sem_loop.h"
 
 using namespace sem;
 
@@ -1736,6 +1738,14 @@ static int slots_launch(sem_plan* pl, const sem_slot* slots, int nslots, int m, 
     const bool nts = use_nt(pl, nslots);
     const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_RECOMPUTE) | (1u << SEM_SLOT_FORWARD_HIST);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
+    if ((mask & fwd) && (mask & adj)) {
+        // one launch for both directions: a second launch per step costs a
+        // kernel boundary (dispatch, drain, cache release) even where none of
+        // its workgroups has work; the per-workgroup branch costs nothing
+        a.damp = 1;
+        a.damp_adj = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
+        return launch_step<T, MODE_ANY>(pl, a, st, nts);
+    }
     if (mask & fwd) {
         a.damp = 1;
         if ((rc = launch_step<T, MODE_FWD>(pl, a, st, nts))) return rc;
@@ -1765,6 +1775,12 @@ static int slots_launch2(sem_plan* pl, const sem_slot* slots, int nslots, int m,
     const bool nts = use_nt(pl, nslots);
     const unsigned fwd = (1u << SEM_SLOT_FORWARD_HIST) | (1u << SEM_SLOT_RECOMPUTE);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
+    if ((mask & fwd) && (mask & adj)) {
+        a.phase_mask = mask & (fwd | adj);
+        a.damp = 1;
+        a.damp_adj = pl->d.adjoint_mode == SEM_ADJ_EXACT ? 1 : 0;
+        return launch_step2<T, MODE_ANY>(pl, a, st, nts);
+    }
     if (mask & fwd) {
         a.phase_mask = mask & fwd;
         a.damp = 1;
@@ -1887,6 +1903,204 @@ int sem_gradient_finalize(sem_plan* pl, const double* G, const void* c2p, double
                                                           pl->d.nelem_z, pl->NX, pl->NZ, pl->pitch,
                                                           (const float*)c2p, nullptr, pl->J * scale, grad_c, grad_c2);
     LAUNCH_CHECK();
+    return SEM_OK;
+} ABI_CATCH
+
+}  // extern "C"
+
+// ---- device-resident request loop ------------------------------------------
+
+struct sem_loop {
+    sem_plan* pl = nullptr;
+    int dev = -1, rec_pitch = 0;
+    LoopParams p{};
+    void* dmem = nullptr;
+    LoopHost* host = nullptr;  // host address of the mapped block
+    long long pushed = 0, rounds = 0;
+    ~sem_loop() {
+        if (dmem) (void)hipFree(dmem);
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+namespace {
+
+// Adjoint steps per correlation chunk of a loop: at most 20, the segment split
+// into near-equal chunks, rounded up to an even length when shorter than the
+// segment (two-step launches never straddle an even chunk boundary); the ring
+// then holds chunk + 1 states per request.
+int loop_chunk(int S) {
+    const int n = (S + 19) / 20;
+    const int c = (S + n - 1) / n;
+    return (c % 2 && c < S) ? c + 1 : c;
+}
+
+template <typename T>
+int loop_round(sem_loop* lp, hipStream_t st) {
+    sem_plan* pl = lp->pl;
+    const LoopParams& p = lp->p;
+    const int C = p.nslots;
+    const unsigned FWD = 1u << SEM_SLOT_FORWARD, RC = 1u << SEM_SLOT_RECOMPUTE, ADJ = 1u << SEM_SLOT_ADJOINT,
+                   FH = 1u << SEM_SLOT_FORWARD_HIST;
+    int rc;
+    loop_begin_kernel<<<1, LOOP_MAX_SLOTS, 0, st>>>(p);
+    LAUNCH_CHECK();
+    for (int m = 0; m < p.S; ++m) {
+        if (p.two_step) {
+            // checkpointed forward slots one step per launch (their state pair
+            // would alias a two-step launch's output); the others in pairs
+            if ((rc = slots_launch<T>(pl, p.table, C, m, FWD, lp->rec_pitch, false, st))) return rc;
+            if (m % 2 == 0 && (rc = slots_launch2<T>(pl, p.table, C, m, FH | RC | ADJ, lp->rec_pitch, st))) return rc;
+        } else if ((rc = slots_launch<T>(pl, p.table, C, m, FWD | RC | FH | ADJ, lp->rec_pitch, false, st))) {
+            return rc;
+        }
+        if ((m + 1) % p.chunk == 0 || m == p.S - 1)
+            if ((rc = slots_corr<T>(pl, p.table, C, m / p.chunk, p.chunk, st))) return rc;
+    }
+    if ((rc = slots_launch<T>(pl, p.ftable, C, 0, FWD | FH, lp->rec_pitch, true, st))) return rc;
+    loop_residual_kernel<T><<<C, RESID_THREADS, 0, st>>>(p, lp->rec_pitch);
+    LAUNCH_CHECK();
+    dim3 grid((unsigned)std::min<long>((pl->N + 255) / 256, 2048), C);
+    loop_finalize_kernel<T><<<grid, 256, 0, st>>>(p, pl->sb, pl->gplane, pl->nexp, pl->d.nelem_x, pl->d.nelem_z,
+                                                  pl->NX, pl->NZ, pl->pitch, pl->J);
+    LAUNCH_CHECK();
+    loop_advance_kernel<<<1, 64, 0, st>>>(p);
+    LAUNCH_CHECK();
+    return SEM_OK;
+}
+
+int loop_device_check(const sem_loop* lp) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != lp->dev) return fail(SEM_E_INVALID, "a loop runs on the device it was created on");
+    return SEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem_loop_create(sem_plan* pl, int nslots, int segment, int two_step, int rec_pitch, int queue_capacity,
+                    sem_loop** out) try {
+    if (!pl || !out || nslots < 1 || nslots > LOOP_MAX_SLOTS || segment < 1 || rec_pitch < 1 || queue_capacity < 1)
+        return fail(SEM_E_INVALID, "sem_loop_create: bad argument");
+    *out = nullptr;
+    const int chunk = loop_chunk(segment);
+    if (two_step && (segment % 2 || chunk % 2))
+        return fail(SEM_E_INVALID, "sem_loop_create: two steps per launch need an even segment and correlation chunk");
+    if ((long)std::max(std::max(pl->ntiles, pl->ntiles2), pl->ctiles) * nslots >= (1L << 31) - 64)
+        return fail(SEM_E_INVALID, "too many slots for this mesh");
+    std::unique_ptr<sem_loop> lp(new sem_loop);
+    lp->pl = pl;
+    lp->rec_pitch = rec_pitch;
+    HIP_TRY(hipGetDevice(&lp->dev));
+    LoopParams& p = lp->p;
+    p.nslots = nslots;
+    p.S = segment;
+    p.two_step = two_step ? 1 : 0;
+    p.chunk = chunk;
+    p.qcap = queue_capacity;
+    p.elem = pl->d.precision == SEM_FP64 ? 8 : 4;
+    p.E = pl->sb;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bq = up(sizeof(sem_request) * queue_capacity), bs = up(sizeof(LoopSlot) * nslots),
+                 bt = up(sizeof(sem_slot) * nslots), bc = up(sizeof(LoopCtl));
+    const size_t total = bq + bs + 2 * bt + bc;
+    hipError_t e = hipMalloc(&lp->dmem, total);
+    if (e != hipSuccess) {
+        lp->dmem = nullptr;
+        return fail(SEM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(hipMemset(lp->dmem, 0, total));
+    char* b = (char*)lp->dmem;
+    p.q = (sem_request*)b;
+    p.st = (LoopSlot*)(b + bq);
+    p.table = (sem_slot*)(b + bq + bs);
+    p.ftable = (sem_slot*)(b + bq + bs + bt);
+    p.ctl = (LoopCtl*)(b + bq + bs + 2 * bt);
+    void* h = nullptr;
+    e = hipHostMalloc(&h, sizeof(LoopHost), hipHostMallocMapped | hipHostMallocCoherent);
+    if (e != hipSuccess) return fail(SEM_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    lp->host = (LoopHost*)h;
+    std::memset(h, 0, sizeof(LoopHost));
+    void* dh = nullptr;
+    HIP_TRY(hipHostGetDevicePointer(&dh, h, 0));
+    p.host = (LoopHost*)dh;
+    HIP_TRY(hipDeviceSynchronize());  // the cleared state is in place before any stream uses it
+    *out = lp.release();
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_loop_destroy(sem_loop* lp) try {
+    delete lp;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_loop_chunk(const sem_loop* lp, int* chunk) try {
+    if (!lp || !chunk) return fail(SEM_E_INVALID, "sem_loop_chunk: bad argument");
+    *chunk = lp->p.chunk;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_loop_push(sem_loop* lp, const sem_request* r, void* stream) try {
+    if (!lp || !r) return fail(SEM_E_INVALID, "sem_loop_push: bad argument");
+    const sem_plan* pl = lp->pl;
+    const LoopParams& p = lp->p;
+    if (r->nt < 1 || r->nt > pl->d.nt || r->nrec < 0 || r->nrec > lp->rec_pitch || r->src_node < 0 ||
+        r->src_node >= pl->sb || r->rec_set < 0 || r->rec_set >= (int)pl->sets.size())
+        return fail(SEM_E_INVALID, "sem_loop_push: bad request (nt, nrec, source node or receiver set)");
+    bool ok = r->c2 && r->wavelet && r->seis && r->resid && r->obs && r->misfit && r->G && r->grad_c && r->w && r->om &&
+              r->ring;
+    if (p.two_step) ok = ok && r->w2 && r->om2 && r->w2 != r->w && r->om2 != r->om;
+    if (r->full_history)
+        ok = ok && r->hist;
+    else
+        ok = ok && r->u0 && r->u1 && r->cku && r->ckw && r->seg;
+    if (!ok) return fail(SEM_E_INVALID, "sem_loop_push: a buffer the request needs is missing");
+    int rc = loop_device_check(lp);
+    if (rc) return rc;
+    const long long nd = __atomic_load_n(&lp->host->ndone, __ATOMIC_ACQUIRE);
+    if (lp->pushed - nd >= p.qcap)
+        return fail(SEM_E_INVALID, "sem_loop_push: queue full (requests pushed and not completed)");
+    loop_push_kernel<<<1, 64, 0, (hipStream_t)stream>>>(p, *r);
+    LAUNCH_CHECK();
+    ++lp->pushed;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_loop_round(sem_loop* lp, void* stream) try {
+    if (!lp) return fail(SEM_E_INVALID, "sem_loop_round: bad argument");
+    int rc = loop_device_check(lp);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = lp->pl->d.precision == SEM_FP64 ? loop_round<double>(lp, st) : loop_round<float>(lp, st);
+    if (!rc) ++lp->rounds;
+    return rc;
+} ABI_CATCH
+
+int sem_loop_poll(sem_loop* lp, long long since, long long* ids, long long* rounds, int max, long long* total) try {
+    if (!lp || !total || max < 0 || (max > 0 && !ids)) return fail(SEM_E_INVALID, "sem_loop_poll: bad argument");
+    const long long nd = __atomic_load_n(&lp->host->ndone, __ATOMIC_ACQUIRE);
+    if (since < 0 || since > nd) return fail(SEM_E_INVALID, "sem_loop_poll: `since` beyond the completions");
+    const long long n = std::min<long long>(max, nd - since);
+    if (nd - since > LOOP_DONE_CAP && n > 0)
+        return fail(SEM_E_INVALID, "sem_loop_poll: completion records overwritten (poll more often)");
+    for (long long i = 0; i < n; ++i) {
+        const long long k = (since + i) % LOOP_DONE_CAP;
+        ids[i] = lp->host->ids[k];
+        if (rounds) rounds[i] = lp->host->id_round[k];
+    }
+    *total = nd;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_loop_counts(const sem_loop* lp, long long* pushed, long long* rounds_enqueued, long long* rounds_done,
+                    int* active) try {
+    if (!lp) return fail(SEM_E_INVALID, "sem_loop_counts: bad argument");
+    if (pushed) *pushed = lp->pushed;
+    if (rounds_enqueued) *rounds_enqueued = lp->rounds;
+    if (rounds_done) *rounds_done = __atomic_load_n(&lp->host->round, __ATOMIC_ACQUIRE);
+    if (active) *active = __atomic_load_n(&lp->host->active, __ATOMIC_ACQUIRE);
     return SEM_OK;
 } ABI_CATCH
 

@@ -24,14 +24,26 @@ time steps share launches.  Requests that arrive while others run are admitted
 at the next round boundary of their lane as soon as a slot is free and their
 HBM footprint (state + checkpoints + segment buffer + seismograms) fits the
 budget; lanes run on separate streams, so different size classes overlap on
-the device.  A round can be replayed from a captured HIP graph (its launch
+the device.  The budget is one device allocation made when the server starts
+(its arena), and each admitted request takes one block of it, so serving
+allocates no device memory per request.  A round can be replayed from a captured HIP graph (its launch
 arguments only point at the lane's device slot table, which is rewritten
 between rounds).
 
-Rounds are pipelined: the host enqueues a lane's next round (with the state
-transitions, final samples, residuals and finishing work of the previous one,
-all deterministic) while the previous round runs, keeping up to
-ROUNDS_AHEAD rounds per lane on the device, so no round waits for the host.
+Two engines drive the rounds.  With ``loop="device"`` (the default) the
+request queue, the slot states and the round's slot table live on the device
+(``sem_loop_*`` in include/sem_capi.h): the host pushes each admitted request
+into the lane's device queue, replays one captured graph per round (the
+device picks queued requests into free slots, runs the segment, forms
+residuals, assembles completed gradients, advances every slot's phase and
+frees completed slots), and polls a completion list in host-mapped memory;
+only the projection onto the request's B-spline nodes and the copies of the
+results are enqueued per completed solve, on a stream of their own.  With
+``loop="host"`` the host keeps the slot states, writes each round's slot
+table and enqueues the transitions, residuals and finishing work itself,
+with one captured graph per set of phases present.  Either way rounds are
+pipelined: up to ROUNDS_AHEAD rounds per lane are enqueued ahead of the
+host, so no round waits for it.
 
 Each slot's arithmetic is the batch path's, so a result does not depend on
 which other requests share its launches.
@@ -97,10 +109,15 @@ class SolveResult:
 
 
 def corr_chunk(S: int) -> int:
-    """Adjoint steps per gradient-correlation chunk in a lane (<= 10, dividing
-    a segment of S steps into equal chunks)."""
-    n = -(-S // 10)
-    return -(-S // n)
+    """Adjoint steps per gradient-correlation chunk in a lane: at most 20, a
+    segment of S steps split into near-equal chunks, rounded up to an even
+    length when shorter than the segment (two-step launches never straddle an
+    even chunk boundary); the device loop's sem_loop_chunk is the same rule.
+    Longer chunks mean fewer correlation launches per round and a longer
+    adjoint ring (chunk + 1 states) per request."""
+    n = -(-S // 20)
+    c = -(-S // n)
+    return c + 1 if c % 2 and c < S else c
 
 
 def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad_elems=None,
@@ -120,6 +137,104 @@ def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad
     return nodal + traces
 
 
+ALIGN = 256  # bytes: every buffer of a request block starts on this boundary
+
+
+def request_layout(pitch_elems, nt, nrec_pitch, S, chunk, grad_elems, node_elems=0, full_history=False,
+                   two_step=False, elem_bytes=4):
+    """The buffers of one request as laid out in its arena block: a list of
+    (name, shape, dtype, zeroed, byte offset) and the block's size.  Buffers
+    a launch reads before writing are zeroed at admission; the full history
+    needs only u^0 cleared (done separately).  node_elems: doubles of the
+    nodal gradient the device loop writes at completion (0: none)."""
+    E = pitch_elems
+    K = max(1, math.ceil(nt / S))
+    f = torch.float32 if elem_bytes == 4 else torch.float64
+    d = torch.float64
+    bufs = [("w", (E,), f, True), ("om", (E,), f, True), ("ring", (chunk + 1, E), f, True), ("c2p", (E,), f, True),
+            ("G", (grad_elems,), d, True), ("seis", (nt, nrec_pitch), f, True), ("res", (nt, nrec_pitch), f, True),
+            ("obs", (nt, nrec_pitch), d, True), ("misfit", (1,), d, True), ("wav", (nt,), f, False)]
+    if two_step:
+        bufs += [("w2", (E,), f, True), ("om2", (E,), f, True)]
+    if full_history:
+        bufs += [("hist", (nt + 1, E), f, False)]
+    else:
+        bufs += [("u0", (E,), f, True), ("u1", (E,), f, True), ("cku", (K, E), f, True), ("ckw", (K, E), f, True),
+                 ("seg", (S + 1, E), f, True)]
+    if node_elems:
+        bufs += [("gc", (node_elems,), d, False)]
+    out, off = [], 0
+    for name, shape, dt, zero in bufs:
+        out.append((name, shape, dt, zero, off))
+        nb = math.prod(shape) * (4 if dt == torch.float32 else 8)
+        off += -(-nb // ALIGN) * ALIGN
+    return out, off
+
+
+class BlockAllocator:
+    """First-fit allocation of ALIGN-multiple blocks in [0, size), freed
+    blocks merged with free neighbours.  Host bookkeeping only."""
+
+    def __init__(self, size: int):
+        self.size = size // ALIGN * ALIGN
+        self.free = [(0, self.size)]  # (offset, length), sorted by offset, never adjacent
+
+    def alloc(self, nbytes: int) -> Optional[int]:
+        n = -(-max(int(nbytes), 1) // ALIGN) * ALIGN
+        for i, (o, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    del self.free[i]
+                else:
+                    self.free[i] = (o + n, ln - n)
+                return o
+        return None
+
+    def release(self, off: int, nbytes: int):
+        n = -(-max(int(nbytes), 1) // ALIGN) * ALIGN
+        i = 0
+        while i < len(self.free) and self.free[i][0] < off:
+            i += 1
+        if i > 0:
+            po, pl = self.free[i - 1]
+            if po + pl > off:
+                raise ValueError("block released twice or overlapping a free range")
+        if i < len(self.free) and off + n > self.free[i][0]:
+            raise ValueError("block released twice or overlapping a free range")
+        self.free.insert(i, (off, n))
+        if i + 1 < len(self.free) and off + n == self.free[i + 1][0]:  # merge right
+            self.free[i] = (off, n + self.free[i + 1][1])
+            del self.free[i + 1]
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == off:  # merge left
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            del self.free[i]
+
+    def free_bytes(self) -> int:
+        return sum(ln for _, ln in self.free)
+
+
+class DeviceArena:
+    """The serving budget as one device allocation made when the server is
+    created: requests take blocks of it (BlockAllocator), so serving itself
+    allocates no device memory.  A fresh device allocation can block its
+    caller for seconds while the driver prepares the pages; inside a
+    request's admission that stall would land in its latency."""
+
+    def __init__(self, nbytes: int, device):
+        self.alloc_ = BlockAllocator(int(nbytes))
+        self.buf = torch.empty(self.alloc_.size, dtype=torch.uint8, device=device)
+
+    def take(self, nbytes: int) -> Optional[int]:
+        return self.alloc_.alloc(nbytes)
+
+    def give(self, off: int, nbytes: int):
+        self.alloc_.release(off, nbytes)
+
+    def view(self, off: int, shape, dtype) -> torch.Tensor:
+        n = math.prod(shape) * (4 if dtype == torch.float32 else 8)
+        return self.buf[off:off + n].view(dtype).view(shape)
+
+
 @dataclass
 class _Active:
     req: SolveRequest
@@ -134,6 +249,7 @@ class _Active:
     nrec: int = 0
     src_node: int = 0
     hist: bool = False            # full forward history (no recompute)
+    block: int = 0                # offset of the request's block in the server's arena
     wflip: bool = False           # two-step rounds: w^n of the forward is in w2, not w
     omflip: bool = False          # two-step rounds: omega of the adjoint is in om2, not om
 
@@ -249,6 +365,10 @@ class Lane:
     def busy(self) -> bool:
         return bool(self.occupied() or self.queue or self.inflight or self.finishing)
 
+    def outstanding(self) -> int:
+        """Admitted solves not yet completed."""
+        return len(self.occupied())
+
     def phase_mask(self) -> int:
         """Phases present in the lane (bit 1 << SLOT_*): a round launches the
         forward kernel only when a slot runs forward or recompute steps, the
@@ -302,6 +422,126 @@ class Lane:
 TWO_STEP_PHASES = (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT)
 
 
+@dataclass
+class _Pushed:
+    req: SolveRequest
+    buf: Dict[str, torch.Tensor]
+    footprint: int
+    admitted: float
+    nrec: int
+    block: int = 0
+
+
+class LoopLane(Lane):
+    """A size class served by the device-resident request loop (sem_loop_*):
+    the host pushes admitted requests and replays the same captured round;
+    the device does the per-round scheduling."""
+
+    QUEUE_CAPACITY = 1024
+
+    def __init__(self, server: "SEMServer", key, capacity: int, nt_max: int, rec_pitch: int):
+        super().__init__(server, key, capacity, nt_max, rec_pitch)
+        h = C.c_void_p()
+        L.check(self.plan.lib.sem_loop_create(self.plan.h, capacity, self.S, 1 if server.two_step else 0, rec_pitch,
+                                              self.QUEUE_CAPACITY, C.byref(h)))
+        self.loop = h
+        ck = C.c_int()
+        L.check(self.plan.lib.sem_loop_chunk(h, C.byref(ck)))
+        self.chunk = ck.value          # ring states per request: chunk + 1
+        self.pushed: Dict[int, _Pushed] = {}
+        self.npushed = 0
+        self.ndone = 0                 # completions seen (sem_loop_poll)
+        self.round_events = {}         # round number -> completion event of that round
+        self.graph = None
+        self._ids = (C.c_longlong * 256)()
+        self._rounds = (C.c_longlong * 256)()
+
+    def close(self):
+        if self.loop:
+            torch.cuda.synchronize(self.server.device)
+            self.plan.lib.sem_loop_destroy(self.loop)
+            self.loop = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ensure_set_capacity(self):
+        if len(self.plan.rec_sizes) + 1 > self.max_sets:
+            self.max_sets *= 2
+            self._reserve()
+            self.graph = None  # captured arguments point at the old table
+
+    def outstanding(self) -> int:
+        return self.npushed - self.ndone
+
+    def busy(self) -> bool:
+        return bool(self.outstanding() or self.queue or self.inflight or self.finishing)
+
+    def push(self, req: SolveRequest, buf: Dict[str, torch.Tensor], hist: bool, footprint: int, now: float,
+             nrec: int, rec_set: int, src_node: int, block: int):
+        b = buf
+        r = L.SemRequest()
+        r.id, r.nt, r.full_history = req.rid, req.nt, 1 if hist else 0
+        r.src_node, r.rec_set, r.nrec = src_node, rec_set, nrec
+        r.inv_sigma2 = 1.0 / (req.noise_std ** 2 + 1e-24)
+        for name, key in (("c2", "c2p"), ("wavelet", "wav"), ("seis", "seis"), ("resid", "res"), ("obs", "obs"),
+                          ("misfit", "misfit"), ("G", "G"), ("grad_c", "gc"), ("w", "w"), ("w2", "w2"),
+                          ("om", "om"), ("om2", "om2"), ("ring", "ring"), ("hist", "hist"), ("u0", "u0"),
+                          ("u1", "u1"), ("cku", "cku"), ("ckw", "ckw"), ("seg", "seg")):
+            if key in b:
+                setattr(r, name, b[key].data_ptr())
+        with torch.cuda.stream(self.stream):
+            L.check(self.plan.lib.sem_loop_push(self.loop, C.byref(r), C.c_void_p(self.stream.cuda_stream)))
+        self.pushed[req.rid] = _Pushed(req, b, footprint, now, nrec, block)
+        self.npushed += 1
+
+    def launch_round(self, use_graph: bool):
+        lib, sp = self.plan.lib, C.c_void_p(self.stream.cuda_stream)
+        with torch.cuda.stream(self.stream):
+            if use_graph:
+                if self.graph is None:
+                    L.check(lib.sem_receivers_sync(self.plan.h))
+                    torch.cuda.synchronize(self.server.device)
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        L.check(lib.sem_loop_round(self.loop, sp))
+                    self.graph = g
+                self.graph.replay()
+            else:
+                L.check(lib.sem_loop_round(self.loop, sp))
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.round_events[self.rounds] = ev
+        self.inflight.append(ev)
+        self.rounds += 1
+
+    def counts(self) -> dict:
+        """The loop's counters: requests pushed, rounds enqueued and completed
+        on the device, slots occupied after the last completed round."""
+        pushed, enq, done = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        active = C.c_int()
+        L.check(self.plan.lib.sem_loop_counts(self.loop, C.byref(pushed), C.byref(enq), C.byref(done),
+                                              C.byref(active)))
+        return {"pushed": pushed.value, "rounds_enqueued": enq.value, "rounds_done": done.value,
+                "active": active.value, "completed": self.ndone}
+
+    def poll(self):
+        """(request id, round) of the solves completed since the last poll."""
+        out = []
+        total = C.c_longlong()
+        while True:
+            L.check(self.plan.lib.sem_loop_poll(self.loop, self.ndone, self._ids, self._rounds, len(self._ids),
+                                                C.byref(total)))
+            n = min(len(self._ids), total.value - self.ndone)
+            out += [(int(self._ids[i]), int(self._rounds[i])) for i in range(n)]
+            self.ndone += n
+            if self.ndone >= total.value:
+                return out
+
+
 class SEMServer:
     """Continuous-batching forward+adjoint SEM service on one GPU."""
 
@@ -309,8 +549,11 @@ class SEMServer:
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
                  autotune: bool = True, history: str = "auto", two_step: bool = False,
-                 tile2: Optional[int] = None):
-        """history: 'auto' admits a request with its full forward history
+                 tile2: Optional[int] = None, loop: str = "device"):
+        """hbm_budget_bytes: the device memory that holds the admitted
+        requests' buffers, allocated once here as the server's arena; each
+        request takes one block of it (a request that finds no block large
+        enough waits).  history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  two_step: advance
         full-history forward, recompute and adjoint slots two steps per launch
@@ -319,9 +562,15 @@ class SEMServer:
         another was torn down once ran 4x slower for a reason not found.
         Results are the same bits either way.  tile2: the two-step kernel's
         tile shape in every lane (default: tuned per lane when two_step and
-        autotune, else shape 0); results do not depend on it."""
+        autotune, else shape 0); results do not depend on it.  loop: 'device'
+        keeps the request queue, slot states and round scheduling on the
+        device (sem_loop_*), 'host' schedules the rounds from the host; the
+        same bits either way."""
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
+        if loop not in ("device", "host"):
+            raise ValueError("loop must be 'device' or 'host'")
+        self.loop = loop
         self.history = history
         # two steps per launch for full-history forward and adjoint slots
         # (sem_slots_step2); needs an even segment and correlation chunk
@@ -339,6 +588,8 @@ class SEMServer:
         self.autotune = bool(autotune)
         self.tile2 = None if tile2 is None else int(tile2)
         self.lanes: Dict[tuple, Lane] = {}
+        self.finish_stream = None
+        self.arena = DeviceArena(int(self.budget), self.device)
         self.in_use = 0
         self.results: Dict[int, SolveResult] = {}
         self.t0 = None
@@ -351,59 +602,72 @@ class SEMServer:
         key = req.lane_key
         if key not in self.lanes:
             cap = self.capacity.get(req.nelem_x, max(4, min(64, 2560 // max(req.nelem_x, 1))))
-            self.lanes[key] = Lane(self, key, cap, self.nt_max, self.rec_pitch)
+            cls = LoopLane if self.loop == "device" else Lane
+            self.lanes[key] = cls(self, key, cap, self.nt_max, self.rec_pitch)
         return self.lanes[key]
 
     # -------------------------------------------------------------- admission
     def _admit(self, lane: Lane, req: SolveRequest, now: float) -> bool:
-        slot = lane.free_slot()
-        if slot is None:
-            return False
+        device_loop = isinstance(lane, LoopLane)
+        if device_loop:
+            # the device queue holds up to one lane's worth of requests waiting for a slot
+            slot = None
+            if lane.outstanding() >= min(2 * lane.C, lane.QUEUE_CAPACITY):
+                return False
+        else:
+            slot = lane.free_slot()
+            if slot is None:
+                return False
         plan = lane.plan
-        fp_args = (plan.NX, plan.NZ, lane.E, req.nt, lane.rec_pitch, self.S)
-        fp_full = request_footprint(*fp_args, grad_elems=lane.GE, full_history=True, two_step=self.two_step)
-        hist = self.history == "auto" and self.in_use + fp_full <= self.budget
-        fp = fp_full if hist else request_footprint(*fp_args, grad_elems=lane.GE, two_step=self.two_step)
-        if self.in_use + fp > self.budget:
-            return False
         dev, E, S = self.device, lane.E, self.S
         K = max(1, math.ceil(req.nt / S))
         nrec = int(np.asarray(req.rec_xz).reshape(-1, 2).shape[0])
         if nrec > lane.rec_pitch:
             raise ValueError(f"request {req.rid}: {nrec} receivers exceed the lane pitch {lane.rec_pitch}")
+        # one arena block per request: its full forward history when that fits
+        # (history='auto'), else checkpoints; none free yet -> it waits
+        lay_args = (E, req.nt, lane.rec_pitch, S, lane.chunk, lane.GE, plan.N if device_loop else 0)
+        hist, block = False, None
+        if self.history == "auto":
+            layout, fp = request_layout(*lay_args, full_history=True, two_step=self.two_step)
+            block = self.arena.take(fp)
+            hist = block is not None
+        if block is None:
+            layout, fp = request_layout(*lay_args, full_history=False, two_step=self.two_step)
+            block = self.arena.take(fp)
+            if block is None:
+                return False
         with torch.cuda.stream(lane.stream):
-            z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
-            b = {"w": z(E), "ring": z(lane.chunk + 1, E), "om": z(E), "G": z(lane.GE, dt=torch.float64),
-                 "seis": z(req.nt, lane.rec_pitch), "res": z(req.nt, lane.rec_pitch),
-                 "obs": z(req.nt, lane.rec_pitch, dt=torch.float64), "c2p": z(E),
-                 "misfit": z(1, dt=torch.float64)}
-            if self.two_step:
-                b["w2"], b["om2"] = z(E), z(E)
+            b = {}
+            for name, shape, dt, zero, off in layout:
+                b[name] = self.arena.view(block + off, shape, dt)
+                if zero:
+                    b[name].zero_()
             if hist:
                 # u^0 .. u^nt: slot m + 1 is written before any launch reads
                 # it, so only u^0 = 0 needs clearing
-                b["hist"] = torch.empty((req.nt + 1, E), dtype=torch.float32, device=dev)
                 b["hist"][0].zero_()
-            else:
-                b.update({"u0": z(E), "u1": z(E), "cku": z(K, E), "ckw": z(K, E), "seg": z(S + 1, E)})
             if req.obs is not None:
                 b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float64, device=dev)
             w = source_wavelet(req.nt, req.dt, req.f0)
-            b["wav"] = torch.as_tensor(w, dtype=torch.float32, device=dev)
+            b["wav"].copy_(torch.as_tensor(w, dtype=torch.float32, device=dev))
             base = None if req.base is None else torch.as_tensor(req.base, device=dev)
             _, c2 = plan.bspline_map(torch.as_tensor(req.coef, dtype=torch.float64, device=dev)[None], base,
                                      want_c=False)
             L.check(plan.lib.sem_pad_nodal(plan.h, C.c_void_p(c2.data_ptr()), C.c_void_p(b["c2p"].data_ptr()), 1,
                                            C.c_void_p(lane.stream.cuda_stream)))
             b["c2_flat"] = c2
-        a = _Active(req=req, slot=slot, K=K, phase=L.SLOT_FORWARD_HIST if hist else L.SLOT_FORWARD, k=0, buf=b,
-                    footprint=fp, admitted=now, hist=hist)
         lane.ensure_set_capacity()
-        a.rec_set = plan.add_receivers(req.rec_xz)
-        a.nrec = nrec
-        a.src_node = plan.source_node(*req.src_xz)
+        rec_set = plan.add_receivers(req.rec_xz)
+        src_node = plan.source_node(*req.src_xz)
         L.check(plan.lib.sem_receivers_sync(plan.h))
-        lane.slots[slot] = a
+        if device_loop:
+            lane.push(req, b, hist, fp, now, nrec, rec_set, src_node, block)
+        else:
+            a = _Active(req=req, slot=slot, K=K, phase=L.SLOT_FORWARD_HIST if hist else L.SLOT_FORWARD, k=0, buf=b,
+                        footprint=fp, admitted=now, hist=hist, rec_set=rec_set, nrec=nrec, src_node=src_node,
+                        block=block)
+            lane.slots[slot] = a
         self.in_use += fp
         return True
 
@@ -480,6 +744,31 @@ class SEMServer:
         lane.finishing.append((a, ev, host, (gc, g)))
         lane.slots[a.slot] = None
 
+    def _finish_pushed(self, lane: "LoopLane", rid: int, rnd: int):
+        """Results of a solve the device loop completed in round `rnd`: its
+        projection onto the request's B-spline nodes and the copies to pinned
+        host memory, on the server's finishing stream once that round is done."""
+        a = lane.pushed.pop(rid)
+        plan, b, req = lane.plan, a.buf, a.req
+        if self.finish_stream is None:
+            self.finish_stream = torch.cuda.Stream(device=self.device)
+        fs = self.finish_stream
+        fs.wait_event(lane.round_events[rnd])
+        for k in [k for k in lane.round_events if k < lane.rounds - 64]:
+            del lane.round_events[k]
+        with torch.cuda.stream(fs):
+            nbz, nbx = np.asarray(req.coef).shape
+            g = plan.bspline_project(b["gc"][None], nbz, nbx)[0]
+            host = {"g": torch.empty(g.shape, dtype=g.dtype, pin_memory=True),
+                    "seis": torch.empty((req.nt, a.nrec), dtype=b["seis"].dtype, pin_memory=True),
+                    "misfit": torch.empty(1, dtype=torch.float64, pin_memory=True)}
+            host["g"].copy_(g, non_blocking=True)
+            host["seis"].copy_(b["seis"][:, :a.nrec], non_blocking=True)
+            host["misfit"].copy_(b["misfit"], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(fs)
+        lane.finishing.append((a, ev, host, (b, g)))
+
     def _collect(self, lane: Lane) -> bool:
         """Results of finished solves whose copies have landed; their memory
         returns to the budget."""
@@ -495,6 +784,7 @@ class SEMServer:
                                                 host["seis"].numpy().copy(), req.arrival, a.admitted, self.clock())
             lane.finishing.remove(item)
             self.in_use -= a.footprint
+            self.arena.give(a.block, a.footprint)
             got = True
         return got
 
@@ -520,21 +810,27 @@ class SEMServer:
                 while lane.inflight and lane.inflight[0].query():
                     lane.inflight.popleft()
                     progressed = True
+                device_loop = isinstance(lane, LoopLane)
+                if device_loop:
+                    for rid, rnd in lane.poll():
+                        self._finish_pushed(lane, rid, rnd)
+                        progressed = True
                 progressed |= self._collect(lane)
                 # keep up to ROUNDS_AHEAD rounds enqueued: admit, enqueue the
-                # next round, then the work that follows it
+                # next round (host loop: then the work that follows it)
                 while len(lane.inflight) < ROUNDS_AHEAD:
                     while lane.queue and self._admit(lane, lane.queue[0], self.clock()):
                         lane.queue.popleft()
-                    if not lane.occupied():
+                    if not (lane.outstanding() if device_loop else lane.occupied()):
                         break
                     lane.launch_round(self.use_graph)
-                    self._after_enqueue(lane)
+                    if not device_loop:
+                        self._after_enqueue(lane)
                     progressed = True
             if progress and self.clock() - last_report > 5.0:
                 last_report = self.clock()
                 progress(f"[serve] t={last_report:.1f}s done {len(self.results)}/{total} "
-                         f"resident {sum(len(l.occupied()) for l in self.lanes.values())} "
+                         f"in flight {sum(l.outstanding() for l in self.lanes.values())} "
                          f"hbm {self.in_use / 1e9:.1f} GB")
             if not progressed:
                 time.sleep(0.0005)

@@ -79,7 +79,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"sem_desc": L.SemDesc, "sem_batch": L.SemBatch, "sem_slot": L.SemSlot,
-               "sem_curve_desc": L.SemCurveDesc}
+               "sem_curve_desc": L.SemCurveDesc, "sem_request": L.SemRequest}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
@@ -120,4 +120,20 @@ def test_schedule_and_streaming_entry_points_validate_arguments():
 def test_noalloc_projection_validates_before_device_work():
     lib = L.lib()
     assert lib.sem_bspline_project_noalloc(None, 1, 8, 8, None, None, None) == -1
+    assert "bad argument" in lib.sem_last_error().decode()
+
+
+def test_loop_entry_points_validate_before_device_work():
+    """The device-resident loop's entry points reject null handles and bad
+    sizes before touching a device."""
+    lib = L.lib()
+    h = C.c_void_p()
+    assert lib.sem_loop_create(None, 4, 100, 0, 64, 16, C.byref(h)) == -1
+    assert lib.sem_loop_push(None, None, None) == -1
+    assert lib.sem_loop_round(None, None) == -1
+    tot = C.c_longlong()
+    assert lib.sem_loop_poll(None, 0, None, None, 0, C.byref(tot)) == -1
+    assert lib.sem_loop_counts(None, None, None, None, None) == -1
+    assert lib.sem_loop_chunk(None, None) == -1
+    assert lib.sem_loop_destroy(None) == 0
     assert "bad argument" in lib.sem_last_error().decode()

@@ -46,3 +46,56 @@ def test_segment_launch_parity(nt):
             flip = not flip
     assert flip == (launches % 2 == 1)
     assert launches == sum((min(S, nt - k * S) + 1) // 2 for k in range(math.ceil(nt / S)))
+
+
+def test_request_layout_is_aligned_disjoint_and_covers_the_footprint():
+    """Every buffer of a request block starts on the alignment, none overlaps
+    the next, and the block holds at least the footprint's bytes."""
+    from mun_amd.serving import ALIGN, request_layout
+    E, nt, pitch, S = 161 * 192 + 17, 1999, 37, 100
+    for full in (False, True):
+        for two in (False, True):
+            lay, size = request_layout(E, nt, pitch, S, corr_chunk(S), E + 9 * 40 * 41, node_elems=161 * 161,
+                                       full_history=full, two_step=two)
+            end = 0
+            for name, shape, dt, zero, off in lay:
+                assert off % ALIGN == 0 and off >= end, name
+                end = off + math.prod(shape) * (4 if str(dt).endswith("32") else 8)
+            assert end <= size < end + ALIGN
+            fp = request_footprint(161, 161, E, nt, pitch, S, grad_elems=E + 9 * 40 * 41, full_history=full,
+                                   two_step=two)
+            assert fp + 161 * 161 * 8 <= size <= fp + 161 * 161 * 8 + ALIGN * len(lay)
+            names = [x[0] for x in lay]
+            assert ("hist" in names) == full and ("seg" in names) != full and ("w2" in names) == two
+
+
+def test_block_allocator_first_fit_merges_and_rejects_double_release():
+    from mun_amd.serving import ALIGN, BlockAllocator
+    a = BlockAllocator(10 * ALIGN + 5)
+    assert a.size == 10 * ALIGN
+    x = a.alloc(3 * ALIGN)
+    y = a.alloc(2 * ALIGN + 1)          # rounds up to 3 blocks
+    z = a.alloc(4 * ALIGN)
+    assert (x, y, z) == (0, 3 * ALIGN, 6 * ALIGN) and a.alloc(1) is None
+    a.release(y, 2 * ALIGN + 1)
+    assert a.alloc(4 * ALIGN) is None    # the hole is 3 blocks
+    assert a.alloc(2 * ALIGN) == 3 * ALIGN
+    assert a.free == [(5 * ALIGN, ALIGN)]
+    a.release(x, 3 * ALIGN)
+    a.release(z, 4 * ALIGN)              # merges with the one-block tail of the hole
+    assert a.free == [(0, 3 * ALIGN), (5 * ALIGN, 5 * ALIGN)]
+    assert a.alloc(6 * ALIGN) is None
+    a.release(3 * ALIGN, 2 * ALIGN)      # fills the gap: everything merges
+    assert a.free == [(0, 10 * ALIGN)] and a.free_bytes() == a.size
+    with pytest.raises(ValueError):
+        a.release(0, ALIGN)
+
+
+def test_serving_correlation_chunk_rule():
+    """Chunks of at most 20 steps, even unless a chunk is the whole (odd)
+    segment, and the last chunk of a segment never longer than the others."""
+    for S in range(1, 300):
+        c = corr_chunk(S)
+        assert 1 <= c <= min(S, 20)
+        assert c % 2 == 0 or c == S
+        assert S - (S // c) * c <= c

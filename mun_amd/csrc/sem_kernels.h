@@ -54,7 +54,8 @@ constexpr size_t LLC_BYTES = size_t(256) << 20;
 // MODE_ADJS: adjoint step; its gradient correlation runs later per chunk of
 // steps in corr_kernel.  MODE_ANY (slot launches only): each workgroup takes
 // the forward or the adjoint step by its slot's phase, so one launch advances
-// slots of both directions.
+// slots of both directions; with a fixed MODE the per-workgroup direction
+// flags below are compile-time constants and fold away.
 enum StepMode { MODE_FWD = 0, MODE_REC = 2, MODE_ADJS = 3, MODE_ANY = 4 };
 
 // Global-memory qualified pointers: lets the compiler emit global_* instead of
@@ -434,23 +435,17 @@ __device__ __forceinline__ void element_action(const ElemConst<T, P>& ec, const 
 // more memory latency; the forward step and the FP64 instances keep the
 // compiler's choice (103 VGPRs, 4 waves; FP64 about 200), since at 5 waves
 // they spill.
-// LDS of step_kernel: each element's published edge contributions.
-template <typename T, int P, int EZ>
-struct StepLds {
-    T xr_col[EZ][EX][P + 1];  // element's i = P column of contributions
-    T xr_row[EZ][EX][P];      // element's j = P row (i < P)
-};
-
-// One step of the workgroup's (tile, solve); damp: apply the sponge.
 template <typename T, int P, int MODE, int EZ, bool NTS>
-__device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int damp, StepLds<T, P, EZ>& lds) {
+__global__ __launch_bounds__(tile_threads(EZ))
+__attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
+void step_kernel(const StepArgs<T, P> a) {
 #pragma clang fp contract(off)
     constexpr int TZ = EZ - 1;  // owned element rows per tile
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
     constexpr int NP = P + 1;
-    constexpr bool ADJ = (MODE == MODE_ADJS);
-    auto& xr_col = lds.xr_col;
-    auto& xr_row = lds.xr_row;
+    constexpr bool ANY = (MODE == MODE_ANY);
+    __shared__ T xr_col[EZ][EX][NP];   // element's i = P column of contributions
+    __shared__ T xr_row[EZ][EX][P];    // element's j = P row (i < P)
 
     // XCD-contiguous numbering of (tile, solve) pairs
     const int wgid = blockIdx.x;
@@ -468,8 +463,14 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
     const bool owned = valid && exl >= 1 && ezl >= 1;
     const int pitch = a.pitch;
     const int base = (P * ez) * pitch + P * ex;  // offset of node (P ez, P ex) within a solve
+    // direction of this workgroup's step (uniform per workgroup)
+    const bool adj = ANY ? a.slots[b].phase == SEM_SLOT_ADJOINT : MODE == MODE_ADJS;
+    const bool fwd = ANY ? !adj : MODE == MODE_FWD;
+    const int damp = (ANY && adj) ? a.damp_adj : a.damp;
     View<T> V;
-    if (!make_view<T, P, MODE>(a, b, V)) return;  // uniform per workgroup
+    if (!(ANY ? (adj ? make_view<T, P, MODE_ADJS>(a, b, V) : make_view<T, P, MODE_FWD>(a, b, V))
+              : make_view<T, P, MODE>(a, b, V)))
+        return;
 
     if constexpr (MODE == MODE_REC) {
         // seismogram samples only: touch the receiver elements alone
@@ -539,7 +540,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
             }
         }
 
-        if constexpr (MODE == MODE_FWD) {
+        if (fwd) {
             // seismogram sample of the field being read (R u at this time level)
             if (V.rec_t >= 0 && owned) {
                 const int set = V.set;
@@ -561,7 +562,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
                 }
             }
         }
-        if constexpr (ADJ) {
+        if (adj) {
             // adjoint source R^T res, injected into the receiver element's slots
             // (entries name in-domain elements only, so lanes outside never match)
             {
@@ -603,7 +604,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
         // point source: at most one element per solve owns the source node
         int s_loc = -1;
         T sterm = T(0);
-        if constexpr (MODE == MODE_FWD) {
+        if (fwd) {
             const int src = V.src;
             const int sz = src / a.NX, sx = src - sz * a.NX;
             const int lj = sz - P * ez, li = sx - P * ex;
@@ -629,7 +630,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
         };
         auto update = [&](int j, int i, T wold) -> T {  // returns w^{n+1}; leaves u^{n+1} in r[j][i]
             T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
-            if constexpr (MODE == MODE_FWD) {
+            if (fwd) {
                 if (j * NP + i == s_loc) ws += sterm;
             }
             const T Dn = damp_at(j, i);
@@ -653,7 +654,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
                 store4(at(V.w, o), wn);
                 store4(at(V.u_out, o), un);
             }
-            if constexpr (MODE == MODE_FWD) {
+            if (fwd) {
                 if (V.ck_u) {
                     store4(at(V.ck_w, o), wn);
                     store4(at(V.ck_u, o), un);
@@ -671,7 +672,7 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
                     const T wn = update(j, i, wp[o]);
                     wp[o] = wn;
                     uo[o] = r[j][i];
-                    if constexpr (MODE == MODE_FWD) {
+                    if (fwd) {
                         if (V.ck_u) {
                             V.ck_w[base + o] = wn;
                             V.ck_u[base + o] = r[j][i];
@@ -679,24 +680,6 @@ __device__ __forceinline__ void step_body(const StepArgs<T, P>& a, const int dam
                     }
                 }
         }
-    }
-}
-
-template <typename T, int P, int MODE, int EZ, bool NTS>
-__global__ __launch_bounds__(tile_threads(EZ))
-__attribute__((amdgpu_waves_per_eu((MODE == MODE_ADJS && sizeof(T) == 4) ? 5 : 1)))
-void step_kernel(const StepArgs<T, P> a) {
-    __shared__ StepLds<T, P, EZ> lds;
-    if constexpr (MODE == MODE_ANY) {
-        const int wgid = blockIdx.x;
-        const int lin = (wgid % NXCD) * a.nper + wgid / NXCD;
-        if (lin >= a.total) return;  // whole workgroup leaves together
-        if (a.slots[lin / a.ntiles].phase == SEM_SLOT_ADJOINT)
-            step_body<T, P, MODE_ADJS, EZ, NTS>(a, a.damp_adj, lds);
-        else
-            step_body<T, P, MODE_FWD, EZ, NTS>(a, a.damp, lds);
-    } else {
-        step_body<T, P, MODE, EZ, NTS>(a, a.damp, lds);
     }
 }
 
@@ -838,37 +821,22 @@ inline constexpr int TILE2_EX[NTILE2] = {19, 19, 35, 35};
 inline constexpr int TILE2_EZ[NTILE2] = {26, 13, 14, 28};
 __host__ __device__ constexpr int tile2_threads(int ex, int ez) { return (ex * ez + 63) / 64 * 64; }
 
-// LDS of step2_kernel: edge contributions, the level-(n+1) nodes of the ring
-// and O, and the increment of O's nodes.
-template <typename T, int P, int EX2, int EZ2>
-struct Step2Lds {
-    static constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;
-    static constexpr int U1_H = 4 * (EZ2 - 1) + 1;
-    static constexpr int W1_W = (4 * (EX2 - 3) + 1 + 3) / 4 * 4;
-    static constexpr int W1_H = 4 * (EZ2 - 3) + 1;
-    T xr_col[EZ2][EX2][P + 1];
-    T xr_row[EZ2][EX2][P];
-    __attribute__((aligned(16))) T U1[U1_H][U1_W];
-    __attribute__((aligned(16))) T W1[W1_H][W1_W];
-};
-
-// Two steps of the workgroup's (tile, solve); damp: apply the sponge.
 template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
-__device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int damp, Step2Lds<T, P, EX2, EZ2>& lds) {
+__global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const StepArgs<T, P> a) {
 #pragma clang fp contract(off)
     static_assert(P == 4, "vector loads assume 5 nodes per element edge");
-    static_assert(MODE == MODE_FWD || MODE == MODE_ADJS, "forward or adjoint steps");
+    static_assert(MODE == MODE_FWD || MODE == MODE_ADJS || MODE == MODE_ANY, "forward or adjoint steps");
     constexpr int TX2 = EX2 - 3, TZ2 = EZ2 - 3;
-    constexpr int U1_W = Step2Lds<T, P, EX2, EZ2>::U1_W;  // level-(n+1) nodes of the ring + O
-    constexpr int U1_H = Step2Lds<T, P, EX2, EZ2>::U1_H;
-    constexpr int W1_W = Step2Lds<T, P, EX2, EZ2>::W1_W;  // increment of O's nodes
-    constexpr int W1_H = Step2Lds<T, P, EX2, EZ2>::W1_H;
+    constexpr int U1_W = (4 * (EX2 - 1) + 1 + 3) / 4 * 4;  // level-(n+1) nodes of the ring + O
+    constexpr int U1_H = 4 * (EZ2 - 1) + 1;
+    constexpr int W1_W = (4 * TX2 + 1 + 3) / 4 * 4;        // increment of O's nodes
+    constexpr int W1_H = 4 * TZ2 + 1;
     constexpr int NP = P + 1;
-    constexpr bool ADJ = (MODE == MODE_ADJS);
-    auto& xr_col = lds.xr_col;
-    auto& xr_row = lds.xr_row;
-    auto& U1 = lds.U1;
-    auto& W1 = lds.W1;
+    constexpr bool ANY = (MODE == MODE_ANY);
+    __shared__ T xr_col[EZ2][EX2][NP];
+    __shared__ T xr_row[EZ2][EX2][P];
+    __shared__ __attribute__((aligned(16))) T U1[U1_H][U1_W];
+    __shared__ __attribute__((aligned(16))) T W1[W1_H][W1_W];
 
     const int wgid = blockIdx.x;
     const int lin = (wgid % NXCD) * a.nper2 + wgid / NXCD;
@@ -891,8 +859,14 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
     const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
 
     // solve b's arrays and time indices (uniform per workgroup)
+    // direction of this workgroup's steps (uniform per workgroup)
+    const bool adj = ANY ? a.slots[b].phase == SEM_SLOT_ADJOINT : MODE == MODE_ADJS;
+    const bool fwd = ANY ? !adj : MODE == MODE_FWD;
+    const int damp = (ANY && adj) ? a.damp_adj : a.damp;
     View2<T> V;
-    if (!make_view2<T, P, MODE>(a, b, V)) return;
+    if (!(ANY ? (adj ? make_view2<T, P, MODE_ADJS>(a, b, V) : make_view2<T, P, MODE_FWD>(a, b, V))
+              : make_view2<T, P, MODE>(a, b, V)))
+        return;
     const GMEM T* __restrict__ u_in = V.u_in;
     GMEM T* __restrict__ u_out = V.u_out;
     GMEM T* __restrict__ u_mid = V.u_mid;
@@ -1003,7 +977,7 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
 #pragma unroll
                 for (int i = 0; i < NP; ++i) r[j][i] = T(0);
         }
-        if constexpr (MODE == MODE_FWD) {
+        if (fwd) {
             if (own) sample(u, V.rec_t);
         }
         if (n1) {
@@ -1018,7 +992,7 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
                 load4(gm(a.Dz) + P * ez, dzv);
             }
         }
-        if constexpr (ADJ) {
+        if (adj) {
             if (valid) inject(r, V.res_t);
         }
         publish(r);
@@ -1029,7 +1003,7 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
             const T sterm = (s_loc >= 0) ? a.ec.dt2 * wav[0] : T(0);
             auto update = [&](int j, int i, T wold, T uold) -> T {  // w^{n+1}; u^{n+1} left in r[j][i]
                 T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
-                if constexpr (MODE == MODE_FWD) {
+                if (fwd) {
                     if (j * NP + i == s_loc) ws += sterm;
                 }
                 const T Dn = damp_at(j, i);
@@ -1102,10 +1076,10 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
 #pragma unroll
                 for (int i = 0; i < NP; ++i) r[j][i] = T(0);
         }
-        if constexpr (MODE == MODE_FWD) {
+        if (fwd) {
             if (own) sample(u, V.rec_t + 1);
         }
-        if constexpr (ADJ) {
+        if (adj) {
             if (r2) inject(r, V.res_t - 1);
         }
         publish(r);
@@ -1117,7 +1091,7 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
         const int X0w = P * (exl - 2), Z0w = P * (ezl - 2);
         auto update = [&](int j, int i, T wold, T uold) -> T {
             T ws = fmadd(-((a.ec.dt2 * inv_wz(j)) * inv_wx(i)), r[j][i], wold);  // w^n + a^n
-            if constexpr (MODE == MODE_FWD) {
+            if (fwd) {
                 if (j * NP + i == s_loc) ws += sterm;
             }
             const T Dn = damp_at(j, i);
@@ -1169,22 +1143,6 @@ __device__ __forceinline__ void step2_body(const StepArgs<T, P>& a, const int da
                     }
                 }
         }
-    }
-}
-
-template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
-__global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const StepArgs<T, P> a) {
-    __shared__ Step2Lds<T, P, EX2, EZ2> lds;
-    if constexpr (MODE == MODE_ANY) {
-        const int wgid = blockIdx.x;
-        const int lin = (wgid % NXCD) * a.nper2 + wgid / NXCD;
-        if (lin >= a.total2) return;  // whole workgroup leaves together
-        if (a.slots[lin / a.ntiles2].phase == SEM_SLOT_ADJOINT)
-            step2_body<T, P, MODE_ADJS, NTS, EX2, EZ2>(a, a.damp_adj, lds);
-        else
-            step2_body<T, P, MODE_FWD, NTS, EX2, EZ2>(a, a.damp, lds);
-    } else {
-        step2_body<T, P, MODE, NTS, EX2, EZ2>(a, a.damp, lds);
     }
 }
 

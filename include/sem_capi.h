@@ -173,7 +173,8 @@ int sem_plan_get_step_fusion(const sem_plan* plan, int* steps);
 
 /* Tile shape of the two-step kernel: shape 0 (default) 19 x 26 element
  * threads per workgroup (16 x 23 owned), 1: 19 x 13 (16 x 10), 2: 35 x 14
- * (32 x 11), 3: 35 x 28 (32 x 25; FP32 plans only).  Results are bitwise the
+ * (32 x 11), 3: 35 x 28 (32 x 25; FP32 plans only), 4: 23 x 11 (20 x 8).
+ * Results are bitwise the
  * same for every shape.  The two-step receiver table is rebuilt on the next
  * use: set the shape before sem_receivers_reserve and not while solves of the
  * plan are in flight.  No reference counterpart (chosen by the autotuner). */
@@ -186,7 +187,7 @@ int sem_plan_get_step_fusion(const sem_plan* plan, int* steps);
 int sem_plan_set_corr_chunk(sem_plan* plan, int steps);
 int sem_plan_get_corr_chunk(const sem_plan* plan, int* steps);
 
-#define SEM_TILE2_SHAPES 4
+#define SEM_TILE2_SHAPES 5
 int sem_plan_set_tile2(sem_plan* plan, int shape);
 int sem_plan_get_tile2(const sem_plan* plan, int* shape, int* ex, int* ez);
 

@@ -153,7 +153,7 @@ def best_tile_rows(nelem_x: int, nelem_z: int, batch: int, precision: str = "fp3
 
 
 # ---------------------------------------------------------------- batched path
-TILE2_SHAPES = {"fp32": (0, 1, 2, 3), "fp64": (0, 1, 2)}
+TILE2_SHAPES = {"fp32": (0, 1, 2, 3, 4), "fp64": (0, 1, 2, 4)}
 
 
 def _probe_plan(plan, steps: int, nrec: int):

@@ -813,12 +813,13 @@ __device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View
 // Two-step tile shapes (sem_plan_set_tile2): EX x EZ element threads per
 // workgroup, the owned block (EX - 3) x (EZ - 3) with two halo columns / rows
 // on the left / below and one on the right / above.  Owned widths of 16 and
-// 32 elements keep owned FP32 rows 256- or 512-byte aligned.  Shape 0 is the
-// default; shape 3 needs 150 KB of LDS in FP32 (one workgroup per CU) and has
-// no FP64 instance (it would exceed the CU's 160 KB).
-constexpr int NTILE2 = 4;
-inline constexpr int TILE2_EX[NTILE2] = {19, 19, 35, 35};
-inline constexpr int TILE2_EZ[NTILE2] = {26, 13, 14, 28};
+// 32 elements keep owned FP32 rows 256- or 512-byte aligned; 20 elements
+// (shape 4) divide the 40-, 120- and 200-element meshes' rows evenly.  Shape
+// 0 is the default; shape 3 needs 150 KB of LDS in FP32 (one workgroup per CU)
+// and has no FP64 instance (it would exceed the CU's 160 KB).
+constexpr int NTILE2 = 5;
+inline constexpr int TILE2_EX[NTILE2] = {19, 19, 35, 35, 23};
+inline constexpr int TILE2_EZ[NTILE2] = {26, 13, 14, 28, 11};
 __host__ __device__ constexpr int tile2_threads(int ex, int ez) { return (ex * ez + 63) / 64 * 64; }
 
 template <typename T, int P, int MODE, bool NTS, int EX2, int EZ2>
@@ -845,18 +846,21 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
     const long b = lin / a.ntiles2;
 
     const int t = threadIdx.x;
-    const int exl = t % EX2, ezl = t / EX2;
+    // This thread's element.  Step one maps thread t to element
+    // (t % EX2, t / EX2) of the whole block; step two re-maps the threads onto
+    // its smaller block and reassigns these variables (the lambdas below read
+    // them by reference, so they see the step's element).
+    int exl = t % EX2, ezl = t / EX2;
     const int tx = tile % a.ntx2, tz = tile / a.ntx2;
-    const int ex = tx * TX2 - 2 + exl;
-    const int ez = tz * TZ2 - 2 + ezl;
-    const bool inb = t < EX2 * EZ2;
-    const bool valid = inb && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
-    const bool n1 = valid && exl >= 1 && ezl >= 1;                                      // updates level n+1
-    const bool r2 = valid && exl >= 1 && exl <= EX2 - 2 && ezl >= 1 && ezl <= EZ2 - 2;  // step-two element
-    const bool own = valid && exl >= 2 && exl <= EX2 - 2 && ezl >= 2 && ezl <= EZ2 - 2; // updates level n+2
+    int ex = tx * TX2 - 2 + exl;
+    int ez = tz * TZ2 - 2 + ezl;
+    bool inb = t < EX2 * EZ2;
+    bool valid = inb && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
+    const bool n1 = valid && exl >= 1 && ezl >= 1;                                // updates level n+1
+    bool own = valid && exl >= 2 && exl <= EX2 - 2 && ezl >= 2 && ezl <= EZ2 - 2; // updates level n+2
     const int pitch = a.pitch;
-    const int base = (P * ez) * pitch + P * ex;
-    const bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
+    int base = (P * ez) * pitch + P * ex;
+    bool last_x = (ex == a.nex - 1), last_z = (ez == a.nez - 1);
 
     // solve b's arrays and time indices (uniform per workgroup)
     // direction of this workgroup's steps (uniform per workgroup)
@@ -886,10 +890,10 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
 
     // every lane reads an in-range block (clamped) so that the cross-lane
     // fifth-node reads see their neighbours' data
-    const int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
-    const int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
-    const int base_c = (P * ezc) * pitch + P * exc;
-    const bool from_right = exl < EX2 - 1 && (t & 63) != 63 && ex + 1 < a.nex;
+    int exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
+    int ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+    int base_c = (P * ezc) * pitch + P * exc;
+    bool from_right = exl < EX2 - 1 && (t & 63) != 63 && ex + 1 < a.nex;
 
     auto inv_wx = [&](int i) -> T { return (i == 0 && ex > 0) ? a.ec.iw_shared : a.ec.iw[i]; };
     auto inv_wz = [&](int j) -> T { return (j == 0 && ez > 0) ? a.ec.iw_shared : a.ec.iw[j]; };
@@ -1054,35 +1058,62 @@ __global__ __launch_bounds__(tile2_threads(EX2, EZ2)) void step2_kernel(const St
     if (V.single) return;
 
     // ---- step two: level n+1 -> n+2 on O and its left / lower ring ----
+    // The threads are re-mapped onto the step-two block of (EX2 - 2) x
+    // (EZ2 - 2) elements, element (xl, zl) in thread xl + (EX2 - 2) zl, so that
+    // the threads past it form whole waves that skip the element math (one of
+    // the four waves of the 19 x 13 tile).  The lanes past the block in the
+    // last partly used wave take a clamped in-range block and publish nothing.
     {
+        constexpr int XW = EX2 - 2, ZW = EZ2 - 2, NS2 = XW * ZW;
+        const bool in2 = t < NS2;
+        const int xl = t % XW, zl = in2 ? t / XW : ZW - 1;
+        exl = xl + 1;
+        ezl = zl + 1;
+        ex = tx * TX2 - 1 + xl;
+        ez = tz * TZ2 - 1 + zl;
+        inb = in2;
+        valid = in2 && ex >= 0 && ex < a.nex && ez >= 0 && ez < a.nez;
+        own = valid && xl >= 1 && zl >= 1;
+        base = (P * ez) * pitch + P * ex;
+        last_x = (ex == a.nex - 1);
+        last_z = (ez == a.nez - 1);
+        exc = ex < 0 ? 0 : (ex >= a.nex ? a.nex - 1 : ex);
+        ezc = ez < 0 ? 0 : (ez >= a.nez ? a.nez - 1 : ez);
+        base_c = (P * ezc) * pitch + P * exc;
+        from_right = xl < XW - 1 && (t & 63) != 63 && ex + 1 < a.nex;
+        const bool wave_on = (t & ~63) < NS2;  // uniform per wave
+
         T u[NP][NP], r[NP][NP];
-        {
-            // u^{n+1} block from LDS (lanes outside the step-two block read a
-            // clamped in-range block and contribute zero)
-            const int xl = exl < 1 ? 0 : (exl > EX2 - 2 ? EX2 - 2 : exl - 1);
-            const int zl = ezl < 1 ? 0 : (ezl > EZ2 - 2 ? EZ2 - 2 : ezl - 1);
+        if (wave_on) {
+            // u^{n+1} block from LDS, c^2 again from memory (L2)
 #pragma unroll
             for (int j = 0; j < NP; ++j)
 #pragma unroll
                 for (int i = 0; i < NP; ++i) u[j][i] = U1[P * zl + j][P * xl + i];
-            T c2[NP][NP];
-            fetch_block(c2p, (unsigned)base_c, (unsigned)pitch, valid && !from_right, c2);
-            finish_block(from_right, c2);
-            element_action<T, P>(a.ec, u, c2, r);
-        }
-        if (!r2) {
+            {
+                T c2[NP][NP];
+                fetch_block(c2p, (unsigned)base_c, (unsigned)pitch, valid && !from_right, c2);
+                finish_block(from_right, c2);
+                element_action<T, P>(a.ec, u, c2, r);
+            }
+            if (!valid) {
 #pragma unroll
-            for (int j = 0; j < NP; ++j)
+                for (int j = 0; j < NP; ++j)
 #pragma unroll
-                for (int i = 0; i < NP; ++i) r[j][i] = T(0);
+                    for (int i = 0; i < NP; ++i) r[j][i] = T(0);
+            }
+            if (fwd) {
+                if (own) sample(u, V.rec_t + 1);
+            }
+            if (own && damp) {  // sponge factors of this step's element
+                load4(gm(a.Dx) + P * ex, dxv);
+                load4(gm(a.Dz) + P * ez, dzv);
+            }
+            if (adj) {
+                if (valid) inject(r, V.res_t - 1);
+            }
+            publish(r);
         }
-        if (fwd) {
-            if (own) sample(u, V.rec_t + 1);
-        }
-        if (adj) {
-            if (r2) inject(r, V.res_t - 1);
-        }
-        publish(r);
         __syncthreads();
         if (!own) return;
         assemble(r);

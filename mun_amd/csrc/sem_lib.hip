@@ -630,6 +630,7 @@ int launch_step2_nt(const sem_plan* pl, const StepArgs<T, 4>& a, hipStream_t st)
             } else {
                 return fail(SEM_E_UNSUPPORTED, "two-step tile shape 3 has no FP64 instance");
             }
+        case 4: launch_step2_shape<T, MODE, NTS, 4>(a, grid, st); break;
         default: return fail(SEM_E_UNSUPPORTED, "two-step tile shape without a kernel instance");
     }
     LAUNCH_CHECK();

@@ -279,6 +279,10 @@ class Plan:
         L.check(self.lib.sem_plan_get_schedule(self.h, C.byref(ck), C.byref(g), C.byref(ln)))
         return {"checkpoint_every": ck.value, "group": g.value, "lanes": ln.value}
 
+    def release_workspace(self) -> None:
+        """Drop the cached device workspace (the next solve allocates it again)."""
+        self._ws = None
+
     def memory_budget(self, reserve_frac: float = 0.01, reserve_min: int = 3 << 30) -> int:
         """Device bytes a workspace may take: free memory plus what this plan's
         workspace and torch's cache already hold, less a reserve."""

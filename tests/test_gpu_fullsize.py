@@ -108,6 +108,9 @@ def test_batch200_model0_in_the_benchmark_schedule_vs_oracle():
     assert s.plan.get_step_fusion() == 2
     print("[batch200 m0] FP32, 18 models, 3 lanes x 6, full history", flush=True)
     ll, gn, seis = s.loglik_grad(torch.as_tensor(wl.coef, device="cuda"), obs, wl.noise_std)
+    # the 18 full histories take 272 GB: free them before the single solve
+    s.plan.release_workspace()
+    torch.cuda.empty_cache()
     # model 0's nodal gradient and misfit: the same solve alone (bits do not depend on the batch)
     s1 = _solver(wl, "fp32")
     s1.plan.set_schedule(0, 0, 1)

@@ -413,6 +413,19 @@ def main():
         out["plain_solves_per_s"] = round(1.0 / tp1, 4)
         out["batched_vs_plain"] = round(value * tp1, 3)
         log(f"[bench] plain path: {tp1:.2f} s per solve")
+        # the same plain path with one launch per time step (the one-step kernel)
+        fusion0 = plan.get_step_fusion()
+        plan.set_step_fusion(1)
+        plain.loglik_grad(coef[:1], obs, noise)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain.loglik_grad(coef[:1], obs, noise)
+        torch.cuda.synchronize()
+        tp1s = time.perf_counter() - t0
+        plan.set_step_fusion(fusion0)
+        out["plain_one_step_per_launch"] = {"solves_per_s": round(1.0 / tp1s, 4),
+                                            "batched_vs_plain": round(value * tp1s, 3)}
+        log(f"[bench] plain path, one step per launch: {tp1s:.2f} s per solve")
     del solver, plan, coef, obs, c2, ll, grad
     if not args.no_plain:
         del plain

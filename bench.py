@@ -378,9 +378,10 @@ def main():
                                            f"{ws_mb:.0f} MiB) fits the 256 MiB last-level cache, so much of "
                                            f"this launch's traffic is served there, not from HBM"),
                         "limiter": ("VALU work and memory latency, not bandwidth: the two-step kernel keeps "
-                                    "u^(n+1) in LDS, and its counters show the VALU about 60 % busy "
-                                    "(profiles/step_kernels_batch200_sq_ta_counters.md)" if fusion == 2
-                                    else "HBM / last-level cache bandwidth"),
+                                    "u^(n+1) in LDS; its counters show the VALU carrying 71 % of the launch's "
+                                    "cycles and waves waiting 43 % of their lifetime at four LDS-bound workgroups "
+                                    "per compute unit (profiles/step_kernels_batch200_sq_ta_counters.md)"
+                                    if fusion == 2 else "HBM / last-level cache bandwidth"),
                         "hbm_resident": {"solves_per_launch": n_hbm, "streams": "non-temporal",
                                          "launch_us": round(t_hbm * 1e6, 1),
                                          "achieved": gbs(STEP_BYTES_PER_NODE * fusion * m.N * n_hbm, t_hbm),

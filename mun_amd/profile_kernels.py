@@ -3,6 +3,7 @@
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15] [--tile2 1]
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
     python -m mun_amd.profile_kernels --summarize-trace KERNEL_TRACE_CSV [--top 30]
+    python -m mun_amd.profile_kernels --summarize-counters SQ1_CSV SQ2_CSV SQ3_CSV
 
 Order of GPU work (so counter runs can attribute dispatches):
   1. a plain device-to-device copy of --copy-mb MiB, three times
@@ -129,6 +130,55 @@ def summarize_trace(trace_csv, top=30):
     return "\n".join(lines)
 
 
+def summarize_counters(csvs, kernels=("step2_kernel", "corr_kernel")):
+    """Markdown table of SQ / TA / TCC counter passes of this driver
+    (rocprofv3 --pmc ... --output-format csv, one CSV per pass): per (kernel,
+    grid) of the named kernels the medians over dispatches, as instructions
+    per wave (SQ_INSTS_*), fractions of wave cycles (SQ_WAIT_*, SQ_ACTIVE_*),
+    the texture addresser's stalled fraction and the L2 hit fraction."""
+    import collections
+    import csv
+    import statistics
+
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in csvs:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            if any(k in name for k in kernels):
+                vals[(name, int(r["Grid_Size"]))][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    rows = [("dispatches", None, None), ("SQ_INSTS_VALU / wave", "SQ_INSTS_VALU", "SQ_WAVES"),
+            ("SQ_INSTS_VMEM_RD / wave", "SQ_INSTS_VMEM_RD", "SQ_WAVES"),
+            ("SQ_INSTS_VMEM_WR / wave", "SQ_INSTS_VMEM_WR", "SQ_WAVES"),
+            ("SQ_INSTS_LDS / wave", "SQ_INSTS_LDS", "SQ_WAVES"),
+            ("SQ_WAIT_ANY / SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+            ("SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+            ("SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+            ("TA_ADDR_STALLED_BY_TC / TA_TA_BUSY", "TA_ADDR_STALLED_BY_TC_CYCLES", "TA_TA_BUSY"),
+            ("TCC hit fraction", "TCC_HIT", None), ("waves", "SQ_WAVES", 1), ("TA_TA_BUSY", "TA_TA_BUSY", 1)]
+    keys = sorted(vals, key=lambda k: (k[0], -k[1]))
+    lines = ["| counter | " + " | ".join(f"`{n}` grid {g}" for n, g in keys) + " |",
+             "|---|" + "---|" * len(keys)]
+    for label, num, den in rows:
+        cells = []
+        for k in keys:
+            med = {c: statistics.median(v) for c, v in vals[k].items()}
+            if num is None:
+                cells.append(str(max(len(v) for v in vals[k].values())))
+            elif num not in med:
+                cells.append("-")
+            elif num == "TCC_HIT":
+                cells.append(f"{med['TCC_HIT'] / (med['TCC_HIT'] + med['TCC_MISS']):.2f}" if "TCC_MISS" in med else "-")
+            elif den == 1:
+                cells.append(f"{med[num]:.4g}")
+            elif den in med:
+                q = med[num] / med[den]
+                cells.append(f"{q:.1f}" if den == "SQ_WAVES" else f"{q:.2f}")
+            else:
+                cells.append("-")
+        lines.append(f"| {label} | " + " | ".join(cells) + " |")
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", type=int, default=256)
@@ -143,7 +193,12 @@ def main(argv=None):
     ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
                     help="per-kernel table of a rocprofv3 kernel trace (no GPU work)")
     ap.add_argument("--top", type=int, default=30, help="rows of --summarize-trace")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="COUNTER_CSV",
+                    help="per-kernel medians of SQ / TA / TCC counter passes of this driver (no GPU work)")
     a = ap.parse_args(argv)
+    if a.summarize_counters:
+        print(summarize_counters(a.summarize_counters))
+        return 0
     if a.summarize_trace:
         print(summarize_trace(a.summarize_trace, a.top))
         return 0

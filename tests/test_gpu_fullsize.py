@@ -92,23 +92,25 @@ def test_single200_fp32_checkpointed_vs_oracle():
 
 
 def test_batch200_model0_in_the_benchmark_schedule_vs_oracle():
-    """Model 0 of the 256-model batch solved inside an 18-model batch under
-    the benchmark's schedule (groups of 6 in 3 lanes, full forward history,
-    two steps per launch) against the oracle's reference-mode solve."""
+    """Model 0 of the 256-model batch solved inside a 16-model batch under
+    the benchmark's schedule (groups of 8 in 2 lanes, full forward history,
+    two steps per launch, the 19 x 13 tile) against the oracle's
+    reference-mode solve."""
     from mun_amd import workloads as W
     fx = fixture("oracle_batch200m0.npz")
-    wl = W.batch200(nmodel=18)
+    wl = W.batch200(nmodel=16)
     # model 0 is the same draw for every batch size; the fixture's observations
     # come from the true model drawn after a batch of one
     wl.true_coef = W.batch200(nmodel=1).true_coef
     print("[batch200 m0] observations (FP64)", flush=True)
     obs = _observations(wl, {0: fx["yobs_rows"]}, int(fx["meta_row_step"]))
     s = _solver(wl, "fp32")
-    s.plan.set_schedule(0, 6, 3)
+    s.plan.set_schedule(0, 8, 2)
+    s.plan.set_tile2(1)
     assert s.plan.get_step_fusion() == 2
-    print("[batch200 m0] FP32, 18 models, 3 lanes x 6, full history", flush=True)
+    print("[batch200 m0] FP32, 16 models, 2 lanes x 8, full history", flush=True)
     ll, gn, seis = s.loglik_grad(torch.as_tensor(wl.coef, device="cuda"), obs, wl.noise_std)
-    # the 18 full histories take 272 GB: free them before the single solve
+    # the 16 full histories take 246 GB: free them before the single solve
     s.plan.release_workspace()
     torch.cuda.empty_cache()
     # model 0's nodal gradient and misfit: the same solve alone (bits do not depend on the batch)

@@ -322,7 +322,8 @@ int sem_curve_project_ex(sem_plan* plan, const sem_curve_desc* cd, int nmodel, c
  * solve per call (core.py:1067) and its drivers loop over samples and sources
  * (CO2_plume/one_stage/single_cubic/RQS_NF.py:1035-1072, 1488-1492). */
 #define SEM_SLOT_IDLE 0
-#define SEM_SLOT_FORWARD 1    /* u0/u1 ping-pong (by step parity), w increment       */
+#define SEM_SLOT_FORWARD 1    /* u0/u1 ping-pong (by step parity, or by step pair:
+                                 pair_parity), w increment                           */
 #define SEM_SLOT_RECOMPUTE 2  /* u0 = checkpoint u of the segment, w = its w, seg    */
 #define SEM_SLOT_ADJOINT 3    /* lambda in the ring, w = omega                        */
 #define SEM_SLOT_FORWARD_HIST 4 /* forward into the full history: u^{n0+m} in seg slot m
@@ -350,7 +351,9 @@ typedef struct {
     double* G;             /* gradient accumulator, sem_plan_gradient_elems() doubles */
     void* ring;            /* adjoint: lambda_t in slot (t+1) % ring_slots, stride seg_stride */
     int ring_slots;        /* >= chunk_len + 1 of sem_slots_correlate                 */
-    int reserved;
+    int pair_parity;       /* forward: 0 = u^n in u1 when n is odd, else in u0 (one step
+                              per launch, sem_slots_step); 1 = u^n in u1 when (n + 1) / 2
+                              is odd, else in u0 (two steps per launch, sem_slots_step2) */
     void* w2;              /* sem_slots_step2: second increment buffer (the increment
                               alternates w -> w2 -> w ... launch by launch)           */
 } sem_slot;
@@ -366,20 +369,26 @@ int sem_plan_gradient_elems(const sem_plan* plan, long* elems);
 /* One step of every slot whose phase is in phase_mask (bit 1 << SEM_SLOT_*):
  * forward/recompute slots run the forward step, adjoint slots the adjoint step
  * (a mask mixing both launches both kernels).  m = step index within the
- * current segments; slots with m >= len skip.  rec_pitch: seismogram row length. */
+ * current segments; slots with m >= len skip.  rec_pitch: seismogram row length.
+ * FORWARD slots with pair_parity set are not advanced (sem_slots_step2 is
+ * theirs). */
 int sem_slots_step(sem_plan* plan, const sem_slot* slots_device, int nslots, int m, unsigned phase_mask,
                    int rec_pitch, void* stream);
-/* Steps m and m + 1 (m even) of every FORWARD_HIST, RECOMPUTE and ADJOINT
- * slot in phase_mask, in one launch per kernel (two-step kernel, the same bits as two
- * sem_slots_step calls); a slot with m + 1 == len takes step m alone.  The
- * increment is read from w and written to w2 on launches with (m / 2) even,
- * the other way round on odd ones, so it ends a segment of len steps in w2
- * when ceil(len / 2) is odd.  Slots of other phases are not advanced (step
- * them with sem_slots_step and a mask without these phases).  No reference
- * counterpart (continuous batching).
- * Conditions on the slot table (a slot that breaks the first two is not
+/* Steps m and m + 1 (m even) of every FORWARD, FORWARD_HIST, RECOMPUTE and
+ * ADJOINT slot in phase_mask, in one launch per kernel (two-step kernel, the
+ * same bits as two sem_slots_step calls); a slot with m + 1 == len takes step
+ * m alone.  The increment is read from w and written to w2 on launches with
+ * (m / 2) even, the other way round on odd ones, so it ends a segment of len
+ * steps in w2 when ceil(len / 2) is odd.  A FORWARD slot reads u^n from the
+ * buffer pair_parity names and writes u^{n+2} (u^{n+1} when it takes a step
+ * alone) to the other one; its checkpoint (ck_u, ck_w) receives the state
+ * after the segment's last pair.  No reference counterpart (continuous
+ * batching).
+ * Conditions on the slot table (a slot that breaks the first three is not
  * advanced at all):
  *   - w2 is non-null and differs from w;
+ *   - FORWARD slots have pair_parity set and an even n0, and an even len
+ *     when ck_u is set;
  *   - ADJOINT slots have ring_slots >= 3 (a pair writes lambda_{t-1} and
  *     lambda_{t-2} while lambda_t is read);
  *   - the chunk_len of the sem_slots_correlate calls between these launches
@@ -481,16 +490,6 @@ int sem_loop_chunk(const sem_loop* loop, int* chunk);
 int sem_loop_push(sem_loop* loop, const sem_request* req, void* stream);
 /* Enqueue one round on `stream`; allocates nothing, capturable. */
 int sem_loop_round(sem_loop* loop, void* stream);
-/* sem_loop_round for a caller that tracks which kinds of request it has in
- * flight: checkpointed = 0 promises that no request pushed with
- * full_history = 0 is queued or in a slot, and leaves out the round's
- * one-step launches for checkpointed forward slots, which a two-step loop
- * otherwise makes at every step even when no slot takes them (a kernel
- * boundary each); checkpointed = 1 is sem_loop_round.  A broken promise
- * leaves such a request's forward steps undone (wrong results, no fault):
- * keep the round with them until the last checkpointed request pushed has
- * been seen complete (sem_loop_poll). */
-int sem_loop_round_ex(sem_loop* loop, int checkpointed, void* stream);
 /* Completion records since index `since` (the list is cumulative): up to `max`
  * ids and the rounds that completed them; *total = completions so far.  Reads
  * host-mapped memory written by the device (no device call).  Records older than

@@ -29,7 +29,7 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_plan_schedule_auto", "sem_plan_set_streaming", "sem_plan_set_step_fusion",
            "sem_plan_get_step_fusion", "sem_slots_step2", "sem_bspline_project_noalloc", "sem_plan_set_tile2",
            "sem_plan_get_tile2", "sem_plan_set_corr_chunk", "sem_plan_get_corr_chunk", "sem_loop_create",
-           "sem_loop_destroy", "sem_loop_chunk", "sem_loop_push", "sem_loop_round", "sem_loop_round_ex", "sem_loop_poll",
+           "sem_loop_destroy", "sem_loop_chunk", "sem_loop_push", "sem_loop_round", "sem_loop_poll",
            "sem_loop_counts", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
@@ -70,7 +70,7 @@ class SemSlot(C.Structure):
                 ("w", C.c_void_p), ("c2", C.c_void_p), ("wavelet", C.c_void_p), ("seis", C.c_void_p),
                 ("ck_u", C.c_void_p), ("ck_w", C.c_void_p), ("seg", C.c_void_p), ("seg_stride", C.c_long),
                 ("resid", C.c_void_p), ("G", C.c_void_p), ("ring", C.c_void_p), ("ring_slots", C.c_int),
-                ("reserved", C.c_int), ("w2", C.c_void_p)]
+                ("pair_parity", C.c_int), ("w2", C.c_void_p)]
 
 
 class SemRequest(C.Structure):
@@ -146,7 +146,6 @@ def lib() -> C.CDLL:
     L.sem_loop_chunk.argtypes = [vp, ip]
     L.sem_loop_push.argtypes = [vp, C.POINTER(SemRequest), vp]
     L.sem_loop_round.argtypes = [vp, vp]
-    L.sem_loop_round_ex.argtypes = [vp, i, vp]
     L.sem_loop_poll.argtypes = [vp, ll, llp, llp, i, llp]
     L.sem_loop_counts.argtypes = [vp, llp, llp, llp, ip]
     for name in EXPORTS:

@@ -242,8 +242,9 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
     v.rec_t = v.res_t = -1;
     if (MODE == MODE_REC) {  // final post-update sample of a forward slot
         const int n = d.n0 + d.len;
+        const bool hi = d.pair_parity ? (((n + 1) >> 1) & 1) : (n & 1);
         v.u_in = d.phase == SEM_SLOT_FORWARD_HIST ? gm((const T*)d.seg + (long)d.len * d.seg_stride)
-                                                  : gm((const T*)((n & 1) ? d.u1 : d.u0));
+                                                  : gm((const T*)(hi ? d.u1 : d.u0));
         v.u_out = nullptr;
         v.w = nullptr;
         v.rec_t = n - 1;
@@ -255,6 +256,7 @@ __device__ __forceinline__ bool make_view(const StepArgs<T, P>& a, long b, View<
         v.wav = gm((const T*)d.wavelet + n);
         v.src = d.src_node;
         if (d.phase == SEM_SLOT_FORWARD) {
+            if (d.pair_parity) return false;  // two-step launches advance it
             v.u_in = gm((const T*)((n & 1) ? d.u1 : d.u0));
             v.u_out = gm((T*)((n & 1) ? d.u0 : d.u1));
             v.w = gm((T*)d.w);
@@ -701,13 +703,14 @@ void step_kernel(const StepArgs<T, P> a) {
 // later), so it is kept out of the last-level cache that holds the state the
 // next launch reads.  When a.u_copy is set (the full-history forward),
 // u^{n+2} also goes there with plain stores, the next launch reads that copy,
-// and u_out (the history slot) is written non-temporally too.  Batch mode only
-// (no slot table); FP32 and FP64.
+// and u_out (the history slot) is written non-temporally too (batch launches
+// only).  FP32 and FP64.
 // Per-workgroup arrays and time indices of one solve in a two-step launch:
-// from the uniform batch arguments, or from the slot table (FORWARD_HIST,
-// RECOMPUTE and ADJOINT slots; the increment alternates between the slot's w and w2 launch
-// by launch, and a slot whose segment ends after step m takes that step
-// alone: `single`).  Returns false if this solve does not advance.
+// from the uniform batch arguments, or from the slot table (FORWARD,
+// FORWARD_HIST, RECOMPUTE and ADJOINT slots; the increment alternates between
+// the slot's w and w2 launch by launch, and a slot whose segment ends after
+// step m takes that step alone: `single`).  Returns false if this solve does
+// not advance.
 template <typename T>
 struct View2 {
     const GMEM T* u_in;
@@ -775,6 +778,28 @@ __device__ __forceinline__ bool make_view2(const StepArgs<T, P>& a, long b, View
         v.u_in = seg + (long)a.m * d.seg_stride;
         v.u_mid = (GMEM T*)(seg + (long)(a.m + 1) * d.seg_stride);
         v.u_out = (GMEM T*)(seg + (long)(a.m + 2) * d.seg_stride);  // not touched when single
+        v.wav = gm((const T*)d.wavelet + n);
+        v.src = d.src_node;
+        if (d.record) {
+            v.seis = gm((T*)d.seis);
+            v.rec_t = a.post ? n - 1 : n;
+        }
+        return true;
+    }
+    if (d.phase == SEM_SLOT_FORWARD) {
+        // u^n in u1 when (n + 1) / 2 is odd, else in u0 (n even here): a pair
+        // writes u^{n+2} to the other buffer, a step taken alone u^{n+1}; the
+        // checkpoint receives (u, w) after the segment's last pair
+        if (MODE != MODE_FWD || !d.pair_parity || (d.n0 & 1) || (d.ck_u && (d.len & 1))) return false;
+        const int n = d.n0 + a.m;
+        const bool hi = ((n >> 1) & 1) != 0;
+        v.u_in = gm((const T*)(hi ? d.u1 : d.u0));
+        v.u_out = gm((T*)(hi ? d.u0 : d.u1));
+        v.u_mid = v.single ? v.u_out : nullptr;
+        if (a.m + 2 == d.len && d.ck_u) {
+            v.ck_u = gm((T*)d.ck_u);
+            v.ck_w = gm((T*)d.ck_w);
+        }
         v.wav = gm((const T*)d.wavelet + n);
         v.src = d.src_node;
         if (d.record) {

@@ -1774,7 +1774,7 @@ static int slots_launch2(sem_plan* pl, const sem_slot* slots, int nslots, int m,
     a.rec_set_of = nullptr;
     a.c2 = nullptr;
     const bool nts = use_nt(pl, nslots);
-    const unsigned fwd = (1u << SEM_SLOT_FORWARD_HIST) | (1u << SEM_SLOT_RECOMPUTE);
+    const unsigned fwd = (1u << SEM_SLOT_FORWARD) | (1u << SEM_SLOT_FORWARD_HIST) | (1u << SEM_SLOT_RECOMPUTE);
     const unsigned adj = 1u << SEM_SLOT_ADJOINT;
     if ((mask & fwd) && (mask & adj)) {
         a.phase_mask = mask & (fwd | adj);
@@ -1937,7 +1937,7 @@ int loop_chunk(int S) {
 }
 
 template <typename T>
-int loop_round(sem_loop* lp, bool checkpointed, hipStream_t st) {
+int loop_round(sem_loop* lp, hipStream_t st) {
     sem_plan* pl = lp->pl;
     const LoopParams& p = lp->p;
     const int C = p.nslots;
@@ -1948,10 +1948,8 @@ int loop_round(sem_loop* lp, bool checkpointed, hipStream_t st) {
     LAUNCH_CHECK();
     for (int m = 0; m < p.S; ++m) {
         if (p.two_step) {
-            // checkpointed forward slots one step per launch (their state pair
-            // would alias a two-step launch's output); the others in pairs
-            if (checkpointed && (rc = slots_launch<T>(pl, p.table, C, m, FWD, lp->rec_pitch, false, st))) return rc;
-            if (m % 2 == 0 && (rc = slots_launch2<T>(pl, p.table, C, m, FH | RC | ADJ, lp->rec_pitch, st))) return rc;
+            if (m % 2 == 0 && (rc = slots_launch2<T>(pl, p.table, C, m, FWD | FH | RC | ADJ, lp->rec_pitch, st)))
+                return rc;
         } else if ((rc = slots_launch<T>(pl, p.table, C, m, FWD | RC | FH | ADJ, lp->rec_pitch, false, st))) {
             return rc;
         }
@@ -2069,19 +2067,14 @@ int sem_loop_push(sem_loop* lp, const sem_request* r, void* stream) try {
     return SEM_OK;
 } ABI_CATCH
 
-int sem_loop_round_ex(sem_loop* lp, int checkpointed, void* stream) try {
+int sem_loop_round(sem_loop* lp, void* stream) try {
     if (!lp) return fail(SEM_E_INVALID, "sem_loop_round: bad argument");
     int rc = loop_device_check(lp);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    const bool ck = checkpointed != 0;
-    rc = lp->pl->d.precision == SEM_FP64 ? loop_round<double>(lp, ck, st) : loop_round<float>(lp, ck, st);
+    rc = lp->pl->d.precision == SEM_FP64 ? loop_round<double>(lp, st) : loop_round<float>(lp, st);
     if (!rc) ++lp->rounds;
     return rc;
-} ABI_CATCH
-
-int sem_loop_round(sem_loop* lp, void* stream) try {
-    return sem_loop_round_ex(lp, 1, stream);
 } ABI_CATCH
 
 int sem_loop_poll(sem_loop* lp, long long since, long long* ids, long long* rounds, int max, long long* total) try {

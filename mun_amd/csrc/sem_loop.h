@@ -89,6 +89,11 @@ __device__ inline sem_slot loop_desc(const LoopParams& p, const LoopSlot& s) {
         d.u0 = r.u0;
         d.u1 = r.u1;
         d.w = r.w;
+        if (p.two_step) {
+            d.w = s.wflip ? r.w2 : r.w;
+            d.w2 = s.wflip ? r.w : r.w2;
+            d.pair_parity = 1;
+        }
         if (s.k + 1 < s.K) {
             d.ck_u = loop_off(r.cku, (long)(s.k + 1) * p.E, p.elem);
             d.ck_w = loop_off(r.ckw, (long)(s.k + 1) * p.E, p.elem);
@@ -196,15 +201,15 @@ __global__ void loop_advance_kernel(LoopParams p) {
     for (int i = 0; i < p.nslots; ++i) {
         LoopSlot& s = p.st[i];
         if (!s.active) continue;
-        if (p.two_step && (s.phase == SEM_SLOT_FORWARD_HIST || s.phase == SEM_SLOT_ADJOINT)) {
+        if (p.two_step && s.phase != SEM_SLOT_RECOMPUTE) {
             // a segment of ln steps took ceil(ln / 2) launches, each of which
             // moved the increment to the other buffer
             const int ln = min(p.S, s.req.nt - s.k * p.S);
             if (((ln + 1) / 2) % 2 == 1) {
-                if (s.phase == SEM_SLOT_FORWARD_HIST)
-                    s.wflip ^= 1;
-                else
+                if (s.phase == SEM_SLOT_ADJOINT)
                     s.omflip ^= 1;
+                else
+                    s.wflip ^= 1;
             }
         }
         bool done = false;

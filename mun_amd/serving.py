@@ -341,6 +341,9 @@ class Lane:
                 d.w, d.w2 = (b["w2"].data_ptr(), b["w"].data_ptr()) if a.wflip else (d.w, b["w2"].data_ptr())
         elif a.phase == L.SLOT_FORWARD:
             d.u0, d.u1, d.w = b["u0"].data_ptr(), b["u1"].data_ptr(), b["w"].data_ptr()
+            if "w2" in b:  # two steps per launch: u by step pair, the increment flips
+                d.w, d.w2 = (b["w2"].data_ptr(), b["w"].data_ptr()) if a.wflip else (d.w, b["w2"].data_ptr())
+                d.pair_parity = 1
             if a.k + 1 < a.K:
                 d.ck_u, d.ck_w = b["cku"][a.k + 1].data_ptr(), b["ckw"][a.k + 1].data_ptr()
         elif a.phase == L.SLOT_RECOMPUTE:
@@ -383,9 +386,9 @@ class Lane:
         sp = C.c_void_p(self.stream.cuda_stream)
         tab = C.c_void_p(self.table_dev.data_ptr())
         adjoint = bool(mask & (1 << L.SLOT_ADJOINT))
-        # two steps per launch for the FORWARD_HIST, RECOMPUTE and ADJOINT slots (step
+        # two steps per launch when the server runs two-step rounds (step
         # pairs m, m + 1 never straddle a correlation chunk: S and the chunk
-        # are even), one for the other phases
+        # are even), else one
         two = mask & TWO_STEP_PHASES if self.server.two_step else 0
         one = mask & ~two
         for m in range(self.S):
@@ -419,7 +422,8 @@ class Lane:
         self.rounds += 1
 
 
-TWO_STEP_PHASES = (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_RECOMPUTE) | (1 << L.SLOT_ADJOINT)
+TWO_STEP_PHASES = ((1 << L.SLOT_FORWARD) | (1 << L.SLOT_FORWARD_HIST) | (1 << L.SLOT_RECOMPUTE)
+                   | (1 << L.SLOT_ADJOINT))
 
 
 @dataclass
@@ -452,11 +456,7 @@ class LoopLane(Lane):
         self.npushed = 0
         self.ndone = 0                 # completions seen (sem_loop_poll)
         self.round_events = {}         # round number -> completion event of that round
-        # checkpointed requests pushed and not yet seen complete: while there
-        # are none, a two-step loop's rounds leave out the one-step launches
-        # for checkpointed forward slots (sem_loop_round_ex)
-        self.ckpt_rids = set()
-        self.graphs = {}               # captured round, per value of that flag
+        self.graph = None              # the captured round
         self._ids = (C.c_longlong * 256)()
         self._rounds = (C.c_longlong * 256)()
 
@@ -476,7 +476,7 @@ class LoopLane(Lane):
         if len(self.plan.rec_sizes) + 1 > self.max_sets:
             self.max_sets *= 2
             self._reserve()
-            self.graphs = {}  # captured arguments point at the old table
+            self.graph = None  # captured arguments point at the old table
 
     def outstanding(self) -> int:
         return self.npushed - self.ndone
@@ -501,36 +501,21 @@ class LoopLane(Lane):
             L.check(self.plan.lib.sem_loop_push(self.loop, C.byref(r), C.c_void_p(self.stream.cuda_stream)))
         self.pushed[req.rid] = _Pushed(req, b, footprint, now, nrec, block)
         self.npushed += 1
-        self.note_pushed(req.rid, hist)
-
-    def note_pushed(self, rid: int, full_history: bool):
-        if not full_history:
-            self.ckpt_rids.add(rid)
-
-    def note_completed(self, rid: int):
-        self.ckpt_rids.discard(rid)
-
-    def round_variant(self) -> int:
-        """sem_loop_round_ex's `checkpointed` flag for the next round: 0 only
-        when rounds take two steps per launch and every checkpointed request
-        pushed has been seen complete."""
-        return 1 if (not self.server.two_step or self.ckpt_rids) else 0
 
     def launch_round(self, use_graph: bool):
         lib, sp = self.plan.lib, C.c_void_p(self.stream.cuda_stream)
-        ck = self.round_variant()
         with torch.cuda.stream(self.stream):
             if use_graph:
-                if ck not in self.graphs:
+                if self.graph is None:
                     L.check(lib.sem_receivers_sync(self.plan.h))
                     torch.cuda.synchronize(self.server.device)
                     g = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(g, stream=self.stream):
-                        L.check(lib.sem_loop_round_ex(self.loop, ck, sp))
-                    self.graphs[ck] = g
-                self.graphs[ck].replay()
+                        L.check(lib.sem_loop_round(self.loop, sp))
+                    self.graph = g
+                self.graph.replay()
             else:
-                L.check(lib.sem_loop_round_ex(self.loop, ck, sp))
+                L.check(lib.sem_loop_round(self.loop, sp))
             ev = torch.cuda.Event()
             ev.record(self.stream)
         self.round_events[self.rounds] = ev
@@ -556,8 +541,6 @@ class LoopLane(Lane):
                                                 C.byref(total)))
             n = min(len(self._ids), total.value - self.ndone)
             out += [(int(self._ids[i]), int(self._rounds[i])) for i in range(n)]
-            for i in range(n):
-                self.note_completed(int(self._ids[i]))
             self.ndone += n
             if self.ndone >= total.value:
                 return out
@@ -577,11 +560,11 @@ class SEMServer:
         enough waits).  history: 'auto' admits a request with its full forward history
         when that fits the budget (else with checkpoints every `segment`
         steps); 'checkpoints' always uses checkpoints.  two_step: advance
-        full-history forward, recompute and adjoint slots two steps per launch
-        (sem_slots_step2; the default where the segment and correlation chunk
-        are even: on the 48-request benchmark stream the device loop served
-        118-119 solves/s saturated against 106-110 with one step per launch,
-        p99 0.25 against 0.30-0.33 s).  Results are the same bits either way.
+        every slot two steps per launch (sem_slots_step2; the default where
+        the segment and correlation chunk are even: on the 48-request
+        benchmark stream the device loop served 118-119 solves/s saturated
+        against 106-110 with one step per launch, p99 0.25 against 0.30-0.33
+        s).  Results are the same bits either way.
         tile2: the two-step kernel's tile shape in every lane (default: tuned
         per lane when two_step and autotune, which keeps the 19 x 13 tile
         unless another is clearly faster, else shape 1, 19 x 13); results do
@@ -595,8 +578,8 @@ class SEMServer:
             raise ValueError("loop must be 'device' or 'host'")
         self.loop = loop
         self.history = history
-        # two steps per launch for full-history forward and adjoint slots
-        # (sem_slots_step2); needs an even segment and correlation chunk
+        # two steps per launch (sem_slots_step2); needs an even segment and
+        # correlation chunk
         self.two_step = bool(two_step) and int(segment) % 2 == 0 and corr_chunk(int(segment)) % 2 == 0
         self.device = torch.device(device if device is not None else "cuda")
         self.budget = float(hbm_budget_bytes)
@@ -720,15 +703,15 @@ class SEMServer:
                                                  a.nrec, a.req.nt, lane.rec_pitch, sp))
             done = []
             for a in lane.occupied():
-                if self.two_step and a.phase in (L.SLOT_FORWARD_HIST, L.SLOT_ADJOINT):
+                if self.two_step and a.phase != L.SLOT_RECOMPUTE:
                     # a segment of ln steps took ceil(ln / 2) launches, each of
                     # which moved the increment to the other buffer
                     ln = min(self.S, a.req.nt - a.k * self.S)
                     if (ln + 1) // 2 % 2 == 1:
-                        if a.phase == L.SLOT_FORWARD_HIST:
-                            a.wflip = not a.wflip
-                        else:
+                        if a.phase == L.SLOT_ADJOINT:
                             a.omflip = not a.omflip
+                        else:
+                            a.wflip = not a.wflip
                 if a.phase in fwd_phases:
                     if a.k + 1 < a.K:
                         a.k += 1

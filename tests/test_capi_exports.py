@@ -131,7 +131,6 @@ def test_loop_entry_points_validate_before_device_work():
     assert lib.sem_loop_create(None, 4, 100, 0, 64, 16, C.byref(h)) == -1
     assert lib.sem_loop_push(None, None, None) == -1
     assert lib.sem_loop_round(None, None) == -1
-    assert lib.sem_loop_round_ex(None, 0, None) == -1
     tot = C.c_longlong()
     assert lib.sem_loop_poll(None, 0, None, None, 0, C.byref(tot)) == -1
     assert lib.sem_loop_counts(None, None, None, None, None) == -1

@@ -120,10 +120,11 @@ def test_history_and_checkpoint_slots_give_the_same_bits():
 
 @pytest.mark.parametrize("history", ["auto", "checkpoints"])
 def test_two_step_rounds_give_the_same_bits(history):
-    """Serving rounds that advance full-history forward and adjoint slots two
-    steps per launch (sem_slots_step2; odd segment tails take one step, the
-    increment buffers swap accordingly) give the same bits as one step per
-    launch, with captured rounds, for history and checkpointed requests."""
+    """Serving rounds that advance every slot two steps per launch
+    (sem_slots_step2; odd segment tails take one step, the increment buffers
+    swap accordingly, checkpointed forward slots keep u by step pair) give the
+    same bits as one step per launch, with captured rounds, for history and
+    checkpointed requests."""
     from mun_amd.serving import SEMServer
     reqs = make_requests(6, seed=11)
     kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420,

@@ -100,31 +100,3 @@ def test_serving_correlation_chunk_rule():
         assert 1 <= c <= min(S, 20)
         assert c % 2 == 0 or c == S
         assert S - (S // c) * c <= c
-
-
-def test_device_loop_round_variant_keeps_checkpointed_launches_until_seen_complete():
-    """A two-step device loop leaves out the one-step launches for
-    checkpointed forward slots only while no checkpointed request it pushed
-    is still outstanding (push -> polled completion); a one-step loop always
-    takes the full round."""
-    from types import SimpleNamespace
-
-    from mun_amd.serving import LoopLane
-    lane = object.__new__(LoopLane)
-    lane.ckpt_rids = set()
-    lane.server = SimpleNamespace(two_step=True)
-    assert lane.round_variant() == 0
-    lane.note_pushed(7, full_history=True)
-    assert lane.round_variant() == 0
-    lane.note_pushed(8, full_history=False)
-    lane.note_pushed(9, full_history=False)
-    assert lane.round_variant() == 1
-    lane.note_completed(8)
-    lane.note_completed(7)
-    assert lane.round_variant() == 1      # request 9 is still in flight
-    lane.note_completed(9)
-    assert lane.round_variant() == 0
-    lane.note_completed(9)                # a repeated poll changes nothing
-    assert lane.round_variant() == 0
-    lane.server = SimpleNamespace(two_step=False)
-    assert lane.round_variant() == 1

@@ -107,7 +107,7 @@ def test_history_and_checkpoint_slots_give_the_same_bits():
     big = max(reqs, key=lambda r: (r.nelem_x, r.nt))
     lb = srv.lanes[big.lane_key]
     fp_full = request_footprint(lb.plan.NX, lb.plan.NZ, lb.E, big.nt, lb.rec_pitch, 50, grad_elems=lb.GE,
-                                full_history=True)
+                                full_history=True, two_step=srv.two_step)
     print(f"[serve-history] mixed budget {1.5 * fp_full / 1e6:.1f} MB", flush=True)
     mixed = SEMServer(hbm_budget_bytes=1.5 * fp_full, history="auto", **kw).run(reqs)
     for r in reqs:

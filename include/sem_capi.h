@@ -163,9 +163,9 @@ int sem_plan_set_streaming(sem_plan* plan, int mode);
  * pair of steps reads the state once, and stores it with non-temporal
  * writes), or 1.  Two-step launches cover the forward pass (full history,
  * checkpointed, forward-only or with snapshots), the recompute and the
- * adjoint; slot launches take one step per launch, and so do the odd last
- * step of a pass and a step pair that would straddle a checkpoint or hold a
- * snapshot in its middle.  Results are
+ * adjoint; sem_slots_step takes one step per launch (sem_slots_step2 two),
+ * and so do the odd last step of a pass and a step pair that would straddle
+ * a checkpoint or hold a snapshot in its middle.  Results are
  * bitwise the same for 1 and 2.  No reference counterpart (the
  * reference loops over single steps, core.py:1197-1275). */
 int sem_plan_set_step_fusion(sem_plan* plan, int steps);
@@ -476,9 +476,9 @@ typedef struct {
 } sem_request;
 
 typedef struct sem_loop sem_loop;
-/* nslots <= 256 slots advancing `segment` steps per round (two_step: full-history
- * forward, recompute and adjoint slots two steps per launch; needs an even
- * segment and correlation chunk); queue_capacity: requests pushed and not yet
+/* nslots <= 256 slots advancing `segment` steps per round (two_step: every
+ * slot two steps per launch; needs an even segment and correlation chunk);
+ * queue_capacity: requests pushed and not yet
  * completed, at most.  Allocates the loop's device and host-mapped state. */
 int sem_loop_create(sem_plan* plan, int nslots, int segment, int two_step, int rec_pitch, int queue_capacity,
                     sem_loop** out);

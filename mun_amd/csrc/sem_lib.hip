@@ -1926,12 +1926,12 @@ struct sem_loop {
 
 namespace {
 
-// Adjoint steps per correlation chunk of a loop: at most 20, the segment split
+// Adjoint steps per correlation chunk of a loop: at most 50, the segment split
 // into near-equal chunks, rounded up to an even length when shorter than the
 // segment (two-step launches never straddle an even chunk boundary); the ring
 // then holds chunk + 1 states per request.
 int loop_chunk(int S) {
-    const int n = (S + 19) / 20;
+    const int n = (S + 49) / 50;
     const int c = (S + n - 1) / n;
     return (c % 2 && c < S) ? c + 1 : c;
 }

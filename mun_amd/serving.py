@@ -109,13 +109,15 @@ class SolveResult:
 
 
 def corr_chunk(S: int) -> int:
-    """Adjoint steps per gradient-correlation chunk in a lane: at most 20, a
+    """Adjoint steps per gradient-correlation chunk in a lane: at most 50, a
     segment of S steps split into near-equal chunks, rounded up to an even
     length when shorter than the segment (two-step launches never straddle an
     even chunk boundary); the device loop's sem_loop_chunk is the same rule.
     Longer chunks mean fewer correlation launches per round and a longer
-    adjoint ring (chunk + 1 states) per request."""
-    n = -(-S // 20)
+    adjoint ring (chunk + 1 states) per request: with 100-step segments, 50
+    served 32 200x200 solves through one lane 4 % faster than 20 and the
+    mixed-size stream as fast; 100 left the stream 8 % slower."""
+    n = -(-S // 50)
     c = -(-S // n)
     return c + 1 if c % 2 and c < S else c
 

@@ -47,7 +47,7 @@ def test_served_results_equal_alone_in_mix_and_plain():
     print("[serve] building requests", flush=True)
     reqs = make_requests()
     print("[serve] mix", flush=True)
-    srv = SEMServer(hbm_budget_bytes=8e9, segment=50, capacity={8: 3, 12: 2, 20: 2}, use_graph=True,
+    srv = SEMServer(hbm_budget_bytes=8e9, segment=60, capacity={8: 3, 12: 2, 20: 2}, use_graph=True,
                     rec_pitch=32, nt_max=420)
     mix = srv.run(reqs)
     assert len(mix) == len(reqs)
@@ -57,13 +57,13 @@ def test_served_results_equal_alone_in_mix_and_plain():
     assert any(adm[i + 1][0] < adm[i][1] for i in range(len(adm) - 1))
     print("[serve] alone", flush=True)
     for r in reqs[:4]:
-        solo = SEMServer(hbm_budget_bytes=8e9, segment=50, use_graph=False, rec_pitch=32, nt_max=420).run([r])
+        solo = SEMServer(hbm_budget_bytes=8e9, segment=60, use_graph=False, rec_pitch=32, nt_max=420).run([r])
         a, b = solo[r.rid], mix[r.rid]
         assert a.loglik == b.loglik
         np.testing.assert_array_equal(a.grad, b.grad)
         np.testing.assert_array_equal(a.seis, b.seis)
     print("[serve] plain", flush=True)
-    plain = run_plain(reqs, segment=50)
+    plain = run_plain(reqs, segment=60)
     for rid, p in plain.items():
         m = mix[rid]
         assert abs(p.loglik - m.loglik) <= 1e-6 * abs(p.loglik)
@@ -96,7 +96,7 @@ def test_history_and_checkpoint_slots_give_the_same_bits():
     from mun_amd.serving import SEMServer, request_footprint
     reqs = make_requests(6, seed=7)
     print("[serve-history] checkpoints", flush=True)
-    kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420)
+    kw = dict(segment=60, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420)
     ck = SEMServer(hbm_budget_bytes=8e9, history="checkpoints", **kw).run(reqs)
     print("[serve-history] full history", flush=True)
     srv = SEMServer(hbm_budget_bytes=8e9, history="auto", **kw)
@@ -106,7 +106,7 @@ def test_history_and_checkpoint_slots_give_the_same_bits():
     # checkpointed ones: admission mixes both kinds
     big = max(reqs, key=lambda r: (r.nelem_x, r.nt))
     lb = srv.lanes[big.lane_key]
-    fp_full = request_footprint(lb.plan.NX, lb.plan.NZ, lb.E, big.nt, lb.rec_pitch, 50, grad_elems=lb.GE,
+    fp_full = request_footprint(lb.plan.NX, lb.plan.NZ, lb.E, big.nt, lb.rec_pitch, 60, grad_elems=lb.GE,
                                 full_history=True, two_step=srv.two_step)
     print(f"[serve-history] mixed budget {1.5 * fp_full / 1e6:.1f} MB", flush=True)
     mixed = SEMServer(hbm_budget_bytes=1.5 * fp_full, history="auto", **kw).run(reqs)
@@ -127,7 +127,7 @@ def test_two_step_rounds_give_the_same_bits(history):
     checkpointed requests."""
     from mun_amd.serving import SEMServer
     reqs = make_requests(6, seed=11)
-    kw = dict(segment=50, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420,
+    kw = dict(segment=60, capacity={8: 3, 12: 3, 20: 3}, use_graph=True, rec_pitch=32, nt_max=420,
               history=history)
     print(f"[serve-two-step] {history}: one step per launch", flush=True)
     one = SEMServer(hbm_budget_bytes=8e9, two_step=False, **kw).run(reqs)
@@ -155,7 +155,7 @@ def test_device_loop_gives_the_host_loops_bits(two_step):
     a warm server again, without graphs, give the same bits too."""
     from mun_amd.serving import LoopLane, SEMServer
     reqs = make_requests(8, seed=5)
-    kw = dict(segment=50, capacity={8: 2, 12: 2, 20: 1}, rec_pitch=32, nt_max=420, two_step=two_step)
+    kw = dict(segment=60, capacity={8: 2, 12: 2, 20: 1}, rec_pitch=32, nt_max=420, two_step=two_step)
     print(f"[serve-loop] two_step={two_step}: host loop", flush=True)
     host = SEMServer(hbm_budget_bytes=8e9, loop="host", use_graph=True, **kw).run(reqs)
     print(f"[serve-loop] two_step={two_step}: device loop", flush=True)

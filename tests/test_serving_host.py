@@ -93,10 +93,10 @@ def test_block_allocator_first_fit_merges_and_rejects_double_release():
 
 
 def test_serving_correlation_chunk_rule():
-    """Chunks of at most 20 steps, even unless a chunk is the whole (odd)
+    """Chunks of at most 50 steps, even unless a chunk is the whole (odd)
     segment, and the last chunk of a segment never longer than the others."""
     for S in range(1, 300):
         c = corr_chunk(S)
-        assert 1 <= c <= min(S, 20)
+        assert 1 <= c <= min(S, 50)
         assert c % 2 == 0 or c == S
         assert S - (S // c) * c <= c

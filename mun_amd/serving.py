@@ -139,6 +139,19 @@ def request_footprint(NX, NZ, pitch_elems, nt, nrec_pitch, S, elem_bytes=4, grad
     return nodal + traces
 
 
+def lanes_for_mesh(nelem_x: int, nelem_z: int, compute_units: int) -> int:
+    """Lanes (streams) for a size class by default: 2 where one solve
+    already fills the GPU (its two-step launch on the 19 x 13 tile, 16 x 10
+    owned elements, has at least one workgroup per compute unit), so that one
+    lane's correlation and round boundaries overlap the other's steps, as the
+    batch path's two lanes do; 1 below that.  32 full-history 200x200 solves
+    through 16 slots: 28.6 solves/s as 2 x 8 against 25.1 as one lane;
+    splitting the 20-160^2 classes of the mixed stream too served it 4-10 %
+    slower."""
+    tiles = -(-nelem_x // 16) * -(-nelem_z // 10)
+    return 2 if tiles >= compute_units else 1
+
+
 ALIGN = 256  # bytes: every buffer of a request block starts on this boundary
 
 
@@ -257,8 +270,12 @@ class _Active:
 
 
 class Lane:
+    """The slots, plan and stream of one size class (or of one of its lanes:
+    `key` is the class's lane_key, followed by the lane's index when the
+    class has more than one)."""
+
     def __init__(self, server: "SEMServer", key, capacity: int, nt_max: int, rec_pitch: int):
-        nex, nez, xmax, zmax, dt, sw, scm = key
+        nex, nez, xmax, zmax, dt, sw, scm = key[:7]
         self.server = server
         self.key = key
         self.C = capacity
@@ -555,7 +572,7 @@ class SEMServer:
                  adjoint_mode: str = "reference", sides=("xmin", "xmax", "zmax"), use_graph: bool = True,
                  device=None, rec_pitch: int = 200, nt_max: int = 5000, max_sets: int = 512,
                  autotune: bool = True, history: str = "auto", two_step: bool = True,
-                 tile2: Optional[int] = None, loop: str = "device"):
+                 tile2: Optional[int] = None, loop: str = "device", lanes_per_class: Optional[int] = None):
         """hbm_budget_bytes: the device memory that holds the admitted
         requests' buffers, allocated once here as the server's arena; each
         request takes one block of it (a request that finds no block large
@@ -573,7 +590,9 @@ class SEMServer:
         not depend on it.  loop: 'device'
         keeps the request queue, slot states and round scheduling on the
         device (sem_loop_*), 'host' schedules the rounds from the host; the
-        same bits either way."""
+        same bits either way.  lanes_per_class: lanes (streams) per size
+        class, its capacity shared among them (default: class_lanes' rule,
+        two where one solve fills the GPU); the same bits in any lane."""
         if history not in ("auto", "checkpoints"):
             raise ValueError("history must be 'auto' or 'checkpoints'")
         if loop not in ("device", "host"):
@@ -596,6 +615,8 @@ class SEMServer:
         self.autotune = bool(autotune)
         self.tile2 = None if tile2 is None else int(tile2)
         self.lanes: Dict[tuple, Lane] = {}
+        self.class_of: Dict[tuple, List[Lane]] = {}  # size class -> its lanes
+        self.lanes_per_class = None if lanes_per_class is None else int(lanes_per_class)
         self.finish_stream = None
         self.arena = DeviceArena(int(self.budget), self.device)
         self.in_use = 0
@@ -606,13 +627,30 @@ class SEMServer:
     def clock(self) -> float:
         return time.perf_counter() - self.t0
 
+    def class_lanes(self, nelem_x: int, nelem_z: int, capacity: int) -> int:
+        """Lanes (streams) a size class is split into: `lanes_per_class` when
+        set, else lanes_for_mesh's rule for this device; at most one per slot."""
+        n = self.lanes_per_class
+        if n is None:
+            n = lanes_for_mesh(nelem_x, nelem_z, torch.cuda.get_device_properties(self.device).multi_processor_count)
+        return max(1, min(n, capacity))
+
     def lane(self, req: SolveRequest) -> Lane:
+        """The lane that takes `req`: of its size class's lanes (created on
+        first use, the class's capacity shared out among them), the one with
+        the least queued and outstanding work per slot."""
         key = req.lane_key
-        if key not in self.lanes:
+        if key not in self.class_of:
             cap = self.capacity.get(req.nelem_x, max(4, min(64, 2560 // max(req.nelem_x, 1))))
+            n = self.class_lanes(req.nelem_x, req.nelem_z, cap)
             cls = LoopLane if self.loop == "device" else Lane
-            self.lanes[key] = cls(self, key, cap, self.nt_max, self.rec_pitch)
-        return self.lanes[key]
+            lanes = []
+            for i in range(n):
+                k = key if i == 0 else key + (i,)
+                self.lanes[k] = cls(self, k, cap // n + (1 if i < cap % n else 0), self.nt_max, self.rec_pitch)
+                lanes.append(self.lanes[k])
+            self.class_of[key] = lanes
+        return min(self.class_of[key], key=lambda l: (len(l.queue) + l.outstanding()) / l.C)
 
     # -------------------------------------------------------------- admission
     def _admit(self, lane: Lane, req: SolveRequest, now: float) -> bool:

@@ -146,6 +146,33 @@ def test_two_step_rounds_give_the_same_bits(history):
             np.testing.assert_array_equal(a.seis, b.seis)
 
 
+@pytest.mark.parametrize("loop", ["device", "host"])
+def test_size_class_split_into_two_lanes_gives_the_same_bits(loop):
+    """A size class split into two lanes (two streams sharing its capacity,
+    each request routed to the less loaded) gives every request the bits of
+    one lane per class; the classes with several requests use both lanes."""
+    from mun_amd.serving import SEMServer
+    reqs = make_requests(8, seed=3)
+    kw = dict(segment=60, capacity={8: 4, 12: 4, 20: 2}, use_graph=True, rec_pitch=32, nt_max=420, loop=loop)
+    print(f"[serve-split] {loop}: one lane per class", flush=True)
+    one = SEMServer(hbm_budget_bytes=8e9, lanes_per_class=1, **kw).run(reqs)
+    print(f"[serve-split] {loop}: two lanes per class", flush=True)
+    srv = SEMServer(hbm_budget_bytes=8e9, lanes_per_class=2, **kw)
+    two = srv.run(reqs)
+    assert len(srv.lanes) == 2 * len(srv.class_of)
+    assert all(sum(l.C for l in ls) == kw["capacity"][ls[0].key[0]] for ls in srv.class_of.values())
+    counts = {}
+    for r in reqs:
+        counts[r.lane_key] = counts.get(r.lane_key, 0) + 1
+    busy = [ls for k, ls in srv.class_of.items() if counts[k] >= 2]
+    assert busy and all(l.rounds > 0 for ls in busy for l in ls)
+    for r in reqs:
+        a, b = one[r.rid], two[r.rid]
+        assert a.loglik == b.loglik
+        np.testing.assert_array_equal(a.grad, b.grad)
+        np.testing.assert_array_equal(a.seis, b.seis)
+
+
 @pytest.mark.parametrize("two_step", [False, True])
 def test_device_loop_gives_the_host_loops_bits(two_step):
     """The device-resident request loop (queue, slot states and round

@@ -1,6 +1,6 @@
 """Host-side serving logic (no GPU): the admission footprint, the
 bookkeeping of the increment buffers under two-step rounds, the request
-arena and the device loop's choice of round."""
+arena and the split of size classes into lanes."""
 import math
 
 import pytest
@@ -100,3 +100,15 @@ def test_serving_correlation_chunk_rule():
         assert 1 <= c <= min(S, 50)
         assert c % 2 == 0 or c == S
         assert S - (S // c) * c <= c
+
+
+def test_size_classes_split_into_two_lanes_where_one_solve_fills_the_gpu():
+    """Two lanes where one solve's two-step launch (19 x 13 tile, 16 x 10
+    owned elements) has a workgroup per compute unit: 200x200 (13 x 20 = 260
+    tiles) on 256 compute units, not 160x160 (160 tiles) or the smaller
+    served meshes."""
+    from mun_amd.serving import lanes_for_mesh
+    assert lanes_for_mesh(200, 200, 256) == 2
+    assert lanes_for_mesh(120, 120, 256) == 1
+    assert [lanes_for_mesh(n, n, 256) for n in (20, 40, 80, 160)] == [1, 1, 1, 1]
+    assert lanes_for_mesh(160, 160, 160) == 2 and lanes_for_mesh(160, 160, 161) == 1

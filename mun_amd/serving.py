@@ -852,16 +852,28 @@ class SEMServer:
                     r.arrival = now
                 self.lane(r).queue.append(r)
             progressed = False
-            for lane in list(self.lanes.values()):
+            lanes = list(self.lanes.values())
+            for lane in lanes:
                 while lane.inflight and lane.inflight[0].query():
                     lane.inflight.popleft()
                     progressed = True
-                device_loop = isinstance(lane, LoopLane)
-                if device_loop:
+                if isinstance(lane, LoopLane):
                     for rid, rnd in lane.poll():
                         self._finish_pushed(lane, rid, rnd)
                         progressed = True
                 progressed |= self._collect(lane)
+            # admission one request per lane in turn, so that no lane takes
+            # the arena's free memory for requests that would wait in its
+            # device queue while another lane has requests and free slots
+            admitted = True
+            while admitted:
+                admitted = False
+                for lane in lanes:
+                    if lane.queue and self._admit(lane, lane.queue[0], self.clock()):
+                        lane.queue.popleft()
+                        admitted = progressed = True
+            for lane in lanes:
+                device_loop = isinstance(lane, LoopLane)
                 # keep up to ROUNDS_AHEAD rounds enqueued: admit, enqueue the
                 # next round (host loop: then the work that follows it)
                 while len(lane.inflight) < ROUNDS_AHEAD:

@@ -3,6 +3,7 @@
     python -m mun_amd.profile_kernels [--batch 256] [--nt 40] [--copy-mb 1024] [--tile-rows 15] [--tile2 1]
     python -m mun_amd.profile_kernels --summarize-pmc FETCH_CSV WRITE_CSV --batch B [--out JSON]
     python -m mun_amd.profile_kernels --summarize-trace KERNEL_TRACE_CSV [--top 30]
+    python -m mun_amd.profile_kernels --summarize-gaps KERNEL_TRACE_CSV
     python -m mun_amd.profile_kernels --summarize-counters SQ1_CSV SQ2_CSV SQ3_CSV
 
 Order of GPU work (so counter runs can attribute dispatches):
@@ -130,6 +131,69 @@ def summarize_trace(trace_csv, top=30):
     return "\n".join(lines)
 
 
+def summarize_gaps(trace_csv, min_gap_us=5.0, top=12):
+    """Idle time between consecutive kernels of each stream in a rocprofv3
+    kernel trace: per stream the kernels, busy time and span, and the gaps of
+    at least min_gap_us attributed to the kernel that ended each gap (what
+    the stream waited to start), largest total first; then the time no
+    kernel of any stream ran while the streams with a two-step kernel were
+    between their first and last one."""
+    import collections
+    import csv
+
+    rows = list(csv.DictReader(open(trace_csv)))
+    sid = "Stream_Id" if rows and "Stream_Id" in rows[0] else "Queue_Id"
+    streams = collections.defaultdict(list)
+    for r in rows:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        streams[r[sid]].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name))
+    lines = [f"| {sid.lower()} | kernels | busy ms | span ms | idle in gaps >= {min_gap_us:g} us (ms) | "
+             f"gaps | largest gaps by the kernel that ended them (count, total ms, max us) |",
+             "|---|---|---|---|---|---|---|"]
+    windows = []
+    for key, ks in sorted(streams.items(), key=lambda kv: -len(kv[1])):
+        ks.sort()
+        busy = sum(e - s_ for s_, e, _ in ks)
+        by = collections.defaultdict(lambda: [0, 0, 0])
+        idle = n = 0
+        for (s0, e0, _), (s1, e1, nm) in zip(ks, ks[1:]):
+            g = s1 - e0
+            if g >= min_gap_us * 1e3:
+                idle += g
+                n += 1
+                v = by[nm]
+                v[0] += 1
+                v[1] += g
+                v[2] = max(v[2], g)
+        worst = sorted(by.items(), key=lambda kv: -kv[1][1])[:3]
+        desc = "; ".join(f"`{nm}` ({c}, {t / 1e6:.2f}, {m / 1e3:.0f})" for nm, (c, t, m) in worst)
+        lines.append(f"| {key} | {len(ks)} | {busy / 1e6:.1f} | {(ks[-1][1] - ks[0][0]) / 1e6:.1f} | "
+                     f"{idle / 1e6:.2f} | {n} | {desc} |")
+        two = [k for k in ks if "step2_kernel" in k[2]]
+        if two:
+            windows.append((two[0][0], max(k[1] for k in two)))
+        if len(lines) >= top + 2:
+            break
+    if windows:
+        w0, w1 = min(w[0] for w in windows), max(w[1] for w in windows)
+        iv = sorted((s_, e) for ks in streams.values() for s_, e, _ in ks if e > w0 and s_ < w1)
+        covered, cur0, cur1 = 0, None, None
+        for s_, e in iv:
+            s_, e = max(s_, w0), min(e, w1)
+            if cur1 is None or s_ > cur1:
+                if cur1 is not None:
+                    covered += cur1 - cur0
+                cur0, cur1 = s_, e
+            else:
+                cur1 = max(cur1, e)
+        if cur1 is not None:
+            covered += cur1 - cur0
+        lines.append("")
+        lines.append(f"two-step window {(w1 - w0) / 1e6:.1f} ms: no kernel running for "
+                     f"{(w1 - w0 - covered) / 1e6:.2f} ms ({100 * (w1 - w0 - covered) / max(w1 - w0, 1):.2f} %)")
+    return "\n".join(lines)
+
+
 def summarize_counters(csvs, kernels=("step2_kernel", "corr_kernel")):
     """Markdown table of SQ / TA / TCC counter passes of this driver
     (rocprofv3 --pmc ... --output-format csv, one CSV per pass): per (kernel,
@@ -193,6 +257,8 @@ def main(argv=None):
     ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
                     help="per-kernel table of a rocprofv3 kernel trace (no GPU work)")
     ap.add_argument("--top", type=int, default=30, help="rows of --summarize-trace")
+    ap.add_argument("--summarize-gaps", metavar="KERNEL_TRACE_CSV",
+                    help="idle gaps between the kernels of each stream of a kernel trace (no GPU work)")
     ap.add_argument("--summarize-counters", nargs="+", metavar="COUNTER_CSV",
                     help="per-kernel medians of SQ / TA / TCC counter passes of this driver (no GPU work)")
     a = ap.parse_args(argv)
@@ -201,6 +267,9 @@ def main(argv=None):
         return 0
     if a.summarize_trace:
         print(summarize_trace(a.summarize_trace, a.top))
+        return 0
+    if a.summarize_gaps:
+        print(summarize_gaps(a.summarize_gaps))
         return 0
     if a.summarize_pmc:
         import json

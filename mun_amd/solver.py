@@ -254,8 +254,9 @@ class Plan:
 
     def set_tile2(self, shape: int):
         """Tile shape of the two-step kernel (0: 19 x 26 element threads,
-        1: 19 x 13, 2: 35 x 14, 3: 35 x 28, FP32 only).  Results do not depend
-        on it; set it before receivers are reserved for captured rounds."""
+        1: 19 x 13, 2: 35 x 14, 3: 35 x 28 (FP32 only), 4: 23 x 11).  Results
+        do not depend on it; set it before receivers are reserved for
+        captured rounds."""
         L.check(self.lib.sem_plan_set_tile2(self.h, int(shape)))
 
     def get_tile2(self):

@@ -146,6 +146,37 @@ def test_two_step_rounds_give_the_same_bits(history):
             np.testing.assert_array_equal(a.seis, b.seis)
 
 
+@pytest.mark.parametrize("history", ["auto", "checkpoints"])
+def test_exact_adjoint_served_matches_one_step_and_plain(history):
+    """The exact adjoint (the damped forward's discrete adjoint) through the
+    server: two steps per launch give the bits of one step per launch, and
+    the results match the plain path in the same mode (its own launch
+    sequence at batch 1)."""
+    from mun_amd.serving import SEMServer, run_plain
+    reqs = make_requests(5, seed=9)
+    kw = dict(segment=60, capacity={8: 2, 12: 2, 20: 2}, use_graph=True, rec_pitch=32, nt_max=420,
+              adjoint_mode="exact", history=history)
+    print(f"[serve-exact] {history}: two steps per launch", flush=True)
+    two = SEMServer(hbm_budget_bytes=8e9, two_step=True, **kw).run(reqs)
+    print(f"[serve-exact] {history}: one step per launch", flush=True)
+    one = SEMServer(hbm_budget_bytes=8e9, two_step=False, **kw).run(reqs)
+    print(f"[serve-exact] {history}: plain path", flush=True)
+    plain = run_plain(reqs, adjoint_mode="exact", segment=60)
+    ref = SEMServer(hbm_budget_bytes=8e9, two_step=True, **dict(kw, adjoint_mode="reference")).run(reqs[:2])
+    for r in reqs:
+        a, b, p = two[r.rid], one[r.rid], plain[r.rid]
+        assert a.loglik == b.loglik
+        np.testing.assert_array_equal(a.grad, b.grad)
+        np.testing.assert_array_equal(a.seis, b.seis)
+        e = rel_l2(a.grad, p.grad)
+        print(f"  request {r.rid} ({r.nelem_x}^2, {r.nt} steps): grad vs plain {e:.2e}", flush=True)
+        assert abs(a.loglik - p.loglik) <= 1e-6 * abs(p.loglik)
+        assert e < 1e-6
+    # the mode reaches the adjoint: reference-mode gradients differ where the
+    # wave energy enters the sponge
+    assert any(rel_l2(ref[r.rid].grad, two[r.rid].grad) > 1e-4 for r in reqs[:2])
+
+
 @pytest.mark.parametrize("loop", ["device", "host"])
 def test_size_class_split_into_two_lanes_gives_the_same_bits(loop):
     """A size class split into two lanes (two streams sharing its capacity,

@@ -670,6 +670,8 @@ class SEMServer:
         nrec = int(np.asarray(req.rec_xz).reshape(-1, 2).shape[0])
         if nrec > lane.rec_pitch:
             raise ValueError(f"request {req.rid}: {nrec} receivers exceed the lane pitch {lane.rec_pitch}")
+        if not 1 <= req.nt <= self.nt_max:
+            raise ValueError(f"request {req.rid}: {req.nt} time steps outside 1..nt_max = {self.nt_max}")
         # one arena block per request: its full forward history when that fits
         # (history='auto'), else checkpoints; none free yet -> it waits
         lay_args = (E, req.nt, lane.rec_pitch, S, lane.chunk, lane.GE, plan.N if device_loop else 0)

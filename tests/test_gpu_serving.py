@@ -146,6 +146,28 @@ def test_two_step_rounds_give_the_same_bits(history):
             np.testing.assert_array_equal(a.seis, b.seis)
 
 
+@pytest.mark.parametrize("loop", ["device", "host"])
+def test_requests_beyond_the_lane_limits_are_refused(loop):
+    """A request with more receivers than the lane's pitch, or more time
+    steps than the server's nt_max, raises ValueError before it takes arena
+    memory; the server then serves valid requests as before."""
+    import dataclasses
+
+    from mun_amd.serving import SEMServer
+    r = make_requests(1, seed=2)[0]
+    srv = SEMServer(hbm_budget_bytes=2e9, segment=60, use_graph=False, rec_pitch=32, nt_max=r.nt, loop=loop)
+    many = np.stack([np.linspace(20.0, r.xmax - 20.0, 40), np.full(40, 2.0)], axis=1)
+    for bad, what in ((dataclasses.replace(r, rid=101, rec_xz=many, obs=np.zeros((r.nt, 40))), "receivers"),
+                      (dataclasses.replace(r, rid=102, nt=r.nt + 1), "time steps")):
+        with pytest.raises(ValueError, match=what):
+            srv.run([bad])
+        assert srv.in_use == 0
+        for lane in srv.lanes.values():
+            lane.queue.clear()
+    ok = srv.run([r])[r.rid]
+    assert np.isfinite(ok.loglik) and np.abs(ok.grad).max() > 0
+
+
 @pytest.mark.parametrize("history", ["auto", "checkpoints"])
 def test_exact_adjoint_served_matches_one_step_and_plain(history):
     """The exact adjoint (the damped forward's discrete adjoint) through the

@@ -147,6 +147,27 @@ def test_two_step_rounds_give_the_same_bits(history):
 
 
 @pytest.mark.parametrize("loop", ["device", "host"])
+def test_receiver_tables_growing_mid_stream_give_the_same_bits(loop):
+    """Lanes that start with room for one receiver set grow their tables
+    (and re-capture their rounds) as requests arrive; every request gets the
+    bits of a server that never grows them."""
+    from mun_amd.serving import SEMServer
+    reqs = make_requests(6, seed=13)
+    kw = dict(segment=60, capacity={8: 2, 12: 2, 20: 2}, use_graph=True, rec_pitch=32, nt_max=420, loop=loop)
+    print(f"[serve-grow] {loop}: roomy tables", flush=True)
+    base = SEMServer(hbm_budget_bytes=8e9, **kw).run(reqs)
+    print(f"[serve-grow] {loop}: one set to start", flush=True)
+    srv = SEMServer(hbm_budget_bytes=8e9, max_sets=1, **kw)
+    grown = srv.run(reqs)
+    assert any(l.max_sets > 1 for l in srv.lanes.values())
+    for r in reqs:
+        a, b = base[r.rid], grown[r.rid]
+        assert a.loglik == b.loglik
+        np.testing.assert_array_equal(a.grad, b.grad)
+        np.testing.assert_array_equal(a.seis, b.seis)
+
+
+@pytest.mark.parametrize("loop", ["device", "host"])
 def test_requests_beyond_the_lane_limits_are_refused(loop):
     """A request with more receivers than the lane's pitch, or more time
     steps than the server's nt_max, raises ValueError before it takes arena

@@ -463,9 +463,7 @@ def main():
             st["makespan_s"] = float(np.mean([p_["makespan_s"] for p_ in passes]))
             st["passes"] = len(passes)
             sat_ = latency_stats(srv.run(at_once))
-            for lane in srv.lanes.values():
-                if hasattr(lane, "close"):
-                    lane.close()
+            srv.close()
             del srv
             gc.collect()
             torch.cuda.synchronize()

@@ -501,6 +501,14 @@ int sem_loop_poll(sem_loop* loop, long long since, long long* ids, long long* ro
 int sem_loop_counts(const sem_loop* loop, long long* pushed, long long* rounds_enqueued, long long* rounds_done,
                     int* active);
 
+/* A non-blocking HIP stream of default priority on the calling thread's
+ * current device, and its release (waits for the work on it first).  No
+ * reference counterpart: a serving lane takes one for its lifetime, so that
+ * each lane's work sits on a hardware queue of its own; lanes on pooled
+ * streams could share one and then run one after the other. */
+int sem_stream_create(void** stream);
+int sem_stream_destroy(void* stream);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* sem_last_error(void);
 

@@ -30,7 +30,7 @@ EXPORTS = ["sem_plan_create", "sem_plan_destroy", "sem_plan_dims", "sem_plan_axe
            "sem_plan_get_step_fusion", "sem_slots_step2", "sem_bspline_project_noalloc", "sem_plan_set_tile2",
            "sem_plan_get_tile2", "sem_plan_set_corr_chunk", "sem_plan_get_corr_chunk", "sem_loop_create",
            "sem_loop_destroy", "sem_loop_chunk", "sem_loop_push", "sem_loop_round", "sem_loop_poll",
-           "sem_loop_counts", "sem_last_error"]
+           "sem_loop_counts", "sem_stream_create", "sem_stream_destroy", "sem_last_error"]
 
 SEM_INTERP_LINEAR, SEM_INTERP_NEAREST = 0, 1
 SEM_STREAM_AUTO, SEM_STREAM_PLAIN, SEM_STREAM_NONTEMPORAL = 0, 1, 2
@@ -148,6 +148,8 @@ def lib() -> C.CDLL:
     L.sem_loop_round.argtypes = [vp, vp]
     L.sem_loop_poll.argtypes = [vp, ll, llp, llp, i, llp]
     L.sem_loop_counts.argtypes = [vp, llp, llp, llp, ip]
+    L.sem_stream_create.argtypes = [C.POINTER(vp)]
+    L.sem_stream_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name != "sem_last_error":
             getattr(L, name).restype = C.c_int

@@ -1164,6 +1164,22 @@ extern "C" {
 
 const char* sem_last_error(void) { return g_err.c_str(); }
 
+int sem_stream_create(void** stream) try {
+    if (!stream) return fail(SEM_E_INVALID, "sem_stream_create: bad argument");
+    *stream = nullptr;
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return SEM_OK;
+} ABI_CATCH
+
+int sem_stream_destroy(void* stream) try {
+    if (!stream) return SEM_OK;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return SEM_OK;
+} ABI_CATCH
+
 int sem_plan_create(const sem_desc* d, sem_plan** out) try {
     if (!d || !out) return fail(SEM_E_INVALID, "null argument");
     *out = nullptr;

@@ -137,3 +137,10 @@ def test_loop_entry_points_validate_before_device_work():
     assert lib.sem_loop_chunk(None, None) == -1
     assert lib.sem_loop_destroy(None) == 0
     assert "bad argument" in lib.sem_last_error().decode()
+
+
+def test_stream_entry_points_validate_before_device_work():
+    lib = L.lib()
+    assert lib.sem_stream_create(None) == -1
+    assert "bad argument" in lib.sem_last_error().decode()
+    assert lib.sem_stream_destroy(None) == 0

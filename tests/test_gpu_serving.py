@@ -146,6 +146,34 @@ def test_two_step_rounds_give_the_same_bits(history):
             np.testing.assert_array_equal(a.seis, b.seis)
 
 
+def test_servers_closed_one_after_another_give_the_same_bits():
+    """Servers created and closed one after another in one process (each
+    lane on a new stream, destroyed by close()) give the same bits; closing
+    twice is harmless, and a server deleted without close() keeps its
+    streams without disturbing the next."""
+    import gc
+
+    from mun_amd.serving import SEMServer
+    reqs = make_requests(3, seed=17)
+    kw = dict(segment=60, capacity={8: 2, 12: 2, 20: 2}, use_graph=True, rec_pitch=32, nt_max=420)
+    first = None
+    for k, loop in enumerate(("device", "host", "device", "host", "device")):
+        print(f"[serve-close] server {k} ({loop})", flush=True)
+        srv = SEMServer(hbm_budget_bytes=4e9, loop=loop, **kw)
+        res = srv.run(reqs)
+        if k < 4:
+            srv.close()
+            srv.close()
+        del srv
+        gc.collect()
+        if first is None:
+            first = res
+        for r in reqs:
+            assert res[r.rid].loglik == first[r.rid].loglik
+            np.testing.assert_array_equal(res[r.rid].grad, first[r.rid].grad)
+            np.testing.assert_array_equal(res[r.rid].seis, first[r.rid].seis)
+
+
 @pytest.mark.parametrize("loop", ["device", "host"])
 def test_receiver_tables_growing_mid_stream_give_the_same_bits(loop):
     """Lanes that start with room for one receiver set grow their tables

@@ -254,13 +254,13 @@ struct sem_plan {
     std::vector<int> set_size;
     DevBuf d_recs, d_rec_ptr;
     bool rec_dirty = true;
-    size_t rec_up_entries = 0, rec_up_ptrs = 0;  // prefix already on the device
+    size_t rec_up_entries = 0, rec_up_ptrs = 0, rec_up_sets = 0;  // prefix already on the device
     // two-step tiling (step2_kernel, tile shape `tile2` of TILE2_EX/TILE2_EZ)
     // and its receiver table
     int tile2 = 0, tx2 = TILE2_EX[0] - 3, tz2 = TILE2_EZ[0] - 3;
     int ntx2 = 0, ntz2 = 0, ntiles2 = 0;
     DevBuf d_recs2, d_rec_ptr2;
-    size_t rec2_up_entries = 0, rec2_up_ptrs = 0;
+    size_t rec2_up_entries = 0, rec2_up_ptrs = 0, rec2_up_sets = 0;
     int fusion = 2;  // time steps per step-kernel launch where the batch path allows two (sem_plan_set_step_fusion)
     int corr_max = 64;  // adjoint steps per gradient-correlation chunk, at most (sem_plan_set_corr_chunk)
     std::map<std::pair<int, int>, BsplineTables*> bsp;
@@ -303,10 +303,11 @@ int upload_tables(sem_plan* pl) {
 // owned elements whose element block reaches lo elements to the left / below
 // and hi to the right / above; `owned` marks the tile that owns the element.
 template <typename T>
-void build_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int ntz, int lo, int hi,
-                     std::vector<RecEntry<T, 4>>& flat, std::vector<int>& ptr) {
+void build_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int ntz, int lo, int hi, size_t first_set,
+                     size_t base, std::vector<RecEntry<T, 4>>& flat, std::vector<int>& ptr) {
     const int ntiles = ntx * ntz;
-    for (auto& s : pl->sets) {
+    for (size_t si = first_set; si < pl->sets.size(); ++si) {
+        const auto& s = pl->sets[si];
         std::vector<std::vector<const RecHost*>> by_tile(ntiles);
         for (auto& e : s) {
             for (int tz = std::max(0, e.ez / tz_own - 2); tz <= std::min(ntz - 1, e.ez / tz_own + 2); ++tz)
@@ -317,7 +318,7 @@ void build_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int nt
                 }
         }
         for (int t = 0; t < ntiles; ++t) {
-            ptr.push_back((int)flat.size());
+            ptr.push_back((int)(base + flat.size()));
             const int tx = t % ntx, tz = t / ntx;
             for (const RecHost* e : by_tile[t]) {
                 RecEntry<T, 4> r;
@@ -332,46 +333,51 @@ void build_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int nt
                 flat.push_back(r);
             }
         }
-        ptr.push_back((int)flat.size());
+        ptr.push_back((int)(base + flat.size()));
     }
 }
 
-// Sets are only ever appended, so a table's device prefix stays valid: upload
-// the new suffix in place (kernels in flight keep reading unchanged bytes).
-// Only growth past the reserved capacity reallocates.
+// Sets are only ever appended, so a table's device prefix stays valid: build
+// and upload only the sets added since the last upload, in place after it
+// (kernels in flight keep reading unchanged bytes), so that adding a set
+// costs its own entries however many came before.  Growth past the reserved
+// capacity reallocates, and then every set is built and uploaded again.
 template <typename T>
-int upload_rec_table(DevBuf& d_recs, DevBuf& d_ptr, size_t& up_entries, size_t& up_ptrs,
-                     const std::vector<RecEntry<T, 4>>& flat, const std::vector<int>& ptr) {
+int update_rec_table(const sem_plan* pl, int tx_own, int tz_own, int ntx, int ntz, int lo, int hi, DevBuf& d_recs,
+                     DevBuf& d_ptr, size_t& up_entries, size_t& up_ptrs, size_t& up_sets) {
     const size_t eb = sizeof(RecEntry<T, 4>);
+    std::vector<RecEntry<T, 4>> flat;
+    std::vector<int> ptr;
+    build_rec_table<T>(pl, tx_own, tz_own, ntx, ntz, lo, hi, up_sets, up_entries, flat, ptr);
     int rc;
     bool m1 = false, m2 = false;
-    if ((rc = d_recs.reserve(std::max<size_t>(flat.size() * eb, 16), m1))) return rc;
-    if ((rc = d_ptr.reserve(ptr.size() * sizeof(int), m2))) return rc;
-    if (m1 || m2) up_entries = up_ptrs = 0;
-    if ((rc = d_recs.write_at(up_entries * eb, flat.data() + up_entries, (flat.size() - up_entries) * eb)))
-        return rc;
-    if ((rc = d_ptr.write_at(up_ptrs * sizeof(int), ptr.data() + up_ptrs, (ptr.size() - up_ptrs) * sizeof(int))))
-        return rc;
-    up_entries = flat.size();
-    up_ptrs = ptr.size();
+    if ((rc = d_recs.reserve(std::max<size_t>((up_entries + flat.size()) * eb, 16), m1))) return rc;
+    if ((rc = d_ptr.reserve((up_ptrs + ptr.size()) * sizeof(int), m2))) return rc;
+    if (m1 || m2) {
+        // the old buffers held the prefix
+        up_entries = up_ptrs = up_sets = 0;
+        flat.clear();
+        ptr.clear();
+        build_rec_table<T>(pl, tx_own, tz_own, ntx, ntz, lo, hi, 0, 0, flat, ptr);
+    }
+    if ((rc = d_recs.write_at(up_entries * eb, flat.data(), flat.size() * eb))) return rc;
+    if ((rc = d_ptr.write_at(up_ptrs * sizeof(int), ptr.data(), ptr.size() * sizeof(int)))) return rc;
+    up_entries += flat.size();
+    up_ptrs += ptr.size();
+    up_sets = pl->sets.size();
     return SEM_OK;
 }
 
 template <typename T>
 int upload_receivers(sem_plan* pl) {
     // single-step tiles: one halo column / row on the left and below
-    std::vector<RecEntry<T, 4>> flat;
-    std::vector<int> ptr;
-    build_rec_table<T>(pl, TX, pl->tz, pl->ntx, pl->ntz, 1, 0, flat, ptr);
-    int rc;
-    if ((rc = upload_rec_table<T>(pl->d_recs, pl->d_rec_ptr, pl->rec_up_entries, pl->rec_up_ptrs, flat, ptr)))
-        return rc;
+    int rc = update_rec_table<T>(pl, TX, pl->tz, pl->ntx, pl->ntz, 1, 0, pl->d_recs, pl->d_rec_ptr,
+                                 pl->rec_up_entries, pl->rec_up_ptrs, pl->rec_up_sets);
+    if (rc) return rc;
     // two-step tiles: two halo columns / rows on the left and below, one on the right and above
-    std::vector<RecEntry<T, 4>> flat2;
-    std::vector<int> ptr2;
-    build_rec_table<T>(pl, pl->tx2, pl->tz2, pl->ntx2, pl->ntz2, 2, 1, flat2, ptr2);
-    if ((rc = upload_rec_table<T>(pl->d_recs2, pl->d_rec_ptr2, pl->rec2_up_entries, pl->rec2_up_ptrs, flat2, ptr2)))
-        return rc;
+    rc = update_rec_table<T>(pl, pl->tx2, pl->tz2, pl->ntx2, pl->ntz2, 2, 1, pl->d_recs2, pl->d_rec_ptr2,
+                             pl->rec2_up_entries, pl->rec2_up_ptrs, pl->rec2_up_sets);
+    if (rc) return rc;
     pl->rec_dirty = false;
     return SEM_OK;
 }
@@ -1402,7 +1408,7 @@ int sem_plan_set_tile2(sem_plan* pl, int shape) try {
     pl->ntz2 = (pl->d.nelem_z + pl->tz2 - 1) / pl->tz2;
     pl->ntiles2 = pl->ntx2 * pl->ntz2;
     // the two-step receiver table is rebuilt for the new tiles on the next use
-    pl->rec2_up_entries = pl->rec2_up_ptrs = 0;
+    pl->rec2_up_entries = pl->rec2_up_ptrs = pl->rec2_up_sets = 0;
     pl->rec_dirty = true;
     return SEM_OK;
 } ABI_CATCH
@@ -1864,11 +1870,11 @@ int sem_receivers_reserve(sem_plan* pl, long max_entries, int max_sets) try {
     if (!rc) rc = pl->d_rec_ptr2.reserve((size_t)max_sets * (pl->ntiles2 + 1) * sizeof(int), m4);
     if (rc) return rc;
     if (m1 || m2) {
-        pl->rec_up_entries = pl->rec_up_ptrs = 0;
+        pl->rec_up_entries = pl->rec_up_ptrs = pl->rec_up_sets = 0;
         pl->rec_dirty = true;
     }
     if (m3 || m4) {
-        pl->rec2_up_entries = pl->rec2_up_ptrs = 0;
+        pl->rec2_up_entries = pl->rec2_up_ptrs = pl->rec2_up_sets = 0;
         pl->rec_dirty = true;
     }
     return prepare_receivers(pl);

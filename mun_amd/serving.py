@@ -333,6 +333,7 @@ class Lane:
         self.GE = gelems.value  # doubles in one solve's gradient accumulator
         # receiver tables: one set per admitted request; each receiver is listed in <= 4 tiles
         self.max_sets = server.max_sets
+        self.rec_set_ids = {}  # (count, positions as bytes) -> receiver set id
         self._reserve()
         self.stream, self._stream_h = owned_stream(server.device)
         self.table_host = (L.SemSlot * capacity)()
@@ -365,6 +366,20 @@ class Lane:
 
     def _reserve(self):
         L.check(self.plan.lib.sem_receivers_reserve(self.plan.h, 4 * self.rec_pitch * self.max_sets, self.max_sets))
+
+    def receiver_set(self, rec_xz) -> int:
+        """The lane's receiver set for these positions, added on first use:
+        requests with the same receiver positions share one, so a server that
+        keeps serving requests of known layouts stops adding sets (a plan
+        never drops one)."""
+        xz = np.ascontiguousarray(np.asarray(rec_xz, dtype=np.float64).reshape(-1, 2))
+        key = (xz.shape[0], xz.tobytes())
+        sid = self.rec_set_ids.get(key)
+        if sid is None:
+            self.ensure_set_capacity()
+            sid = self.plan.add_receivers(xz)
+            self.rec_set_ids[key] = sid
+        return sid
 
     def ensure_set_capacity(self):
         """Grow the receiver table before it would move under a captured graph."""
@@ -767,8 +782,7 @@ class SEMServer:
             L.check(plan.lib.sem_pad_nodal(plan.h, C.c_void_p(c2.data_ptr()), C.c_void_p(b["c2p"].data_ptr()), 1,
                                            C.c_void_p(lane.stream.cuda_stream)))
             b["c2_flat"] = c2
-        lane.ensure_set_capacity()
-        rec_set = plan.add_receivers(req.rec_xz)
+        rec_set = lane.receiver_set(req.rec_xz)
         src_node = plan.source_node(*req.src_xz)
         L.check(plan.lib.sem_receivers_sync(plan.h))
         if device_loop:

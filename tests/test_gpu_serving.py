@@ -302,8 +302,11 @@ def test_device_loop_gives_the_host_loops_bits(two_step):
     big = [r for r in reqs if r.nelem_x == 20]
     assert len(big) >= 2
     srv.use_graph = False
+    sets = {k: len(l.plan.rec_sizes) for k, l in srv.lanes.items()}
     print(f"[serve-loop] two_step={two_step}: device loop again, warm, direct launches", flush=True)
     again = srv.run(reqs)
+    # the same receiver layouts reuse the lanes' sets
+    assert {k: len(l.plan.rec_sizes) for k, l in srv.lanes.items()} == sets
     for r in reqs:
         for other in (dev, again):
             a, b = host[r.rid], other[r.rid]

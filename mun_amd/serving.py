@@ -66,6 +66,14 @@ from .solver import Plan, source_wavelet
 ROUNDS_AHEAD = 2  # rounds a lane keeps enqueued ahead of the host
 
 
+def staged(x, dtype, device) -> torch.Tensor:
+    """x on the device, copied from pinned host memory without blocking the
+    host: a copy from pageable memory would wait for the work already queued
+    on the stream (the lane's rounds in flight) before it returns."""
+    host = torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).pin_memory()
+    return host.to(device, non_blocking=True)
+
+
 def owned_stream(device):
     """A new stream of the caller's own (sem_stream_create), wrapped for
     torch, and its handle for release_stream.  Lanes take these rather than
@@ -773,11 +781,11 @@ class SEMServer:
                 # it, so only u^0 = 0 needs clearing
                 b["hist"][0].zero_()
             if req.obs is not None:
-                b["obs"][:, :nrec] = torch.as_tensor(req.obs, dtype=torch.float64, device=dev)
+                b["obs"][:, :nrec] = staged(req.obs, torch.float64, dev)
             w = source_wavelet(req.nt, req.dt, req.f0)
-            b["wav"].copy_(torch.as_tensor(w, dtype=torch.float32, device=dev))
-            base = None if req.base is None else torch.as_tensor(req.base, device=dev)
-            _, c2 = plan.bspline_map(torch.as_tensor(req.coef, dtype=torch.float64, device=dev)[None], base,
+            b["wav"].copy_(staged(w, torch.float32, dev))
+            base = None if req.base is None else staged(req.base, torch.float64, dev)
+            _, c2 = plan.bspline_map(staged(req.coef, torch.float64, dev)[None], base,
                                      want_c=False)
             L.check(plan.lib.sem_pad_nodal(plan.h, C.c_void_p(c2.data_ptr()), C.c_void_p(b["c2p"].data_ptr()), 1,
                                            C.c_void_p(lane.stream.cuda_stream)))

@@ -1003,7 +1003,12 @@ def run_plain(requests: List[SolveRequest], device=None, adjoint_mode="reference
                               checkpoint_every=segment, device=dev, schedule="auto")
         p = plans[key]
         start = time.perf_counter() - t0
-        rs = p.add_receivers(r.rec_xz)
+        # one receiver set per layout, as the lanes keep them
+        xz = np.ascontiguousarray(np.asarray(r.rec_xz, dtype=np.float64).reshape(-1, 2))
+        ids = plans.setdefault(key + ("receiver sets",), {})
+        rs = ids.get(xz.tobytes())
+        if rs is None:
+            rs = ids[xz.tobytes()] = p.add_receivers(xz)
         _, c2 = p.bspline_map(torch.as_tensor(r.coef, dtype=torch.float64)[None],
                               None if r.base is None else torch.as_tensor(r.base))
         w = torch.as_tensor(source_wavelet(r.nt, r.dt, r.f0), dtype=torch.float32, device=dev)

@@ -1,10 +1,11 @@
 """Continuous-batching server for forward+adjoint SEM solves of uneven sizes.
 
 Requests (mesh size class, step count, velocity B-spline nodes, source,
-receivers, observations) arrive over time.  Each size class is a *lane*: one
-plan, one HIP stream and a device-resident slot table (``sem_slot``, see
-include/sem_capi.h).  A solve occupies one slot from admission to completion
-and walks through rounds of S steps.  When its full forward history fits the
+receivers, observations) arrive over time.  Each size class is served by a
+*lane* (two where one solve already fills the GPU, lanes_for_mesh): one
+plan, one HIP stream of its own and a device-resident slot table
+(``sem_slot``, see include/sem_capi.h).  A solve occupies one slot from
+admission to completion and walks through rounds of S steps.  When its full forward history fits the
 HBM budget at admission it keeps it (no recompute):
 
     FORWARD_HIST seg 0 .. K-1  ->  (final sample, residual, misfit)
@@ -17,18 +18,20 @@ otherwise it keeps checkpoints of (u, w) every S steps:
     -> RECOMPUTE seg K-1 -> ADJOINT seg K-1 -> ... -> RECOMPUTE seg 0 -> ADJOINT seg 0
     -> (gradient, projection) -> done
 
-Every round advances *all* occupied slots of a lane together: per step one
-forward-kernel launch covers the slots in FORWARD/RECOMPUTE and one
-adjoint-kernel launch the slots in ADJOINT, so solves at different phases and
-time steps share launches.  Requests that arrive while others run are admitted
+Every round advances *all* occupied slots of a lane together, two steps per
+launch by default: the launches of a step pair cover the slots of every
+phase (forward, recompute and adjoint slots in one launch in the device
+loop), so solves at different phases and time steps share launches.  Requests that arrive while others run are admitted
 at the next round boundary of their lane as soon as a slot is free and their
 HBM footprint (state + checkpoints + segment buffer + seismograms) fits the
 budget; lanes run on separate streams, so different size classes overlap on
 the device.  The budget is one device allocation made when the server starts
 (its arena), and each admitted request takes one block of it, so serving
-allocates no device memory per request.  A round can be replayed from a captured HIP graph (its launch
-arguments only point at the lane's device slot table, which is rewritten
-between rounds).
+allocates no device memory per request; its inputs reach the device through
+pinned staging, and requests with the same receiver positions share the
+lane's receiver set.  A round can be replayed from a captured HIP graph (its
+launch arguments only point at the lane's device slot table, which is
+rewritten between rounds).  close() destroys the lanes' streams.
 
 Two engines drive the rounds.  With ``loop="device"`` (the default) the
 request queue, the slot states and the round's slot table live on the device
